@@ -1,0 +1,147 @@
+/*
+ * C ABI of the host library libsol_host.so (C++20 inside).
+ *
+ * The host library keeps sol's scene model (Scene, TriangleMesh with its BVH builder, Bsdf,
+ * Light, Camera) and turns it into flat arrays:
+ *   - the raw mesh (sol_mesh_view), the form sol's TriangleMesh constructor takes
+ *     (reference: include/sol/triangle_mesh.h:18-24);
+ *   - the binary BVH in bvh v2 layout (sol_bvh_view), the result of
+ *     TriangleMesh::build_bvh (reference: src/triangle_mesh.cpp:119-146);
+ *   - the GPU scene descriptor (solhip_scene_desc) that solhip_scene_upload copies to HBM.
+ * It also generates the procedural scenes and ray batches the tests and benchmarks use.
+ *
+ * Conventions: every function returns 0 on success and a negative SOL_E_* code on failure;
+ * sol_host_last_error() returns a thread-local message for the last failure. No C++
+ * exception crosses this boundary.
+ */
+#ifndef SOL_HOST_H
+#define SOL_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "sol_types.h"
+#include "solhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SOL_OK            = 0,
+    SOL_E_INVALID     = -1, /* invalid argument                          */
+    SOL_E_NO_LIGHTS   = -2, /* scene without light (undefined in sol, path_tracer.cpp:27,36) */
+    SOL_E_ALLOC       = -3, /* host allocation failed                    */
+    SOL_E_INTERNAL    = -4
+};
+
+typedef struct sol_scene sol_scene;
+
+/* Options of the CPU BVH builder: sweep SAH (bvh v2 SweepSahBuilder semantics) followed by
+   the reinsertion optimiser (bvh v2 SequentialReinsertionOptimizer semantics). */
+typedef struct sol_bvh_config {
+    uint32_t min_leaf_size;      /* ranges of this size or smaller always become leaves   */
+    uint32_t max_leaf_size;      /* ranges larger than this are always split (<= 8)      */
+    float traversal_cost;        /* SAH cost of one inner-node visit                      */
+    float intersection_cost;     /* SAH cost of one triangle test                         */
+    uint32_t reinsertion_iters;  /* passes of the reinsertion optimiser (0 = off)         */
+    float reinsertion_batch;     /* fraction of nodes, largest area first, moved per pass */
+    uint32_t top_levels;         /* BVH levels laid out first (BFS) for staging in LDS    */
+} sol_bvh_config;
+
+/* The raw mesh and scene tables, in original triangle order. Pointers stay owned by the
+   sol_scene they come from. */
+typedef struct sol_mesh_view {
+    uint32_t num_vertices;
+    const float* positions;     /* 3 floats per vertex */
+    const float* normals;       /* 3 floats per vertex */
+    uint32_t num_tris;
+    const uint32_t* indices;    /* 3 vertex indices per triangle */
+    const int32_t* tri_material;/* material per triangle, -1 = no BSDF */
+    const int32_t* tri_light;   /* light per triangle, -1 = not emissive */
+    uint32_t num_materials;
+    const sol_material* materials;
+    uint32_t num_lights;
+    const sol_light* lights;
+    sol_camera_params camera;
+} sol_mesh_view;
+
+typedef struct sol_bvh_view {
+    uint32_t num_nodes;
+    const sol_bvh_node* nodes;
+    uint32_t num_prims;
+    const uint32_t* prim_indices;
+} sol_bvh_view;
+
+typedef struct sol_build_stats {
+    double build_seconds;        /* sweep SAH                        */
+    double optimize_seconds;     /* reinsertion                      */
+    double flatten_seconds;      /* GPU layout                       */
+    float sah_cost_built;        /* SAH cost after the sweep         */
+    float sah_cost_final;        /* SAH cost after reinsertion       */
+    uint32_t depth;              /* longest root-to-leaf path, nodes */
+    uint32_t num_nodes;
+    uint32_t num_leaves;
+} sol_build_stats;
+
+/* Procedural scene parameters for the sphere field (400 icospheres on a ground quad, three
+   quad lights and one point light). */
+typedef struct sol_sphere_field_params {
+    uint32_t num_spheres;   /* 400                                   */
+    uint32_t subdivision;   /* icosphere level: 20 * 4^level triangles */
+    uint64_t seed;          /* PCG seed of the generator             */
+    uint32_t num_groups;    /* material groups; 0 = one per sphere   */
+    float aspect;           /* camera aspect ratio (width / height)  */
+    float extent;           /* sphere centres in [-extent, extent]^2 */
+    float radius_min;       /* log-uniform radius range              */
+    float radius_max;
+} sol_sphere_field_params;
+
+const char* sol_host_last_error(void);
+
+void sol_default_bvh_config(sol_bvh_config* out);
+void sol_default_pt_config(sol_pt_config* out);
+void sol_default_sphere_field_params(sol_sphere_field_params* out);
+
+/* Procedural Cornell box: five walls, a tall and a short box, one ceiling quad light with
+   emission (17, 12, 4); camera of cornell_box.toml:3-8 with the given aspect ratio. */
+int sol_scene_cornell_box(float aspect, const sol_bvh_config* cfg, sol_scene** out);
+
+/* Sphere field. */
+int sol_scene_sphere_field(const sol_sphere_field_params* p, const sol_bvh_config* cfg,
+                           sol_scene** out);
+
+/* Random triangle soup: centres uniform in [0,1]^3, vertices offset by U[-0.01,0.01]^3.
+   Every triangle is diffuse; one point light so that the scene is renderable. */
+int sol_scene_triangle_soup(uint32_t num_tris, uint64_t seed, const sol_bvh_config* cfg,
+                            sol_scene** out);
+
+/* Scene from caller arrays (copied). */
+int sol_scene_from_arrays(const sol_mesh_view* mesh, const sol_bvh_config* cfg,
+                          sol_scene** out);
+
+void sol_scene_destroy(sol_scene* scene);
+
+int sol_scene_mesh(const sol_scene* scene, sol_mesh_view* out);
+int sol_scene_bvh(const sol_scene* scene, sol_bvh_view* out);
+int sol_scene_gpu_desc(const sol_scene* scene, solhip_scene_desc* out);
+int sol_scene_build_stats(const sol_scene* scene, sol_build_stats* out);
+
+/* Replaces the material / light tables (geometry and BVH unchanged). */
+int sol_scene_set_materials(sol_scene* scene, const sol_material* m, uint32_t n);
+int sol_scene_set_lights(sol_scene* scene, const sol_light* l, uint32_t n);
+
+/* Computes the camera basis from its parameters (cameras.cpp:5-21). */
+void sol_camera_from_params(const sol_camera_params* p, sol_camera* out);
+
+/* Ray batches of the intersection benchmark.
+   kind 0: origins U[-0.5,1.5]^3, unit directions uniform on the sphere, tmin 0, tmax +inf.
+   kind 1: the same origins, towards targets U[0,1]^3 with between_points conventions:
+           dir = target - org (not normalised), tmin = offset, tmax = 1 - offset. */
+int sol_gen_rays(uint32_t kind, uint64_t num_rays, uint64_t seed, float offset, sol_ray* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,179 @@
+/*
+ * C ABI of the GPU library libsolhip.so: hand-written HIP kernels for gfx950 behind plain
+ * pointers and sizes. This is the drop-in boundary of the path-tracing hot path.
+ *
+ * Reference interfaces replaced (paths relative to the sol sources):
+ *   solhip_render        <- sol::Renderer::render(Image&, sample_index, sample_count)
+ *                           (include/sol/renderer.h:34) as implemented by PathTracer::render
+ *                           (src/algorithms/path_tracer.cpp:9-23): adds, per pixel, the sum of
+ *                           the samples [sample_index, sample_index + sample_count) into a
+ *                           3-channel planar fp32 image (include/sol/image.h:54-60).
+ *   solhip_intersect     <- sol::Geometry::intersect_closest / intersect_any
+ *                           (include/sol/geometry.h:33-43; src/triangle_mesh.cpp:64-117),
+ *                           batched.
+ *   solhip_scene_upload  <- the TriangleMesh / Scene state the renderer reads
+ *                           (include/sol/scene.h:34-57, include/sol/triangle_mesh.h:18-60).
+ *   solhip_render_batch, solhip_backward, solhip_autotune: no counterpart in sol (many scenes
+ *                           in one launch; adjoint path replay; launch-parameter search).
+ *
+ * Conventions:
+ *   - return 0 on success, a negative SOLHIP_E_* code on failure; solhip_last_error() gives
+ *     a thread-local message. Nothing throws across the ABI.
+ *   - one HIP stream per context; calls on one context must be serialised by the caller (as a
+ *     sol::RenderJob is controlled from one thread, include/sol/render_job.h:15-17);
+ *     distinct contexts are independent.
+ *   - every call returns after its GPU work has finished.
+ *   - a "device" pointer argument is a HIP device address on the context's device; a "host"
+ *     pointer is ordinary host memory. Each entry point says which.
+ */
+#ifndef SOLHIP_H
+#define SOLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "sol_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SOLHIP_OK          = 0,
+    SOLHIP_E_INVALID   = -1,  /* invalid argument (null, zero size, bad enum)          */
+    SOLHIP_E_NO_LIGHTS = -2,  /* scene has no light                                    */
+    SOLHIP_E_HIP       = -3,  /* a HIP runtime call failed                             */
+    SOLHIP_E_NO_DEVICE = -4,  /* no HIP device / device index out of range             */
+    SOLHIP_E_TOO_DEEP  = -5,  /* BVH deeper than the traversal stack                   */
+    SOLHIP_E_CONTEXT   = -6   /* objects from different contexts mixed                 */
+};
+
+enum {
+    SOLHIP_RENDERER_MEGAKERNEL = 0, /* one thread per pixel runs whole paths (plain GPU)   */
+    SOLHIP_RENDERER_WAVEFRONT  = 1  /* ray queues, material-sorted shading, graph loop    */
+};
+
+typedef struct solhip_ctx solhip_ctx;
+typedef struct solhip_scene solhip_scene;
+
+/* Flat scene, produced by the host library (sol_scene_gpu_desc). All pointers are host
+   pointers; solhip_scene_upload copies them, the caller keeps ownership. */
+typedef struct solhip_scene_desc {
+    uint32_t num_nodes;
+    const sol_gpu_node* nodes;      /* node 0 is the root                         */
+    uint32_t num_tris;
+    const sol_gpu_tri* tris;        /* leaf order                                 */
+    const sol_gpu_shade* shade;     /* leaf order                                 */
+    uint32_t num_materials;
+    const sol_material* materials;
+    uint32_t num_lights;
+    const sol_light* lights;
+    sol_camera camera;
+    uint32_t bvh_depth;             /* longest root-to-leaf path (nodes)          */
+    uint32_t top_nodes;             /* nodes [0, top_nodes) are the top levels    */
+} solhip_scene_desc;
+
+/* Launch parameters the autotuner chooses. Zero fields mean "library default". */
+typedef struct solhip_tuning {
+    uint32_t block_size;    /* threads per workgroup of the traversal kernels: 64, 128, 256 */
+    uint32_t lds_stack;     /* short-stack entries per lane kept in LDS: 8, 16, 32           */
+    uint32_t path_slots;    /* wavefront: paths in flight                                    */
+    uint32_t tile_w;        /* batch: tile shape of the global work queue                    */
+    uint32_t tile_h;
+    uint32_t graph_iters;   /* bounce-loop iterations per captured graph                     */
+} solhip_tuning;
+
+/* Counters accumulated by the context since the last solhip_reset_stats. */
+typedef struct solhip_stats {
+    uint64_t extension_rays;   /* closest-hit queries                          */
+    uint64_t shadow_rays;      /* any-hit queries                              */
+    uint64_t samples;          /* paths started                                */
+    uint64_t launches;         /* kernel launches (graph nodes count once each) */
+    uint64_t graph_replays;
+    double extend_ms;          /* summed durations measured with HIP events (profiling on) */
+    double shade_ms;
+    double shadow_ms;
+    double regen_ms;
+    double other_ms;
+    uint64_t extend_launches;  /* launches covered by extend_ms                */
+    double render_ms;          /* wall time of render calls (HIP events)       */
+} solhip_stats;
+
+typedef struct solhip_device_info {
+    char name[64];
+    char arch[32];
+    int32_t compute_units;
+    int32_t wavefront_size;
+    uint64_t total_memory;
+    int32_t lds_per_block;
+} solhip_device_info;
+
+/* One image of a batched render. accum is a DEVICE pointer to 3 * width * height floats
+   (planar R, G, B), to which the samples are added. */
+typedef struct solhip_batch_job {
+    solhip_scene* scene;
+    uint32_t width;
+    uint32_t height;
+    uint64_t sample_index;
+    uint32_t sample_count;
+    uint32_t pad;
+    float* accum;
+} solhip_batch_job;
+
+const char* solhip_last_error(void);
+
+int solhip_create(int device, solhip_ctx** out);
+void solhip_destroy(solhip_ctx* ctx);
+int solhip_device_query(solhip_ctx* ctx, solhip_device_info* out);
+
+int solhip_set_tuning(solhip_ctx* ctx, const solhip_tuning* t);
+int solhip_get_tuning(solhip_ctx* ctx, solhip_tuning* out);
+/* profiling != 0: record HIP events around each extend/shade/shadow launch into the stats. */
+int solhip_set_profiling(solhip_ctx* ctx, int profiling);
+int solhip_get_stats(solhip_ctx* ctx, solhip_stats* out);
+int solhip_reset_stats(solhip_ctx* ctx);
+
+int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* desc, solhip_scene** out);
+void solhip_scene_destroy(solhip_scene* scene);
+/* Replace the material or light table (host arrays, same counts as uploaded). */
+int solhip_scene_set_materials(solhip_scene* scene, const sol_material* m, uint32_t n);
+int solhip_scene_set_lights(solhip_scene* scene, const sol_light* l, uint32_t n);
+
+/* Batched intersection (Geometry::intersect_closest / intersect_any).
+   rays: n sol_ray; any_hit == 0 -> out is n sol_hit (prim = -1 on a miss);
+   any_hit != 0 -> out is n int32 (1 = occluded). device_ptrs != 0: rays/out are device
+   pointers, else host pointers. Ties in t resolve to the smaller original triangle index. */
+int solhip_intersect(solhip_scene* scene, const sol_ray* rays, uint64_t n, int any_hit,
+                     void* out, int device_ptrs);
+
+/* Renderer::render: accum += sum of samples [sample_index, sample_index + sample_count) per
+   pixel; accum is 3 * width * height floats, planar RGB, pixel (0,0) top-left.
+   device_ptr != 0: accum is a device pointer, else a host pointer. */
+int solhip_render(solhip_scene* scene, const sol_pt_config* cfg, int renderer,
+                  uint32_t width, uint32_t height, uint64_t sample_index,
+                  uint32_t sample_count, float* accum, int device_ptr);
+
+/* Many scenes and image tiles in one wavefront run fed by a global work queue. All scenes
+   must belong to ctx. Accumulation semantics per job as solhip_render. */
+int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t num_jobs,
+                        const sol_pt_config* cfg);
+
+/* Adjoint path replay. d_image: 3 * width * height floats (planar RGB), dLoss/d(accum).
+   Adds gradients into d_materials (num_materials x 4: d kd.r, d kd.g, d kd.b, d log ns) and
+   d_lights (num_lights x 3: d intensity r, g, b). Sampling decisions, MIS weights and Russian
+   roulette are treated as constants (detached), pdfs use ns_sample.
+   device_ptrs != 0: all three arrays are device pointers, else host pointers. */
+int solhip_backward(solhip_scene* scene, const sol_pt_config* cfg, uint32_t width,
+                    uint32_t height, uint64_t sample_index, uint32_t sample_count,
+                    const float* d_image, float* d_materials, float* d_lights, int device_ptrs);
+
+/* Times candidate launch parameters on the given scene and image and keeps the fastest in the
+   context (also written to *out if not null). budget_samples bounds the work per trial. */
+int solhip_autotune(solhip_scene* scene, const sol_pt_config* cfg, uint32_t width,
+                    uint32_t height, uint32_t budget_spp, solhip_tuning* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,217 @@
+"""ctypes mirror of the C ABI (include/sol_types.h, include/sol_host.h, include/solhip.h).
+
+The two product libraries are loaded from this package directory. There is no fallback: if
+libsolhip.so or libsol_host.so is missing, importing this module raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+BSDF_NONE, BSDF_DIFFUSE, BSDF_PHONG, BSDF_GLASS, BSDF_MIRROR = -1, 0, 1, 2, 3
+LIGHT_TRIANGLE, LIGHT_POINT = 0, 1
+PT_DISABLE_MIS, PT_DISABLE_NEE, PT_DISABLE_RR = 1, 2, 4
+RENDERER_MEGAKERNEL, RENDERER_WAVEFRONT = 0, 1
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("kd", C.c_float * 3), ("ks", C.c_float * 3), ("kt", C.c_float * 3),
+                ("ns", C.c_float), ("eta", C.c_float), ("ns_sample", C.c_float), ("group", C.c_int32),
+                ("pad", C.c_float * 2)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_int32), ("v0", C.c_float * 3), ("v1", C.c_float * 3), ("v2", C.c_float * 3),
+                ("normal", C.c_float * 3), ("inv_area", C.c_float), ("intensity", C.c_float * 3),
+                ("group", C.c_int32), ("pad", C.c_int32 * 2)]
+
+
+class CameraParams(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("dir", C.c_float * 3), ("up", C.c_float * 3), ("fov", C.c_float),
+                ("aspect", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("dir", C.c_float * 3), ("right", C.c_float * 3), ("up", C.c_float * 3)]
+
+
+class PtConfig(C.Structure):
+    _fields_ = [("max_path_len", C.c_uint32), ("min_rr_path_len", C.c_uint32), ("min_survival_prob", C.c_float),
+                ("max_survival_prob", C.c_float), ("ray_offset", C.c_float), ("flags", C.c_uint32)]
+
+
+class Ray(C.Structure):
+    _fields_ = [("org", C.c_float * 3), ("tmin", C.c_float), ("dir", C.c_float * 3), ("tmax", C.c_float)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("prim", C.c_int32), ("t", C.c_float), ("u", C.c_float), ("v", C.c_float)]
+
+
+class BvhNode(C.Structure):
+    _fields_ = [("bounds", C.c_float * 6), ("index", C.c_uint32), ("prim_count", C.c_uint32)]
+
+
+class GpuNode(C.Structure):
+    _fields_ = [("b0", C.c_float * 4), ("b1", C.c_float * 4), ("bz", C.c_float * 4), ("child", C.c_int32 * 2),
+                ("pad", C.c_int32 * 2)]
+
+
+class GpuTri(C.Structure):
+    _fields_ = [("v0", C.c_float * 3), ("prim", C.c_int32), ("v1", C.c_float * 3), ("pad0", C.c_int32),
+                ("v2", C.c_float * 3), ("pad1", C.c_int32)]
+
+
+class GpuShade(C.Structure):
+    _fields_ = [("n0", C.c_float * 3), ("n1", C.c_float * 3), ("n2", C.c_float * 3), ("material", C.c_int32),
+                ("light", C.c_int32), ("pad", C.c_int32)]
+
+
+class BvhConfig(C.Structure):
+    _fields_ = [("min_leaf_size", C.c_uint32), ("max_leaf_size", C.c_uint32), ("traversal_cost", C.c_float),
+                ("intersection_cost", C.c_float), ("reinsertion_iters", C.c_uint32),
+                ("reinsertion_batch", C.c_float), ("top_levels", C.c_uint32)]
+
+
+class MeshView(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("positions", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("num_tris", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
+                ("tri_material", C.POINTER(C.c_int32)), ("tri_light", C.POINTER(C.c_int32)),
+                ("num_materials", C.c_uint32), ("materials", C.POINTER(Material)), ("num_lights", C.c_uint32),
+                ("lights", C.POINTER(Light)), ("camera", CameraParams)]
+
+
+class BvhView(C.Structure):
+    _fields_ = [("num_nodes", C.c_uint32), ("nodes", C.POINTER(BvhNode)), ("num_prims", C.c_uint32),
+                ("prim_indices", C.POINTER(C.c_uint32))]
+
+
+class BuildStats(C.Structure):
+    _fields_ = [("build_seconds", C.c_double), ("optimize_seconds", C.c_double), ("flatten_seconds", C.c_double),
+                ("sah_cost_built", C.c_float), ("sah_cost_final", C.c_float), ("depth", C.c_uint32),
+                ("num_nodes", C.c_uint32), ("num_leaves", C.c_uint32)]
+
+
+class SphereFieldParams(C.Structure):
+    _fields_ = [("num_spheres", C.c_uint32), ("subdivision", C.c_uint32), ("seed", C.c_uint64),
+                ("num_groups", C.c_uint32), ("aspect", C.c_float), ("extent", C.c_float),
+                ("radius_min", C.c_float), ("radius_max", C.c_float)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("num_nodes", C.c_uint32), ("nodes", C.POINTER(GpuNode)), ("num_tris", C.c_uint32),
+                ("tris", C.POINTER(GpuTri)), ("shade", C.POINTER(GpuShade)), ("num_materials", C.c_uint32),
+                ("materials", C.POINTER(Material)), ("num_lights", C.c_uint32), ("lights", C.POINTER(Light)),
+                ("camera", Camera), ("bvh_depth", C.c_uint32), ("top_nodes", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("lds_stack", C.c_uint32), ("path_slots", C.c_uint32),
+                ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("graph_iters", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("samples", C.c_uint64),
+                ("launches", C.c_uint64), ("graph_replays", C.c_uint64), ("extend_ms", C.c_double),
+                ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("regen_ms", C.c_double),
+                ("other_ms", C.c_double), ("extend_launches", C.c_uint64), ("render_ms", C.c_double)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int32),
+                ("wavefront_size", C.c_int32), ("total_memory", C.c_uint64), ("lds_per_block", C.c_int32)]
+
+
+class BatchJob(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("sample_index", C.c_uint64),
+                ("sample_count", C.c_uint32), ("pad", C.c_uint32), ("accum", C.c_void_p)]
+
+
+assert C.sizeof(Material) == 64 and C.sizeof(Light) == 80 and C.sizeof(GpuNode) == 64
+assert C.sizeof(GpuTri) == 48 and C.sizeof(GpuShade) == 48 and C.sizeof(Ray) == 32 and C.sizeof(Hit) == 16
+
+
+def _load(name):
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run __graft_entry__.build() (make -C sol_amd)")
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+hip = _load("libsolhip.so")
+host = _load("libsol_host.so")
+
+_P = C.c_void_p
+_SIGS = {
+    hip: {
+        "solhip_last_error": (C.c_char_p, []),
+        "solhip_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+        "solhip_destroy": (None, [_P]),
+        "solhip_device_query": (C.c_int, [_P, C.POINTER(DeviceInfo)]),
+        "solhip_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+        "solhip_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+        "solhip_set_profiling": (C.c_int, [_P, C.c_int]),
+        "solhip_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+        "solhip_reset_stats": (C.c_int, [_P]),
+        "solhip_scene_upload": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
+        "solhip_scene_destroy": (None, [_P]),
+        "solhip_scene_set_materials": (C.c_int, [_P, C.POINTER(Material), C.c_uint32]),
+        "solhip_scene_set_lights": (C.c_int, [_P, C.POINTER(Light), C.c_uint32]),
+        "solhip_intersect": (C.c_int, [_P, _P, C.c_uint64, C.c_int, _P, C.c_int]),
+        "solhip_render": (C.c_int, [_P, C.POINTER(PtConfig), C.c_int, C.c_uint32, C.c_uint32, C.c_uint64,
+                                    C.c_uint32, _P, C.c_int]),
+        "solhip_render_batch": (C.c_int, [_P, C.POINTER(BatchJob), C.c_uint32, C.POINTER(PtConfig)]),
+        "solhip_backward": (C.c_int, [_P, C.POINTER(PtConfig), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
+                                      _P, _P, _P, C.c_int]),
+        "solhip_autotune": (C.c_int, [_P, C.POINTER(PtConfig), C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.POINTER(Tuning)]),
+    },
+    host: {
+        "sol_host_last_error": (C.c_char_p, []),
+        "sol_default_bvh_config": (None, [C.POINTER(BvhConfig)]),
+        "sol_default_pt_config": (None, [C.POINTER(PtConfig)]),
+        "sol_default_sphere_field_params": (None, [C.POINTER(SphereFieldParams)]),
+        "sol_scene_cornell_box": (C.c_int, [C.c_float, C.POINTER(BvhConfig), C.POINTER(_P)]),
+        "sol_scene_sphere_field": (C.c_int, [C.POINTER(SphereFieldParams), C.POINTER(BvhConfig), C.POINTER(_P)]),
+        "sol_scene_triangle_soup": (C.c_int, [C.c_uint32, C.c_uint64, C.POINTER(BvhConfig), C.POINTER(_P)]),
+        "sol_scene_from_arrays": (C.c_int, [C.POINTER(MeshView), C.POINTER(BvhConfig), C.POINTER(_P)]),
+        "sol_scene_destroy": (None, [_P]),
+        "sol_scene_mesh": (C.c_int, [_P, C.POINTER(MeshView)]),
+        "sol_scene_bvh": (C.c_int, [_P, C.POINTER(BvhView)]),
+        "sol_scene_gpu_desc": (C.c_int, [_P, C.POINTER(SceneDesc)]),
+        "sol_scene_build_stats": (C.c_int, [_P, C.POINTER(BuildStats)]),
+        "sol_scene_set_materials": (C.c_int, [_P, C.POINTER(Material), C.c_uint32]),
+        "sol_scene_set_lights": (C.c_int, [_P, C.POINTER(Light), C.c_uint32]),
+        "sol_camera_from_params": (None, [C.POINTER(CameraParams), C.POINTER(Camera)]),
+        "sol_gen_rays": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, C.c_float, _P]),
+    },
+}
+
+for _lib, _sigs in _SIGS.items():
+    for _name, (_res, _args) in _sigs.items():
+        _fn = getattr(_lib, _name)
+        _fn.restype = _res
+        _fn.argtypes = _args
+
+
+class SolError(RuntimeError):
+    """A sol_* or solhip_* call returned a non-zero status."""
+
+    def __init__(self, func, status, message):
+        super().__init__(f"{func} failed with status {status}: {message}")
+        self.status = status
+
+
+def check_hip(status, func):
+    if status != 0:
+        raise SolError(func, status, hip.solhip_last_error().decode(errors="replace"))
+
+
+def check_host(status, func):
+    if status != 0:
+        raise SolError(func, status, host.sol_host_last_error().decode(errors="replace"))
+
+
+def exported_symbols():
+    """Names every header in include/ declares, per library."""
+    return {"libsolhip.so": sorted(_SIGS[hip]), "libsol_host.so": sorted(_SIGS[host])}

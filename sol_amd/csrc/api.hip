@@ -1,0 +1,367 @@
+// C ABI of libsolhip.so (include/solhip.h): contexts, scene upload, and the entry points that
+// launch the kernels.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "solhip.h"
+#include "launch.h"
+#include "state.h"
+
+using namespace solhip;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e__ = (expr);                                                                 \
+        if (e__ != hipSuccess)                                                                   \
+            return fail(SOLHIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e__));        \
+    } while (0)
+
+bool valid_shape_values(uint32_t block, uint32_t lds) {
+    return (block == 64 || block == 128 || block == 256) && (lds == 8 || lds == 16 || lds == 32);
+}
+
+} // namespace
+
+namespace solhip {
+
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+int Context::ensure_scratch(int threads) {
+    if (threads <= scratch.max_threads && scratch.spill) return SOLHIP_OK;
+    if (scratch.spill) HIP_TRY(hipFree(scratch.spill));
+    scratch.spill = nullptr;
+    size_t bytes = static_cast<size_t>(threads) * kSpillEntries * sizeof(int);
+    HIP_TRY(hipMalloc(&scratch.spill, bytes));
+    scratch.max_threads = threads;
+    scratch.spill_entries = kSpillEntries;
+    return SOLHIP_OK;
+}
+
+LaunchShape Context::shape(int block, int lds) const {
+    LaunchShape s;
+    s.block = block;
+    s.lds_stack = lds;
+    s.grid = persistent_grid(info.compute_units, block);
+    return s;
+}
+
+int Context::check_overflow() {
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, scratch.overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (flag) {
+        int zero = 0;
+        HIP_TRY(hipMemcpy(scratch.overflow, &zero, sizeof(int), hipMemcpyHostToDevice));
+        return fail(SOLHIP_E_TOO_DEEP, "traversal stack overflow");
+    }
+    return SOLHIP_OK;
+}
+
+int Context::read_counters(unsigned long long* host, int n) {
+    HIP_TRY(hipMemcpyAsync(host, counters, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return SOLHIP_OK;
+}
+
+} // namespace solhip
+
+extern "C" {
+
+const char* solhip_last_error(void) { return g_error.c_str(); }
+
+int solhip_create(int device, solhip_ctx** out) {
+    if (!out) return fail(SOLHIP_E_INVALID, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(SOLHIP_E_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(SOLHIP_E_NO_DEVICE, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    auto ctx = std::make_unique<solhip_ctx>();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    std::snprintf(ctx->info.name, sizeof(ctx->info.name), "%s", prop.name);
+    std::snprintf(ctx->info.arch, sizeof(ctx->info.arch), "%s", prop.gcnArchName);
+    ctx->info.compute_units = prop.multiProcessorCount;
+    ctx->info.wavefront_size = prop.warpSize;
+    ctx->info.total_memory = prop.totalGlobalMem;
+    ctx->info.lds_per_block = static_cast<int32_t>(prop.sharedMemPerBlock);
+    if (prop.warpSize != 64) return fail(SOLHIP_E_NO_DEVICE, "wave64 device required");
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&ctx->counters, sizeof(unsigned long long) * kNumCounters));
+    HIP_TRY(hipMalloc(&ctx->scratch.overflow, sizeof(int)));
+    HIP_TRY(hipMemset(ctx->scratch.overflow, 0, sizeof(int)));
+    HIP_TRY(hipEventCreate(&ctx->ev_begin));
+    HIP_TRY(hipEventCreate(&ctx->ev_end));
+    ctx->tuning = default_tuning();
+    int st = ctx->ensure_scratch(persistent_grid(ctx->info.compute_units, 64) * 64);
+    if (st) return st;
+    *out = ctx.release();
+    return SOLHIP_OK;
+}
+
+void solhip_destroy(solhip_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->release_wavefront();
+    if (ctx->scratch.spill) (void)hipFree(ctx->scratch.spill);
+    if (ctx->scratch.overflow) (void)hipFree(ctx->scratch.overflow);
+    if (ctx->counters) (void)hipFree(ctx->counters);
+    (void)hipEventDestroy(ctx->ev_begin);
+    (void)hipEventDestroy(ctx->ev_end);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int solhip_device_query(solhip_ctx* ctx, solhip_device_info* out) {
+    if (!ctx || !out) return fail(SOLHIP_E_INVALID, "null argument");
+    *out = ctx->info;
+    return SOLHIP_OK;
+}
+
+int solhip_set_tuning(solhip_ctx* ctx, const solhip_tuning* t) {
+    if (!ctx || !t) return fail(SOLHIP_E_INVALID, "null argument");
+    solhip_tuning d = default_tuning();
+    solhip_tuning n = *t;
+    if (!n.block_size) n.block_size = d.block_size;
+    if (!n.lds_stack) n.lds_stack = d.lds_stack;
+    if (!n.path_slots) n.path_slots = d.path_slots;
+    if (!n.tile_w) n.tile_w = d.tile_w;
+    if (!n.tile_h) n.tile_h = d.tile_h;
+    if (!n.graph_iters) n.graph_iters = d.graph_iters;
+    if (!valid_shape_values(n.block_size, n.lds_stack)) return fail(SOLHIP_E_INVALID, "unsupported block size or LDS stack");
+    if (n.path_slots < 4096 || n.path_slots > (1u << 26)) return fail(SOLHIP_E_INVALID, "path_slots out of range");
+    if (n.graph_iters > 256) return fail(SOLHIP_E_INVALID, "graph_iters out of range");
+    ctx->tuning = n;
+    return SOLHIP_OK;
+}
+
+int solhip_get_tuning(solhip_ctx* ctx, solhip_tuning* out) {
+    if (!ctx || !out) return fail(SOLHIP_E_INVALID, "null argument");
+    *out = ctx->tuning;
+    return SOLHIP_OK;
+}
+
+int solhip_set_profiling(solhip_ctx* ctx, int profiling) {
+    if (!ctx) return fail(SOLHIP_E_INVALID, "null context");
+    ctx->profiling = profiling != 0;
+    return SOLHIP_OK;
+}
+
+int solhip_get_stats(solhip_ctx* ctx, solhip_stats* out) {
+    if (!ctx || !out) return fail(SOLHIP_E_INVALID, "null argument");
+    *out = ctx->stats;
+    return SOLHIP_OK;
+}
+
+int solhip_reset_stats(solhip_ctx* ctx) {
+    if (!ctx) return fail(SOLHIP_E_INVALID, "null context");
+    ctx->stats = solhip_stats{};
+    return SOLHIP_OK;
+}
+
+int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scene** out) {
+    if (!ctx || !d || !out) return fail(SOLHIP_E_INVALID, "null argument");
+    *out = nullptr;
+    if (!d->num_nodes || !d->nodes || !d->num_tris || !d->tris || !d->shade)
+        return fail(SOLHIP_E_INVALID, "scene without geometry");
+    if (!d->num_lights || !d->lights) return fail(SOLHIP_E_NO_LIGHTS, "scene without light");
+    if (d->num_materials && !d->materials) return fail(SOLHIP_E_INVALID, "null material table");
+    if (d->num_tris >= (1u << 28)) return fail(SOLHIP_E_INVALID, "too many triangles");
+    if (d->bvh_depth > static_cast<uint32_t>(8 + kSpillEntries))
+        return fail(SOLHIP_E_TOO_DEEP, "BVH depth " + std::to_string(d->bvh_depth) + " exceeds the traversal stack");
+    for (uint32_t i = 0; i < d->num_tris; ++i) {
+        int m = d->shade[i].material, l = d->shade[i].light;
+        if (m >= static_cast<int>(d->num_materials) || m < -1 || l >= static_cast<int>(d->num_lights) || l < -1)
+            return fail(SOLHIP_E_INVALID, "material or light index out of range");
+    }
+    for (uint32_t i = 0; i < d->num_nodes; ++i) {
+        for (int c = 0; c < 2; ++c) {
+            int32_t ref = d->nodes[i].child[c];
+            if (ref >= 0) {
+                if (static_cast<uint32_t>(ref) >= d->num_nodes) return fail(SOLHIP_E_INVALID, "node index out of range");
+            } else {
+                uint32_t code = static_cast<uint32_t>(~ref);
+                if ((code >> 3) + (code & 7u) + 1u > d->num_tris) return fail(SOLHIP_E_INVALID, "leaf range out of range");
+            }
+        }
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto s = std::make_unique<solhip_scene>();
+    s->ctx = ctx;
+    s->num_nodes = d->num_nodes;
+    s->num_tris = d->num_tris;
+    s->num_materials = d->num_materials;
+    s->num_lights = d->num_lights;
+    s->bvh_depth = d->bvh_depth;
+    s->materials_host.assign(d->materials, d->materials + d->num_materials);
+    s->lights_host.assign(d->lights, d->lights + d->num_lights);
+    HIP_TRY(hipMalloc(&s->nodes, sizeof(sol_gpu_node) * d->num_nodes));
+    HIP_TRY(hipMalloc(&s->tris, sizeof(sol_gpu_tri) * d->num_tris));
+    HIP_TRY(hipMalloc(&s->shade, sizeof(sol_gpu_shade) * d->num_tris));
+    HIP_TRY(hipMalloc(&s->materials, sizeof(sol_material) * std::max<uint32_t>(1, d->num_materials)));
+    HIP_TRY(hipMalloc(&s->lights, sizeof(sol_light) * d->num_lights));
+    HIP_TRY(hipMemcpy(s->nodes, d->nodes, sizeof(sol_gpu_node) * d->num_nodes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->tris, d->tris, sizeof(sol_gpu_tri) * d->num_tris, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->shade, d->shade, sizeof(sol_gpu_shade) * d->num_tris, hipMemcpyHostToDevice));
+    if (d->num_materials)
+        HIP_TRY(hipMemcpy(s->materials, d->materials, sizeof(sol_material) * d->num_materials, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->lights, d->lights, sizeof(sol_light) * d->num_lights, hipMemcpyHostToDevice));
+    SceneView& v = s->view;
+    v.nodes = reinterpret_cast<const float4*>(s->nodes);
+    v.tris = reinterpret_cast<const float4*>(s->tris);
+    v.shade = reinterpret_cast<const float4*>(s->shade);
+    v.materials = s->materials;
+    v.lights = s->lights;
+    v.num_lights = d->num_lights;
+    v.top_nodes = std::min<uint32_t>(d->top_nodes, d->num_nodes);
+    v.num_nodes = d->num_nodes;
+    v.num_tris = d->num_tris;
+    v.camera = d->camera;
+    *out = s.release();
+    return SOLHIP_OK;
+}
+
+void solhip_scene_destroy(solhip_scene* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(s->nodes);
+    (void)hipFree(s->tris);
+    (void)hipFree(s->shade);
+    (void)hipFree(s->materials);
+    (void)hipFree(s->lights);
+    delete s;
+}
+
+int solhip_scene_set_materials(solhip_scene* s, const sol_material* m, uint32_t n) {
+    if (!s || !m || n != s->num_materials) return fail(SOLHIP_E_INVALID, "material count mismatch");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    HIP_TRY(hipMemcpy(s->materials, m, sizeof(sol_material) * n, hipMemcpyHostToDevice));
+    s->materials_host.assign(m, m + n);
+    return SOLHIP_OK;
+}
+
+int solhip_scene_set_lights(solhip_scene* s, const sol_light* l, uint32_t n) {
+    if (!s || !l || n != s->num_lights) return fail(SOLHIP_E_INVALID, "light count mismatch");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    HIP_TRY(hipMemcpy(s->lights, l, sizeof(sol_light) * n, hipMemcpyHostToDevice));
+    s->lights_host.assign(l, l + n);
+    return SOLHIP_OK;
+}
+
+int solhip_intersect(solhip_scene* s, const sol_ray* rays, uint64_t n, int any_hit, void* out, int device_ptrs) {
+    if (!s || (n && (!rays || !out))) return fail(SOLHIP_E_INVALID, "null argument");
+    if (n == 0) return SOLHIP_OK;
+    Context* ctx = s->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const sol_ray* d_rays = rays;
+    void* d_out = out;
+    DeviceBuffer tmp_rays, tmp_out;
+    size_t out_bytes = any_hit ? sizeof(int32_t) * n : sizeof(sol_hit) * n;
+    if (!device_ptrs) {
+        HIP_TRY(tmp_rays.alloc(sizeof(sol_ray) * n));
+        HIP_TRY(tmp_out.alloc(out_bytes));
+        HIP_TRY(hipMemcpyAsync(tmp_rays.ptr, rays, sizeof(sol_ray) * n, hipMemcpyHostToDevice, ctx->stream));
+        d_rays = static_cast<const sol_ray*>(tmp_rays.ptr);
+        d_out = tmp_out.ptr;
+    }
+    LaunchShape shape = ctx->shape(ctx->tuning.block_size, ctx->tuning.lds_stack);
+    int st = ctx->ensure_scratch(shape.grid * shape.block);
+    if (st) return st;
+    HIP_TRY(launch_zero_u64(ctx->counters, 1, ctx->stream));
+    if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
+    HIP_TRY(launch_intersect(s->view, d_rays, n, any_hit != 0, any_hit ? nullptr : static_cast<sol_hit*>(d_out),
+                             any_hit ? static_cast<int32_t*>(d_out) : nullptr, ctx->counters, ctx->scratch, shape,
+                             ctx->stream));
+    if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->ev_end, ctx->stream));
+    if (!device_ptrs) HIP_TRY(hipMemcpyAsync(out, d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->profiling) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+        ctx->stats.extend_ms += ms;
+        ctx->stats.extend_launches += 1;
+    }
+    (any_hit ? ctx->stats.shadow_rays : ctx->stats.extension_rays) += n;
+    ctx->stats.launches += 1;
+    return ctx->check_overflow();
+}
+
+int solhip_render(solhip_scene* s, const sol_pt_config* cfg, int renderer, uint32_t width, uint32_t height,
+                  uint64_t sample_index, uint32_t sample_count, float* accum, int device_ptr) {
+    if (!s || !cfg || !accum) return fail(SOLHIP_E_INVALID, "null argument");
+    if (width == 0 || height == 0) return fail(SOLHIP_E_INVALID, "empty image");
+    if (renderer != SOLHIP_RENDERER_MEGAKERNEL && renderer != SOLHIP_RENDERER_WAVEFRONT)
+        return fail(SOLHIP_E_INVALID, "unknown renderer");
+    if (cfg->min_survival_prob < 0 || cfg->max_survival_prob > 1 || cfg->min_survival_prob > cfg->max_survival_prob)
+        return fail(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    if (sample_count == 0) return SOLHIP_OK;
+    Context* ctx = s->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    size_t plane = static_cast<size_t>(width) * height;
+    DeviceBuffer tmp;
+    float* d_accum = accum;
+    if (!device_ptr) {
+        HIP_TRY(tmp.alloc(sizeof(float) * 3 * plane));
+        HIP_TRY(hipMemcpyAsync(tmp.ptr, accum, sizeof(float) * 3 * plane, hipMemcpyHostToDevice, ctx->stream));
+        d_accum = static_cast<float*>(tmp.ptr);
+    }
+    int st;
+    if (renderer == SOLHIP_RENDERER_MEGAKERNEL)
+        st = render_megakernel(*s, *cfg, width, height, sample_index, sample_count, d_accum);
+    else
+        st = render_wavefront(*s, *cfg, width, height, sample_index, sample_count, d_accum);
+    if (st) return st;
+    if (!device_ptr)
+        HIP_TRY(hipMemcpyAsync(accum, d_accum, sizeof(float) * 3 * plane, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ctx->check_overflow();
+}
+
+} // extern "C"
+
+namespace solhip {
+
+int render_megakernel(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,
+                      float* d_accum) {
+    Context* ctx = s.ctx;
+    LaunchShape shape = ctx->shape(ctx->tuning.block_size, ctx->tuning.lds_stack);
+    int st = ctx->ensure_scratch(shape.grid * shape.block);
+    if (st) return st;
+    HIP_TRY(launch_zero_u64(ctx->counters, 4, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
+    HIP_TRY(launch_megakernel(s.view, cfg, w, h, si, sc, d_accum, ctx->counters, ctx->counters + 1, ctx->scratch,
+                              shape, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev_end, ctx->stream));
+    unsigned long long c[4];
+    st = ctx->read_counters(c, 4);
+    if (st) return st;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+    ctx->stats.render_ms += ms;
+    ctx->stats.extension_rays += c[1];
+    ctx->stats.shadow_rays += c[2];
+    ctx->stats.samples += c[3];
+    ctx->stats.launches += 1;
+    return SOLHIP_OK;
+}
+
+} // namespace solhip

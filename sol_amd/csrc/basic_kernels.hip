@@ -1,0 +1,223 @@
+// Batched intersection (Geometry::intersect_closest / intersect_any, src/triangle_mesh.cpp:64-117
+// of sol) and the plain GPU megakernel (PathTracer::render + trace_path,
+// src/algorithms/path_tracer.cpp:9-120), one thread per pixel looping over its samples.
+// Both are persistent: each wave fetches 64 work items at a time from an atomic counter.
+#include "launch.h"
+#include "path.h"
+#include "traverse.h"
+
+namespace solhip {
+
+namespace {
+
+SOL_DEV unsigned long long wave_grab(unsigned long long* counter, unsigned chunk) {
+    unsigned long long base = 0;
+    if (__lane_id() == 0) base = atomicAdd(counter, static_cast<unsigned long long>(chunk));
+    return __shfl(base, 0);
+}
+
+SOL_DEV void wave_add(unsigned long long* counter, unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(counter, v);
+}
+
+template <int BLOCK, int LDS>
+SOL_DEV Stack make_stack(int* lds_stack, const TraversalScratch& sc) {
+    int gtid = blockIdx.x * BLOCK + threadIdx.x;
+    Stack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.spill = sc.spill + gtid;
+    st.spill_stride = sc.max_threads;
+    st.capacity = LDS + sc.spill_entries;
+    st.overflow = sc.overflow;
+    return st;
+}
+
+template <bool ANY, int BLOCK, int LDS>
+__global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol_ray* __restrict__ rays, uint64_t n,
+                                                          sol_hit* __restrict__ hits, int32_t* __restrict__ occluded,
+                                                          unsigned long long* counter, TraversalScratch sc) {
+    __shared__ float4 lds_nodes[kTopNodesMax * 4];
+    __shared__ int lds_stack[LDS * BLOCK];
+    load_top_nodes<BLOCK>(s, lds_nodes);
+    Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
+    while (true) {
+        unsigned long long base = wave_grab(counter, 64);
+        if (base >= n) break;
+        uint64_t i = base + __lane_id();
+        if (i >= n) continue;
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        float4 a = rp[0], b = rp[1];
+        V3 org{a.x, a.y, a.z}, dir{b.x, b.y, b.z};
+        RayPre r = make_ray_pre(org, dir);
+        HitRecord hit;
+        bool found = traverse<ANY, BLOCK, LDS>(s, lds_nodes, st, r, a.w, b.w, hit);
+        if (ANY) {
+            occluded[i] = found ? 1 : 0;
+        } else {
+            sol_hit h;
+            h.prim = found ? hit.prim : -1;
+            h.t = found ? hit.t : b.w;
+            h.u = found ? hit.u : 0.0f;
+            h.v = found ? hit.v : 0.0f;
+            reinterpret_cast<float4*>(hits)[i] = make_float4(__int_as_float(h.prim), h.t, h.u, h.v);
+        }
+    }
+}
+
+template <int BLOCK, int LDS>
+SOL_DEV V3 trace_path(const SceneView& s, const sol_pt_config& cfg, const float4* lds_nodes, const Stack& st,
+                      Pcg& rng, V3 org, V3 dir, unsigned& n_ext, unsigned& n_shadow) {
+    const float pick = 1.0f / static_cast<float>(s.num_lights);
+    float pdf_prev = 0.0f;
+    V3 T{1.0f, 1.0f, 1.0f};
+    V3 color{0.0f, 0.0f, 0.0f};
+    float tmin = 0.0f;
+    for (uint32_t depth = 0; depth < cfg.max_path_len; ++depth) {
+        RayPre r = make_ray_pre(org, dir);
+        HitRecord hit;
+        ++n_ext;
+        if (!traverse<false, BLOCK, LDS>(s, lds_nodes, st, r, tmin, FLT_MAX, hit)) break;
+        Surface sf = decode_hit(s, org, dir, hit.slot, hit.t, hit.u, hit.v);
+        if (sf.light >= 0 && sf.front) {
+            V3 Le;
+            float w;
+            color = color + hit_emission(s, cfg.flags, sf, dir, hit.t, pdf_prev, pick, T, Le, w);
+        }
+        if (sf.mat < 0) break;
+        const sol_material m = s.materials[sf.mat];
+        bool skip_nee = (cfg.flags & SOL_PT_DISABLE_NEE) || is_specular(m.type);
+        V3 out = -dir;
+        if (!skip_nee) {
+            NeeSample ns;
+            if (nee_sample(s, cfg.flags, sf, m, out, T, rng, pick, ns)) {
+                RayPre sr = make_ray_pre(sf.point, ns.shadow_dir);
+                HitRecord unused;
+                ++n_shadow;
+                if (!traverse<true, BLOCK, LDS>(s, lds_nodes, st, sr, cfg.ray_offset, 1.0f - cfg.ray_offset, unused))
+                    color = color + ns.contrib;
+            }
+        }
+        float q;
+        if (!russian_roulette(cfg, depth, T, rng, q)) break;
+        BsdfSample bs;
+        if (!bsdf_sample(m, sf, ortho_basis(sf.n), out, rng, bs)) break;
+        T = T * (bs.color * (bs.cos / (bs.pdf * q)));
+        org = sf.point;
+        dir = bs.in;
+        tmin = cfg.ray_offset;
+        pdf_prev = skip_nee ? 0.0f : bs.pdf;
+    }
+    return color;
+}
+
+// Pixels are handed out in 8x8 tiles (one tile per wave) so that primary rays of a wave are
+// coherent.
+template <int BLOCK, int LDS>
+__global__ __launch_bounds__(BLOCK) void megakernel(SceneView s, sol_pt_config cfg, uint32_t w, uint32_t h,
+                                                    uint64_t sample_index, uint32_t sample_count,
+                                                    float* __restrict__ accum, unsigned long long* counter,
+                                                    unsigned long long* stats, TraversalScratch sc) {
+    __shared__ float4 lds_nodes[kTopNodesMax * 4];
+    __shared__ int lds_stack[LDS * BLOCK];
+    load_top_nodes<BLOCK>(s, lds_nodes);
+    Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
+    const uint32_t tiles_x = (w + 7) / 8, tiles_y = (h + 7) / 8;
+    const unsigned long long total = static_cast<unsigned long long>(tiles_x) * tiles_y * 64;
+    const size_t plane = static_cast<size_t>(w) * h;
+    unsigned n_ext = 0, n_shadow = 0, n_samples = 0;
+    while (true) {
+        unsigned long long base = wave_grab(counter, 64);
+        if (base >= total) break;
+        uint32_t tile = static_cast<uint32_t>(base / 64);
+        uint32_t lane = __lane_id();
+        uint32_t x = (tile % tiles_x) * 8 + (lane & 7);
+        uint32_t y = (tile / tiles_x) * 8 + (lane >> 3);
+        if (x >= w || y >= h) continue;
+        V3 color{0.0f, 0.0f, 0.0f};
+        for (uint32_t i = 0; i < sample_count; ++i) {
+            Pcg rng;
+            rng.seed(pixel_seed(sample_index + i, x, y));
+            V3 org, dir;
+            camera_ray(s.camera, x, y, w, h, rng, org, dir);
+            color = color + trace_path<BLOCK, LDS>(s, cfg, lds_nodes, st, rng, org, dir, n_ext, n_shadow);
+        }
+        n_samples += sample_count;
+        size_t p = static_cast<size_t>(y) * w + x;
+        accum[p] += color.x;
+        accum[plane + p] += color.y;
+        accum[2 * plane + p] += color.z;
+    }
+    wave_add(stats + 0, n_ext);
+    wave_add(stats + 1, n_shadow);
+    wave_add(stats + 2, n_samples);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+template <int BLOCK, int LDS>
+hipError_t intersect_impl(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
+                          int32_t* occluded, unsigned long long* counter, const TraversalScratch& sc, int grid,
+                          hipStream_t stream) {
+    if (any_hit)
+        hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, rays, n, hits,
+                           occluded, counter, sc);
+    else
+        hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, rays, n,
+                           hits, occluded, counter, sc);
+    return hipGetLastError();
+}
+
+template <int BLOCK, int LDS>
+hipError_t megakernel_impl(const SceneView& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si,
+                           uint32_t sc_count, float* accum, unsigned long long* counter, unsigned long long* stats,
+                           const TraversalScratch& sc, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL((megakernel<BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, cfg, w, h, si, sc_count, accum,
+                       counter, stats, sc);
+    return hipGetLastError();
+}
+
+} // namespace
+
+#define SOLHIP_DISPATCH(FN, ...)                                                                   \
+    switch (shape.block * 100 + shape.lds_stack) {                                                 \
+        case 6408: return FN<64, 8>(__VA_ARGS__);                                                  \
+        case 6416: return FN<64, 16>(__VA_ARGS__);                                                 \
+        case 6432: return FN<64, 32>(__VA_ARGS__);                                                 \
+        case 12808: return FN<128, 8>(__VA_ARGS__);                                                \
+        case 12816: return FN<128, 16>(__VA_ARGS__);                                               \
+        case 12832: return FN<128, 32>(__VA_ARGS__);                                               \
+        case 25608: return FN<256, 8>(__VA_ARGS__);                                                \
+        case 25616: return FN<256, 16>(__VA_ARGS__);                                               \
+        case 25632: return FN<256, 32>(__VA_ARGS__);                                               \
+        default: return hipErrorInvalidValue;                                                      \
+    }
+
+hipError_t launch_intersect(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
+                            int32_t* occluded, unsigned long long* work_counter, const TraversalScratch& scratch,
+                            const LaunchShape& shape, hipStream_t stream) {
+    SOLHIP_DISPATCH(intersect_impl, s, rays, n, any_hit, hits, occluded, work_counter, scratch, shape.grid, stream)
+}
+
+hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint32_t width, uint32_t height,
+                             uint64_t sample_index, uint32_t sample_count, float* accum,
+                             unsigned long long* work_counter, StatCounters stats, const TraversalScratch& scratch,
+                             const LaunchShape& shape, hipStream_t stream) {
+    SOLHIP_DISPATCH(megakernel_impl, s, cfg, width, height, sample_index, sample_count, accum, work_counter, stats,
+                    scratch, shape.grid, stream)
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+int persistent_grid(int compute_units, int block) {
+    int per_cu = 2048 / block;
+    return compute_units * per_cu;
+}
+
+} // namespace solhip

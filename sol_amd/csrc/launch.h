@@ -1,0 +1,42 @@
+// Host-side launchers of the kernels, one translation unit per kernel family. Every launcher
+// enqueues on the given stream and returns the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace solhip {
+
+// Traversal scratch shared by every traversal kernel of a context.
+struct TraversalScratch {
+    int* spill;                       // spill_entries * max_threads ints
+    int spill_entries;                // per lane
+    int max_threads;                  // capacity of the spill area, in lanes
+    int* overflow;                    // set when a stack would overflow
+};
+
+struct LaunchShape {
+    int block;       // threads per workgroup: 64, 128 or 256
+    int lds_stack;   // LDS stack entries per lane: 8, 16 or 32
+    int grid;        // workgroups (persistent: work is fetched from an atomic counter)
+};
+
+// Counters a kernel adds to: [0] extension rays, [1] shadow rays, [2] samples.
+using StatCounters = unsigned long long*;
+
+hipError_t launch_intersect(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
+                            int32_t* occluded, unsigned long long* work_counter, const TraversalScratch& scratch,
+                            const LaunchShape& shape, hipStream_t stream);
+
+hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint32_t width, uint32_t height,
+                             uint64_t sample_index, uint32_t sample_count, float* accum,
+                             unsigned long long* work_counter, StatCounters stats, const TraversalScratch& scratch,
+                             const LaunchShape& shape, hipStream_t stream);
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t stream);
+
+// Workgroups for a persistent launch: enough for every CU at full occupancy.
+int persistent_grid(int compute_units, int block);
+
+} // namespace solhip

@@ -1,0 +1,94 @@
+// Host-side state behind the opaque handles of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "solhip.h"
+#include "launch.h"
+
+namespace solhip {
+
+// Spill entries per lane behind the LDS short stack.
+constexpr int kSpillEntries = 88;
+constexpr int kNumCounters = 16;
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { if (ptr) (void)hipFree(ptr); }
+    hipError_t alloc(size_t n) {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&ptr, n ? n : 1);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    // Grows to at least n bytes (contents are not preserved).
+    hipError_t reserve(size_t n) { return n <= bytes && ptr ? hipSuccess : alloc(n); }
+    template <typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct WavefrontState;  // defined in wavefront.hip
+
+inline solhip_tuning default_tuning() {
+    solhip_tuning t{};
+    t.block_size = 64;
+    t.lds_stack = 16;
+    t.path_slots = 1u << 20;
+    t.tile_w = 32;
+    t.tile_h = 32;
+    t.graph_iters = 16;
+    return t;
+}
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    solhip_device_info info{};
+    solhip_tuning tuning{};
+    bool profiling = false;
+    solhip_stats stats{};
+    unsigned long long* counters = nullptr;  // kNumCounters device counters
+    TraversalScratch scratch{};
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    WavefrontState* wave = nullptr;
+
+    int ensure_scratch(int threads);
+    LaunchShape shape(int block, int lds) const;
+    int check_overflow();
+    int read_counters(unsigned long long* host, int n);
+    void release_wavefront();
+};
+
+int set_error(int code, const std::string& msg);
+
+} // namespace solhip
+
+struct solhip_ctx : solhip::Context {};
+
+struct solhip_scene {
+    solhip::Context* ctx = nullptr;
+    sol_gpu_node* nodes = nullptr;
+    sol_gpu_tri* tris = nullptr;
+    sol_gpu_shade* shade = nullptr;
+    sol_material* materials = nullptr;
+    sol_light* lights = nullptr;
+    uint32_t num_nodes = 0, num_tris = 0, num_materials = 0, num_lights = 0, bvh_depth = 0;
+    std::vector<sol_material> materials_host;
+    std::vector<sol_light> lights_host;
+    solhip::SceneView view{};
+};
+
+namespace solhip {
+int render_megakernel(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,
+                      float* d_accum);
+int render_wavefront(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,
+                     float* d_accum);
+} // namespace solhip

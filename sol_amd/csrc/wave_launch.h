@@ -1,0 +1,32 @@
+// Launchers of the wavefront kernels.
+#pragma once
+
+#include "launch.h"
+#include "wave.h"
+
+namespace solhip {
+
+struct WaveLaunch {
+    SceneView single;         // the scene of a single-scene run (unused when multi)
+    WaveTables tabs;
+    WaveBuf buf;
+    sol_pt_config cfg;
+    TraversalScratch scratch;
+    int block;                // traversal kernels
+    int lds_stack;
+    int grid_traverse;
+    int grid_shade;
+    bool multi;
+    hipStream_t stream;
+};
+
+hipError_t wave_launch_extend(const WaveLaunch& L, int cur);
+hipError_t wave_launch_shade(const WaveLaunch& L, int queue, int nxt);
+hipError_t wave_launch_shadow(const WaveLaunch& L);
+hipError_t wave_launch_regen(const WaveLaunch& L, int nxt, bool initial);
+hipError_t wave_launch_reset(const WaveLaunch& L, int nxt);
+hipError_t wave_launch_set_chunk(const WaveLaunch& L, unsigned long long begin, unsigned long long end);
+hipError_t wave_launch_reduce(const WaveLaunch& L, uint32_t first_tile, uint32_t num_tiles, uint32_t max_tile_pixels,
+                              unsigned long long chunk_begin);
+
+} // namespace solhip

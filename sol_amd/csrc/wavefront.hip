@@ -1,0 +1,406 @@
+// Host side of the wavefront renderer: buffers, the tile work queue, the graph-captured bounce
+// loop, and the single-scene and batched entry points.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "state.h"
+#include "wave_launch.h"
+
+namespace solhip {
+
+namespace {
+
+#define HIP_OK(expr)                                                                             \
+    do {                                                                                         \
+        hipError_t e__ = (expr);                                                                 \
+        if (e__ != hipSuccess)                                                                   \
+            return set_error(SOLHIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e__));   \
+    } while (0)
+
+// Items (path samples) whose colours the sample buffer can hold at once: 12 bytes each.
+constexpr unsigned long long kDefaultSampleCapacity = 192ull << 20;
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Job {
+    uint32_t scene;
+    uint32_t w, h;
+    uint64_t sample_index;
+    uint32_t sample_count;
+    float* accum;
+};
+
+} // namespace
+
+struct WavefrontState {
+    uint32_t slots = 0;
+    DeviceBuffer arena, counters, samples;
+    unsigned long long sample_capacity = 0;
+    WaveBuf buf{};
+    DeviceBuffer t_scenes, t_job_scene, t_job_accum, t_job_w, t_job_h, t_tile_job, t_tile_x0, t_tile_y0, t_tile_w,
+        t_tile_h, t_tile_item0, t_tile_sample0;
+    uint64_t generation = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
+    std::vector<hipEvent_t> events;
+
+    ~WavefrontState() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+
+    int ensure_slots(uint32_t n) {
+        if (n == slots && arena.ptr) return SOLHIP_OK;
+        // per slot: path 7 f + 2 u64 + 2 u32, hit 13, extend 2 x 8, shade 3, shadow 10, regen 1
+        const int words32 = 7 + 2 + 13 + 16 + 3 + 10 + 1;
+        size_t bytes = 0;
+        bytes += align_up(sizeof(float) * n) * words32 + align_up(sizeof(unsigned long long) * n) * 2 + 4096;
+        HIP_OK(arena.alloc(bytes));
+        HIP_OK(counters.alloc(256 * 3));
+        char* p = arena.as<char>();
+        auto f32 = [&]() { float* r = reinterpret_cast<float*>(p); p += align_up(sizeof(float) * n); return r; };
+        auto u32 = [&]() { uint32_t* r = reinterpret_cast<uint32_t*>(p); p += align_up(sizeof(uint32_t) * n); return r; };
+        auto u64 = [&]() {
+            unsigned long long* r = reinterpret_cast<unsigned long long*>(p);
+            p += align_up(sizeof(unsigned long long) * n);
+            return r;
+        };
+        WaveBuf& b = buf;
+        b.slots = n;
+        b.tx = f32(); b.ty = f32(); b.tz = f32();
+        b.cx = f32(); b.cy = f32(); b.cz = f32();
+        b.pdf = f32();
+        b.rng = u64(); b.item = u64();
+        b.depth = u32(); b.scene = u32();
+        b.hpx = f32(); b.hpy = f32(); b.hpz = f32();
+        b.hnx = f32(); b.hny = f32(); b.hnz = f32();
+        b.hfx = f32(); b.hfy = f32(); b.hfz = f32();
+        b.hdx = f32(); b.hdy = f32(); b.hdz = f32();
+        b.hmat = reinterpret_cast<int*>(u32());
+        for (int q = 0; q < 2; ++q) {
+            for (int k = 0; k < 7; ++k) b.eq[q][k] = f32();
+            b.es[q] = u32();
+        }
+        for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = u32();
+        for (int k = 0; k < 9; ++k) b.sq[k] = f32();
+        b.ss = u32();
+        b.rq = u32();
+        char* c = counters.as<char>();
+        b.counts = reinterpret_cast<unsigned*>(c);
+        b.items = reinterpret_cast<unsigned long long*>(c + 256);
+        b.stats = reinterpret_cast<unsigned long long*>(c + 512);
+        slots = n;
+        ++generation;
+        return SOLHIP_OK;
+    }
+
+    int ensure_samples(unsigned long long items) {
+        if (items <= sample_capacity && samples.ptr) return SOLHIP_OK;
+        HIP_OK(samples.alloc(sizeof(float) * 3 * items));
+        sample_capacity = items;
+        buf.samples = samples.as<float>();
+        buf.sample_capacity = items;
+        ++generation;
+        return SOLHIP_OK;
+    }
+
+    template <typename T>
+    int upload(DeviceBuffer& d, const std::vector<T>& v, hipStream_t stream) {
+        size_t bytes = sizeof(T) * std::max<size_t>(1, v.size());
+        if (d.bytes < bytes || !d.ptr) {
+            HIP_OK(d.alloc(bytes * 2));
+            ++generation;
+        }
+        if (!v.empty()) HIP_OK(hipMemcpyAsync(d.ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream));
+        return SOLHIP_OK;
+    }
+
+    int ensure_events(size_t n) {
+        while (events.size() < n) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            events.push_back(e);
+        }
+        return SOLHIP_OK;
+    }
+};
+
+void Context::release_wavefront() {
+    delete wave;
+    wave = nullptr;
+}
+
+namespace {
+
+// One bounce of every active path: reset, extend, shade (three queues), shadow, regen.
+hipError_t enqueue_iteration(const WaveLaunch& L, int cur, hipEvent_t* ev) {
+    int nxt = 1 - cur;
+    hipError_t e;
+    if ((e = wave_launch_reset(L, nxt)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev[0], L.stream)) != hipSuccess) return e;
+    if ((e = wave_launch_extend(L, cur)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev[1], L.stream)) != hipSuccess) return e;
+    for (int q = 0; q < kNumShadeQueues; ++q)
+        if ((e = wave_launch_shade(L, q, nxt)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev[2], L.stream)) != hipSuccess) return e;
+    if ((e = wave_launch_shadow(L)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev[3], L.stream)) != hipSuccess) return e;
+    if ((e = wave_launch_regen(L, nxt, false)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev[4], L.stream)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+std::string graph_key(const WaveLaunch& L, const WavefrontState& st, int iters) {
+    std::string k(reinterpret_cast<const char*>(&L.cfg), sizeof(L.cfg));
+    k.append(reinterpret_cast<const char*>(&L.single), sizeof(L.single));
+    k.append(reinterpret_cast<const char*>(&L.tabs), sizeof(L.tabs));
+    int meta[6] = {L.block, L.lds_stack, L.grid_traverse, L.grid_shade, L.multi ? 1 : 0, iters};
+    k.append(reinterpret_cast<const char*>(meta), sizeof(meta));
+    k.append(reinterpret_cast<const char*>(&st.generation), sizeof(st.generation));
+    k.append(reinterpret_cast<const char*>(&L.scratch), sizeof(L.scratch));
+    return k;
+}
+
+// Runs the jobs to completion. Jobs index `views` through Job::scene.
+int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::vector<Job>& jobs,
+                  const sol_pt_config& cfg, uint32_t max_depth) {
+    if (!ctx.wave) ctx.wave = new WavefrontState();
+    WavefrontState& st = *ctx.wave;
+    const solhip_tuning& tu = ctx.tuning;
+    int rc = st.ensure_slots(tu.path_slots);
+    if (rc) return rc;
+
+    // Tiles: sample passes small enough that one tile's samples fit the sample buffer.
+    std::vector<uint32_t> tile_job, tile_x0, tile_y0, tile_w, tile_h;
+    std::vector<unsigned long long> tile_item0, tile_sample0;
+    unsigned long long total = 0;
+    uint32_t max_tile_pixels = 0;
+    const unsigned long long cap = kDefaultSampleCapacity;
+    for (uint32_t j = 0; j < jobs.size(); ++j) {
+        const Job& job = jobs[j];
+        for (uint32_t y0 = 0; y0 < job.h; y0 += tu.tile_h) {
+            for (uint32_t x0 = 0; x0 < job.w; x0 += tu.tile_w) {
+                uint32_t tw = std::min(tu.tile_w, job.w - x0), th = std::min(tu.tile_h, job.h - y0);
+                unsigned long long px = static_cast<unsigned long long>(tw) * th;
+                uint32_t max_pass = static_cast<uint32_t>(std::max<unsigned long long>(1, cap / px));
+                // Samples beyond one pass are split into further tiles over the same pixels, each
+                // accumulated on its own (only for sample counts far above the capacity).
+                for (uint32_t s0 = 0; s0 < job.sample_count; s0 += max_pass) {
+                    uint32_t n = std::min(max_pass, job.sample_count - s0);
+                    tile_job.push_back(j);
+                    tile_x0.push_back(x0);
+                    tile_y0.push_back(y0);
+                    tile_w.push_back(tw);
+                    tile_h.push_back(th);
+                    tile_item0.push_back(total);
+                    tile_sample0.push_back(job.sample_index + s0);
+                    total += px * n;
+                    max_tile_pixels = std::max<uint32_t>(max_tile_pixels, static_cast<uint32_t>(px));
+                }
+            }
+        }
+    }
+    tile_item0.push_back(total);
+    const uint32_t num_tiles = static_cast<uint32_t>(tile_job.size());
+
+    std::vector<uint32_t> job_scene, job_w, job_h;
+    std::vector<float*> job_accum;
+    for (const Job& j : jobs) {
+        job_scene.push_back(j.scene);
+        job_w.push_back(j.w);
+        job_h.push_back(j.h);
+        job_accum.push_back(j.accum);
+    }
+    hipStream_t s = ctx.stream;
+    if ((rc = st.upload(st.t_scenes, views, s)) || (rc = st.upload(st.t_job_scene, job_scene, s)) ||
+        (rc = st.upload(st.t_job_accum, job_accum, s)) || (rc = st.upload(st.t_job_w, job_w, s)) ||
+        (rc = st.upload(st.t_job_h, job_h, s)) || (rc = st.upload(st.t_tile_job, tile_job, s)) ||
+        (rc = st.upload(st.t_tile_x0, tile_x0, s)) || (rc = st.upload(st.t_tile_y0, tile_y0, s)) ||
+        (rc = st.upload(st.t_tile_w, tile_w, s)) || (rc = st.upload(st.t_tile_h, tile_h, s)) ||
+        (rc = st.upload(st.t_tile_item0, tile_item0, s)) || (rc = st.upload(st.t_tile_sample0, tile_sample0, s)))
+        return rc;
+
+    // Chunks of whole tiles whose items fit the sample buffer.
+    struct Chunk { uint32_t first, count; unsigned long long begin, end; };
+    std::vector<Chunk> chunks;
+    unsigned long long biggest = 0;
+    for (uint32_t t = 0; t < num_tiles;) {
+        Chunk c{t, 0, tile_item0[t], tile_item0[t]};
+        while (t < num_tiles && (c.count == 0 || tile_item0[t + 1] - c.begin <= cap)) {
+            c.end = tile_item0[t + 1];
+            ++c.count;
+            ++t;
+        }
+        biggest = std::max(biggest, c.end - c.begin);
+        chunks.push_back(c);
+    }
+    if ((rc = st.ensure_samples(std::max<unsigned long long>(biggest, 1)))) return rc;
+
+    const bool multi = views.size() > 1;
+    WaveLaunch L{};
+    L.single = views[0];
+    L.tabs.scenes = st.t_scenes.as<SceneView>();
+    L.tabs.job_scene = st.t_job_scene.as<uint32_t>();
+    L.tabs.job_accum = st.t_job_accum.as<float*>();
+    L.tabs.job_w = st.t_job_w.as<uint32_t>();
+    L.tabs.job_h = st.t_job_h.as<uint32_t>();
+    L.tabs.tile_job = st.t_tile_job.as<uint32_t>();
+    L.tabs.tile_x0 = st.t_tile_x0.as<uint32_t>();
+    L.tabs.tile_y0 = st.t_tile_y0.as<uint32_t>();
+    L.tabs.tile_w = st.t_tile_w.as<uint32_t>();
+    L.tabs.tile_h = st.t_tile_h.as<uint32_t>();
+    L.tabs.tile_item0 = st.t_tile_item0.as<unsigned long long>();
+    L.tabs.tile_sample0 = st.t_tile_sample0.as<unsigned long long>();
+    L.tabs.num_tiles = num_tiles;
+    L.buf = st.buf;
+    L.cfg = cfg;
+    L.block = static_cast<int>(tu.block_size);
+    L.lds_stack = static_cast<int>(tu.lds_stack);
+    L.grid_traverse = persistent_grid(ctx.info.compute_units, L.block);
+    L.grid_shade = persistent_grid(ctx.info.compute_units, 256);
+    L.multi = multi;
+    L.stream = s;
+    if ((rc = ctx.ensure_scratch(L.grid_traverse * L.block))) return rc;
+    L.scratch = ctx.scratch;
+    if (max_depth > static_cast<uint32_t>(L.lds_stack + ctx.scratch.spill_entries + 1))
+        return set_error(SOLHIP_E_TOO_DEEP, "BVH deeper than the traversal stack");
+
+    const int iters = static_cast<int>(std::max<uint32_t>(2, tu.graph_iters & ~1u));
+    const bool use_graph = !ctx.profiling;
+    if (use_graph) {
+        std::string key = graph_key(L, st, iters);
+        if (key != st.graph_key || !st.graph_exec) {
+            if (st.graph_exec) HIP_OK(hipGraphExecDestroy(st.graph_exec));
+            if (st.graph) HIP_OK(hipGraphDestroy(st.graph));
+            st.graph_exec = nullptr;
+            st.graph = nullptr;
+            st.graph_key.clear();
+            HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int it = 0; it < iters && e == hipSuccess; ++it) e = enqueue_iteration(L, it & 1, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(s, &g);
+            HIP_OK(e);
+            HIP_OK(e2);
+            st.graph = g;
+            HIP_OK(hipGraphInstantiate(&st.graph_exec, g, nullptr, nullptr, 0));
+            st.graph_key = key;
+        }
+    } else {
+        if ((rc = st.ensure_events(static_cast<size_t>(iters) * 5))) return rc;
+    }
+
+    HIP_OK(hipMemsetAsync(st.buf.stats, 0, sizeof(unsigned long long) * 4, s));
+    HIP_OK(hipEventRecord(ctx.ev_begin, s));
+    uint64_t replays = 0, launches = 0;
+    for (const Chunk& c : chunks) {
+        HIP_OK(wave_launch_set_chunk(L, c.begin, c.end));
+        HIP_OK(wave_launch_regen(L, 0, true));
+        launches += 2;
+        while (true) {
+            if (use_graph) {
+                HIP_OK(hipGraphLaunch(st.graph_exec, s));
+            } else {
+                for (int it = 0; it < iters; ++it) HIP_OK(enqueue_iteration(L, it & 1, &st.events[it * 5]));
+            }
+            ++replays;
+            launches += static_cast<uint64_t>(iters) * 7;
+            unsigned pending = 0;
+            HIP_OK(hipMemcpyAsync(&pending, st.buf.counts + kCntExtend0, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            if (!use_graph) {
+                for (int it = 0; it < iters; ++it) {
+                    float ms[4];
+                    for (int k = 0; k < 4; ++k)
+                        HIP_OK(hipEventElapsedTime(&ms[k], st.events[it * 5 + k], st.events[it * 5 + k + 1]));
+                    ctx.stats.extend_ms += ms[0];
+                    ctx.stats.shade_ms += ms[1];
+                    ctx.stats.shadow_ms += ms[2];
+                    ctx.stats.regen_ms += ms[3];
+                    ctx.stats.extend_launches += 1;
+                }
+            }
+            if (pending == 0) break;
+        }
+        HIP_OK(wave_launch_reduce(L, c.first, c.count, max_tile_pixels, c.begin));
+        launches += 1;
+    }
+    HIP_OK(hipEventRecord(ctx.ev_end, s));
+    unsigned long long stats[3];
+    HIP_OK(hipMemcpyAsync(stats, st.buf.stats, sizeof(stats), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, ctx.ev_begin, ctx.ev_end));
+    ctx.stats.render_ms += ms;
+    ctx.stats.extension_rays += stats[0];
+    ctx.stats.shadow_rays += stats[1];
+    ctx.stats.samples += stats[2];
+    ctx.stats.graph_replays += use_graph ? replays : 0;
+    ctx.stats.launches += launches;
+    return SOLHIP_OK;
+}
+
+} // namespace
+
+int render_wavefront(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,
+                     float* d_accum) {
+    if (cfg.max_path_len == 0) return SOLHIP_OK;
+    std::vector<SceneView> views{s.view};
+    std::vector<Job> jobs{Job{0, w, h, si, sc, d_accum}};
+    return run_wavefront(*s.ctx, views, jobs, cfg, s.bvh_depth);
+}
+
+} // namespace solhip
+
+using namespace solhip;
+
+extern "C" {
+
+int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t num_jobs, const sol_pt_config* cfg) {
+    if (!ctx || !cfg || (num_jobs && !jobs)) return set_error(SOLHIP_E_INVALID, "null argument");
+    if (cfg->min_survival_prob < 0 || cfg->max_survival_prob > 1 || cfg->min_survival_prob > cfg->max_survival_prob)
+        return set_error(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    std::vector<SceneView> views;
+    std::unordered_map<const solhip_scene*, uint32_t> index;
+    std::vector<Job> list;
+    uint32_t depth = 0;
+    for (uint32_t i = 0; i < num_jobs; ++i) {
+        const solhip_batch_job& j = jobs[i];
+        if (!j.scene || !j.accum || j.width == 0 || j.height == 0) return set_error(SOLHIP_E_INVALID, "invalid batch job");
+        if (j.scene->ctx != ctx) return set_error(SOLHIP_E_CONTEXT, "scene belongs to another context");
+        if (j.sample_count == 0) continue;
+        auto it = index.find(j.scene);
+        uint32_t si;
+        if (it == index.end()) {
+            si = static_cast<uint32_t>(views.size());
+            index[j.scene] = si;
+            views.push_back(j.scene->view);
+            depth = std::max(depth, j.scene->bvh_depth);
+        } else {
+            si = it->second;
+        }
+        list.push_back(Job{si, j.width, j.height, j.sample_index, j.sample_count, j.accum});
+    }
+    if (list.empty() || cfg->max_path_len == 0) return SOLHIP_OK;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return set_error(SOLHIP_E_HIP, hipGetErrorString(e));
+    int rc = run_wavefront(*ctx, views, list, *cfg, depth);
+    if (rc) return rc;
+    return ctx->check_overflow();
+}
+
+int solhip_backward(solhip_scene*, const sol_pt_config*, uint32_t, uint32_t, uint64_t, uint32_t, const float*, float*,
+                    float*, int) {
+    return set_error(SOLHIP_E_INVALID, "adjoint not built");
+}
+
+int solhip_autotune(solhip_scene*, const sol_pt_config*, uint32_t, uint32_t, uint32_t, solhip_tuning*) {
+    return set_error(SOLHIP_E_INVALID, "autotuner not built");
+}
+
+} // extern "C"

@@ -1,0 +1,69 @@
+// BSDFs, as sol's class family (reference include/sol/bsdfs.h:28-182). On this side a BSDF is
+// a description: the shading arithmetic runs on the GPU. flatten() writes the constant-texture
+// parameters into the flat record the GPU library reads.
+#ifndef SOL_BSDFS_H
+#define SOL_BSDFS_H
+
+#include "sol/textures.h"
+#include "sol_types.h"
+
+namespace sol {
+
+class Bsdf {
+public:
+    enum class Tag { DiffuseBsdf, PhongBsdf, MirrorBsdf, GlassBsdf };
+    // Classification used to skip next-event estimation on specular surfaces
+    // (bsdfs.h:39-43, path_tracer.cpp:66 of the reference).
+    enum class Type { Diffuse = 0, Glossy = 1, Specular = 2 };
+
+    const Tag tag;
+    const Type type;
+
+    /// Parameter group of the material for gradient computations (-1: not differentiated).
+    int32_t group = -1;
+
+    Bsdf(Tag tag, Type type) : tag(tag), type(type) {}
+    virtual ~Bsdf() = default;
+
+    virtual sol_material flatten() const = 0;
+};
+
+class DiffuseBsdf final : public Bsdf {
+public:
+    explicit DiffuseBsdf(const ColorTexture& kd) : Bsdf(Tag::DiffuseBsdf, Type::Diffuse), kd_(kd) {}
+    sol_material flatten() const override;
+private:
+    const ColorTexture& kd_;
+};
+
+class PhongBsdf final : public Bsdf {
+public:
+    PhongBsdf(const ColorTexture& ks, const Texture& ns) : Bsdf(Tag::PhongBsdf, Type::Glossy), ks_(ks), ns_(ns) {}
+    sol_material flatten() const override;
+private:
+    const ColorTexture& ks_;
+    const Texture& ns_;
+};
+
+class MirrorBsdf final : public Bsdf {
+public:
+    explicit MirrorBsdf(const ColorTexture& ks) : Bsdf(Tag::MirrorBsdf, Type::Specular), ks_(ks) {}
+    sol_material flatten() const override;
+private:
+    const ColorTexture& ks_;
+};
+
+class GlassBsdf final : public Bsdf {
+public:
+    GlassBsdf(const ColorTexture& ks, const ColorTexture& kt, const Texture& eta)
+        : Bsdf(Tag::GlassBsdf, Type::Specular), ks_(ks), kt_(kt), eta_(eta) {}
+    sol_material flatten() const override;
+private:
+    const ColorTexture& ks_;
+    const ColorTexture& kt_;
+    const Texture& eta_;
+};
+
+} // namespace sol
+
+#endif

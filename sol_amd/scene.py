@@ -1,0 +1,176 @@
+"""Host-side scenes: thin Python handles over libsol_host.so (C++20 scene model, BVH builder,
+flattening). A Scene owns the C++ sol::Scene and its flat arrays."""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi as F
+
+
+def default_bvh_config(**overrides):
+    cfg = F.BvhConfig()
+    F.host.sol_default_bvh_config(C.byref(cfg))
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def default_sphere_field_params(**overrides):
+    p = F.SphereFieldParams()
+    F.host.sol_default_sphere_field_params(C.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def _array(ptr, count, dtype, shape=None):
+    if count == 0:
+        return np.zeros(shape or (0,), dtype=dtype)
+    a = np.ctypeslib.as_array(ptr, shape=(count,)).copy()
+    a = a.view(dtype) if a.dtype != dtype else a
+    return a.reshape(shape) if shape else a
+
+
+class Scene:
+    """A procedural or array-built scene with its BVH (sweep SAH + reinsertion)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def cornell_box(cls, aspect=1.0, bvh=None):
+        h = C.c_void_p()
+        F.check_host(F.host.sol_scene_cornell_box(aspect, C.byref(bvh or default_bvh_config()), C.byref(h)),
+                     "sol_scene_cornell_box")
+        return cls(h.value)
+
+    @classmethod
+    def sphere_field(cls, params=None, bvh=None, **kw):
+        p = params or default_sphere_field_params(**kw)
+        h = C.c_void_p()
+        F.check_host(F.host.sol_scene_sphere_field(C.byref(p), C.byref(bvh or default_bvh_config()), C.byref(h)),
+                     "sol_scene_sphere_field")
+        return cls(h.value)
+
+    @classmethod
+    def triangle_soup(cls, num_tris, seed=1, bvh=None):
+        h = C.c_void_p()
+        F.check_host(F.host.sol_scene_triangle_soup(num_tris, seed, C.byref(bvh or default_bvh_config()), C.byref(h)),
+                     "sol_scene_triangle_soup")
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, positions, normals, indices, tri_material, tri_light, materials, lights, camera, bvh=None):
+        positions = np.ascontiguousarray(positions, np.float32)
+        normals = np.ascontiguousarray(normals, np.float32)
+        indices = np.ascontiguousarray(indices, np.uint32)
+        tri_material = np.ascontiguousarray(tri_material, np.int32)
+        tri_light = np.ascontiguousarray(tri_light, np.int32)
+        mats = (F.Material * max(1, len(materials)))(*materials)
+        lts = (F.Light * max(1, len(lights)))(*lights)
+        mv = F.MeshView()
+        mv.num_vertices = positions.size // 3
+        mv.positions = positions.ctypes.data_as(C.POINTER(C.c_float))
+        mv.normals = normals.ctypes.data_as(C.POINTER(C.c_float))
+        mv.num_tris = indices.size // 3
+        mv.indices = indices.ctypes.data_as(C.POINTER(C.c_uint32))
+        mv.tri_material = tri_material.ctypes.data_as(C.POINTER(C.c_int32))
+        mv.tri_light = tri_light.ctypes.data_as(C.POINTER(C.c_int32))
+        mv.num_materials = len(materials)
+        mv.materials = mats
+        mv.num_lights = len(lights)
+        mv.lights = lts
+        mv.camera = camera
+        h = C.c_void_p()
+        F.check_host(F.host.sol_scene_from_arrays(C.byref(mv), C.byref(bvh or default_bvh_config()), C.byref(h)),
+                     "sol_scene_from_arrays")
+        return cls(h.value)
+
+    def close(self):
+        if self._h and self._h.value:
+            F.host.sol_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- flat views ---------------------------------------------------------------------------
+    def mesh_view(self):
+        mv = F.MeshView()
+        F.check_host(F.host.sol_scene_mesh(self._h, C.byref(mv)), "sol_scene_mesh")
+        return mv
+
+    def bvh_view(self):
+        bv = F.BvhView()
+        F.check_host(F.host.sol_scene_bvh(self._h, C.byref(bv)), "sol_scene_bvh")
+        return bv
+
+    def gpu_desc(self):
+        d = F.SceneDesc()
+        F.check_host(F.host.sol_scene_gpu_desc(self._h, C.byref(d)), "sol_scene_gpu_desc")
+        return d
+
+    def build_stats(self):
+        s = F.BuildStats()
+        F.check_host(F.host.sol_scene_build_stats(self._h, C.byref(s)), "sol_scene_build_stats")
+        return s
+
+    @property
+    def num_tris(self):
+        return self.mesh_view().num_tris
+
+    def materials(self):
+        mv = self.mesh_view()
+        return [mv.materials[i] for i in range(mv.num_materials)]
+
+    def lights(self):
+        mv = self.mesh_view()
+        return [mv.lights[i] for i in range(mv.num_lights)]
+
+    def set_materials(self, mats):
+        arr = (F.Material * len(mats))(*mats)
+        F.check_host(F.host.sol_scene_set_materials(self._h, arr, len(mats)), "sol_scene_set_materials")
+
+    def set_lights(self, lights):
+        arr = (F.Light * len(lights))(*lights)
+        F.check_host(F.host.sol_scene_set_lights(self._h, arr, len(lights)), "sol_scene_set_lights")
+
+    def arrays(self):
+        """Copies of the raw mesh arrays (original triangle order)."""
+        mv = self.mesh_view()
+        return {
+            "positions": _array(mv.positions, mv.num_vertices * 3, np.float32, (mv.num_vertices, 3)),
+            "normals": _array(mv.normals, mv.num_vertices * 3, np.float32, (mv.num_vertices, 3)),
+            "indices": _array(mv.indices, mv.num_tris * 3, np.uint32, (mv.num_tris, 3)),
+            "tri_material": _array(mv.tri_material, mv.num_tris, np.int32),
+            "tri_light": _array(mv.tri_light, mv.num_tris, np.int32),
+        }
+
+    def bvh_arrays(self):
+        bv = self.bvh_view()
+        nodes = np.ctypeslib.as_array(C.cast(bv.nodes, C.POINTER(C.c_uint32)), shape=(bv.num_nodes * 8,)).copy()
+        nodes = nodes.reshape(bv.num_nodes, 8)
+        bounds = nodes[:, :6].view(np.float32)
+        prims = _array(bv.prim_indices, bv.num_prims, np.uint32)
+        return bounds, nodes[:, 6].copy(), nodes[:, 7].copy(), prims
+
+
+def gen_rays(kind, n, seed=7, offset=1e-5):
+    """Ray batch of the intersection benchmark as a structured (n, 8) float32 array:
+    org xyz, tmin, dir xyz, tmax."""
+    out = np.empty((n, 8), np.float32)
+    F.check_host(F.host.sol_gen_rays(kind, n, seed, offset, out.ctypes.data), "sol_gen_rays")
+    return out
+
+
+def camera_basis(params):
+    cam = F.Camera()
+    F.host.sol_camera_from_params(C.byref(params), C.byref(cam))
+    return cam

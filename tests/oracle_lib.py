@@ -1,0 +1,115 @@
+"""Loads the CPU oracle (oracle/liboracle.so) for the tests. Test infrastructure only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from sol_amd import _ffi as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+if not os.path.exists(_PATH):
+    raise ImportError(f"{_PATH} is not built; run __graft_entry__.build() (make -C oracle)")
+lib = C.CDLL(_PATH)
+
+
+class OracleStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("extension_rays", "shadow_rays", "ext_inner_visits", "ext_tri_tests",
+                                           "shadow_inner_visits", "shadow_tri_tests", "bsdf_hits", "samples")]
+
+
+_P = C.c_void_p
+lib.oracle_scene_create.argtypes = [C.POINTER(F.MeshView), C.POINTER(F.BvhView), C.POINTER(_P)]
+lib.oracle_scene_destroy.argtypes = [_P]
+lib.oracle_scene_destroy.restype = None
+lib.oracle_scene_set_materials.argtypes = [_P, C.POINTER(F.Material), C.c_uint32]
+lib.oracle_scene_set_lights.argtypes = [_P, C.POINTER(F.Light), C.c_uint32]
+lib.oracle_intersect.argtypes = [_P, _P, C.c_uint64, C.c_int, C.c_int, _P, C.POINTER(OracleStats)]
+lib.oracle_render.argtypes = [_P, C.POINTER(F.PtConfig), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, _P, C.c_int,
+                              C.c_uint32, C.c_uint32, C.POINTER(OracleStats)]
+lib.oracle_pixel_seed.restype = C.c_uint32
+lib.oracle_pixel_seed.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+lib.oracle_pcg.restype = None
+lib.oracle_pcg.argtypes = [C.c_uint64, C.c_uint32, _P, _P]
+for name in ("oracle_sample_cosine_hemisphere",):
+    getattr(lib, name).argtypes = [C.c_float, C.c_float, _P, _P]
+    getattr(lib, name).restype = None
+lib.oracle_sample_cosine_power_hemisphere.argtypes = [C.c_float, C.c_float, C.c_float, _P, _P]
+lib.oracle_sample_cosine_power_hemisphere.restype = None
+lib.oracle_ortho_basis.argtypes = [_P, _P, _P]
+lib.oracle_ortho_basis.restype = None
+lib.oracle_ray_triangle.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
+lib.oracle_bsdf_sample.argtypes = [C.POINTER(F.Material), _P, _P, C.c_int, _P, _P, _P, _P, _P]
+lib.oracle_fresnel.restype = C.c_float
+lib.oracle_fresnel.argtypes = [C.c_float, C.c_float, C.c_float]
+
+
+def f32(*v):
+    return np.array(v, np.float32)
+
+
+class OracleScene:
+    """The oracle's copy of a sol_amd.Scene; own BVH unless use_product_bvh."""
+
+    def __init__(self, scene, use_product_bvh=False):
+        mv = scene.mesh_view()
+        bv = scene.bvh_view() if use_product_bvh else None
+        h = C.c_void_p()
+        if lib.oracle_scene_create(C.byref(mv), C.byref(bv) if bv is not None else None, C.byref(h)) != 0:
+            raise RuntimeError("oracle_scene_create failed")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.oracle_scene_destroy(self._h)
+
+    def set_materials(self, mats):
+        arr = (F.Material * len(mats))(*mats)
+        assert lib.oracle_scene_set_materials(self._h, arr, len(mats)) == 0
+
+    def set_lights(self, lights):
+        arr = (F.Light * len(lights))(*lights)
+        assert lib.oracle_scene_set_lights(self._h, arr, len(lights)) == 0
+
+    def intersect(self, rays, any_hit=False, brute_force=False, stats=None):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        out = np.empty(n, np.int32) if any_hit else np.empty((n, 4), np.float32)
+        st = stats if stats is not None else OracleStats()
+        assert lib.oracle_intersect(self._h, rays.ctypes.data, n, int(any_hit), int(brute_force), out.ctypes.data,
+                                    C.byref(st)) == 0
+        if any_hit:
+            return out
+        return out[:, 0].view(np.int32).copy(), out[:, 1].copy(), out[:, 2].copy(), out[:, 3].copy()
+
+    def render(self, image, sample_index, sample_count, config, threads=0, rows=(0, 0), stats=None):
+        assert image.dtype == np.float32 and image.flags.c_contiguous and image.shape[0] == 3
+        cfg = config.to_c()
+        st = stats if stats is not None else OracleStats()
+        assert lib.oracle_render(self._h, C.byref(cfg), image.shape[2], image.shape[1], sample_index, sample_count,
+                                 image.ctypes.data, threads, rows[0], rows[1], C.byref(st)) == 0
+        return st
+
+
+def pcg(seed, n):
+    u = np.empty(n, np.uint32)
+    f = np.empty(n, np.float32)
+    lib.oracle_pcg(seed, n, u.ctypes.data, f.ctypes.data)
+    return u, f
+
+
+def ray_triangle(org, d, v0, v1, v2):
+    t, u, v = (np.zeros(1, np.float32) for _ in range(3))
+    args = [f32(*a) for a in (org, d, v0, v1, v2)]   # kept alive across the call
+    hit = lib.oracle_ray_triangle(*(a.ctypes.data for a in args), t.ctypes.data, u.ctypes.data, v.ctypes.data)
+    return bool(hit), float(t[0]), float(u[0]), float(v[0])
+
+
+def bsdf_sample(mat, n, out, randoms, front=True):
+    n, out = f32(*n), f32(*out)
+    rnd = np.asarray(randoms, np.float32)
+    i3, rgb = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    pdf, cos = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    ok = lib.oracle_bsdf_sample(C.byref(mat), n.ctypes.data, out.ctypes.data, int(front), rnd.ctypes.data,
+                                i3.ctypes.data, pdf.ctypes.data, cos.ctypes.data, rgb.ctypes.data)
+    return bool(ok), i3, float(pdf[0]), float(cos[0]), rgb
