@@ -144,15 +144,17 @@ typedef struct sol_gpu_tri {
     int32_t pad1;
 } sol_gpu_tri;
 
-/* Shading record in leaf order: the three vertex normals, material and light of the triangle.
-   48 bytes. */
+/* Shading record in leaf order: the three vertex normals, the geometric normal
+   cross(v1 - v0, v2 - v0) (computed in fp32 without fused multiply-add), material and light of
+   the triangle. 64 bytes. */
 typedef struct sol_gpu_shade {
     float n0[3];
     float n1[3];
     float n2[3];
+    float face_n[3];
     int32_t material;  /* index into the material table, -1 = no BSDF */
     int32_t light;     /* index into the light table, -1 = not emissive */
-    int32_t pad;
+    int32_t pad[2];
 } sol_gpu_shade;
 
 #ifdef __cplusplus

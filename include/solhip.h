@@ -90,12 +90,11 @@ typedef struct solhip_stats {
     uint64_t samples;          /* paths started                                */
     uint64_t launches;         /* kernel launches (graph nodes count once each) */
     uint64_t graph_replays;
-    double extend_ms;          /* summed durations measured with HIP events (profiling on) */
-    double shade_ms;
-    double shadow_ms;
-    double regen_ms;
-    double other_ms;
-    uint64_t extend_launches;  /* launches covered by extend_ms                */
+    double trace_ms;           /* summed durations measured with HIP events (profiling on):  */
+    double sort_ms;            /*   traversal, hit sorting and shading kernels of the        */
+    double shade_ms;           /*   wavefront loop; traversal of solhip_intersect            */
+    double reserved_ms[2];
+    uint64_t trace_launches;   /* launches covered by trace_ms                 */
     double render_ms;          /* wall time of render calls (HIP events)       */
 } solhip_stats;
 
@@ -128,7 +127,8 @@ int solhip_device_query(solhip_ctx* ctx, solhip_device_info* out);
 
 int solhip_set_tuning(solhip_ctx* ctx, const solhip_tuning* t);
 int solhip_get_tuning(solhip_ctx* ctx, solhip_tuning* out);
-/* profiling != 0: record HIP events around each extend/shade/shadow launch into the stats. */
+/* profiling != 0: record HIP events around the traversal, sort and shade launches of the
+   wavefront loop (launched eagerly then, not from the captured graph) into the stats. */
 int solhip_set_profiling(solhip_ctx* ctx, int profiling);
 int solhip_get_stats(solhip_ctx* ctx, solhip_stats* out);
 int solhip_reset_stats(solhip_ctx* ctx);

@@ -63,8 +63,8 @@ class GpuTri(C.Structure):
 
 
 class GpuShade(C.Structure):
-    _fields_ = [("n0", C.c_float * 3), ("n1", C.c_float * 3), ("n2", C.c_float * 3), ("material", C.c_int32),
-                ("light", C.c_int32), ("pad", C.c_int32)]
+    _fields_ = [("n0", C.c_float * 3), ("n1", C.c_float * 3), ("n2", C.c_float * 3), ("face_n", C.c_float * 3),
+                ("material", C.c_int32), ("light", C.c_int32), ("pad", C.c_int32 * 2)]
 
 
 class BvhConfig(C.Structure):
@@ -112,9 +112,9 @@ class Tuning(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("samples", C.c_uint64),
-                ("launches", C.c_uint64), ("graph_replays", C.c_uint64), ("extend_ms", C.c_double),
-                ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("regen_ms", C.c_double),
-                ("other_ms", C.c_double), ("extend_launches", C.c_uint64), ("render_ms", C.c_double)]
+                ("launches", C.c_uint64), ("graph_replays", C.c_uint64), ("trace_ms", C.c_double),
+                ("sort_ms", C.c_double), ("shade_ms", C.c_double), ("reserved_ms", C.c_double * 2),
+                ("trace_launches", C.c_uint64), ("render_ms", C.c_double)]
 
 
 class DeviceInfo(C.Structure):
@@ -128,7 +128,7 @@ class BatchJob(C.Structure):
 
 
 assert C.sizeof(Material) == 64 and C.sizeof(Light) == 80 and C.sizeof(GpuNode) == 64
-assert C.sizeof(GpuTri) == 48 and C.sizeof(GpuShade) == 48 and C.sizeof(Ray) == 32 and C.sizeof(Hit) == 16
+assert C.sizeof(GpuTri) == 48 and C.sizeof(GpuShade) == 64 and C.sizeof(Ray) == 32 and C.sizeof(Hit) == 16
 
 
 def _load(name):

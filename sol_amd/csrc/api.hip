@@ -26,8 +26,10 @@ int fail(int code, const std::string& msg) {
 #define HIP_TRY(expr)                                                                            \
     do {                                                                                         \
         hipError_t e__ = (expr);                                                                 \
-        if (e__ != hipSuccess)                                                                   \
+        if (e__ != hipSuccess) {                                                                 \
+            (void)hipGetLastError();                                                             \
             return fail(SOLHIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e__));        \
+        }                                                                                        \
     } while (0)
 
 bool valid_shape_values(uint32_t block, uint32_t lds) {
@@ -114,8 +116,8 @@ int solhip_create(int device, solhip_ctx** out) {
     return SOLHIP_OK;
 }
 
-void solhip_destroy(solhip_ctx* ctx) {
-    if (!ctx) return;
+namespace {
+void destroy_context(solhip_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->release_wavefront();
@@ -126,6 +128,18 @@ void solhip_destroy(solhip_ctx* ctx) {
     (void)hipEventDestroy(ctx->ev_end);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+} // namespace
+
+// A context outlives its scenes: destroying it while scenes are alive defers the release to the
+// destruction of the last scene.
+void solhip_destroy(solhip_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->live_scenes > 0) {
+        ctx->destroy_requested = true;
+        return;
+    }
+    destroy_context(ctx);
 }
 
 int solhip_device_query(solhip_ctx* ctx, solhip_device_info* out) {
@@ -233,6 +247,7 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     v.num_nodes = d->num_nodes;
     v.num_tris = d->num_tris;
     v.camera = d->camera;
+    ctx->live_scenes += 1;
     *out = s.release();
     return SOLHIP_OK;
 }
@@ -246,7 +261,10 @@ void solhip_scene_destroy(solhip_scene* s) {
     (void)hipFree(s->shade);
     (void)hipFree(s->materials);
     (void)hipFree(s->lights);
+    solhip_ctx* ctx = static_cast<solhip_ctx*>(s->ctx);
     delete s;
+    ctx->live_scenes -= 1;
+    if (ctx->live_scenes == 0 && ctx->destroy_requested) destroy_context(ctx);
 }
 
 int solhip_scene_set_materials(solhip_scene* s, const sol_material* m, uint32_t n) {
@@ -297,8 +315,8 @@ int solhip_intersect(solhip_scene* s, const sol_ray* rays, uint64_t n, int any_h
     if (ctx->profiling) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
-        ctx->stats.extend_ms += ms;
-        ctx->stats.extend_launches += 1;
+        ctx->stats.trace_ms += ms;
+        ctx->stats.trace_launches += 1;
     }
     (any_hit ? ctx->stats.shadow_rays : ctx->stats.extension_rays) += n;
     ctx->stats.launches += 1;

@@ -162,6 +162,7 @@ template <int BLOCK, int LDS>
 hipError_t intersect_impl(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
                           int32_t* occluded, unsigned long long* counter, const TraversalScratch& sc, int grid,
                           hipStream_t stream) {
+    (void)hipGetLastError();
     if (any_hit)
         hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, rays, n, hits,
                            occluded, counter, sc);
@@ -175,6 +176,7 @@ template <int BLOCK, int LDS>
 hipError_t megakernel_impl(const SceneView& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si,
                            uint32_t sc_count, float* accum, unsigned long long* counter, unsigned long long* stats,
                            const TraversalScratch& sc, int grid, hipStream_t stream) {
+    (void)hipGetLastError();
     hipLaunchKernelGGL((megakernel<BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, cfg, w, h, si, sc_count, accum,
                        counter, stats, sc);
     return hipGetLastError();
@@ -211,6 +213,7 @@ hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint3
 }
 
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t stream) {
+    (void)hipGetLastError();
     hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, p, n);
     return hipGetLastError();
 }

@@ -153,7 +153,7 @@ SOL_DEV Basis ortho_basis(V3 n) {
 struct SceneView {
     const float4* nodes;   // 4 x float4 per node (sol_gpu_node)
     const float4* tris;    // 3 x float4 per triangle (sol_gpu_tri), leaf order
-    const float4* shade;   // 3 x float4 per triangle (sol_gpu_shade), leaf order
+    const float4* shade;   // 4 x float4 per triangle (sol_gpu_shade), leaf order
     const sol_material* materials;
     const sol_light* lights;
     uint32_t num_lights;

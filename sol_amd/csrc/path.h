@@ -21,14 +21,13 @@ struct Surface {
 };
 
 // Hit decoding (triangle_mesh.cpp:78-104).
+// The geometric normal cross(v1 - v0, v2 - v0) comes precomputed (same fp32 operations) in the
+// shading record.
 SOL_DEV Surface decode_hit(const SceneView& s, V3 org, V3 dir, int slot, float t, float u, float v) {
-    const float4* sp = s.shade + slot * 3;
-    float4 a = sp[0], b = sp[1], c = sp[2];
-    const float4* tp = s.tris + slot * 3;
-    float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-    V3 v0{p0.x, p0.y, p0.z}, v1{p1.x, p1.y, p1.z}, v2{p2.x, p2.y, p2.z};
+    const float4* sp = s.shade + slot * 4;
+    float4 a = sp[0], b = sp[1], c = sp[2], d = sp[3];
     Surface sf;
-    V3 face_n = cross(v1 - v0, v2 - v0);
+    V3 face_n{c.y, c.z, c.w};
     V3 normal = lerp3(V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, u, v);
     sf.front = dot(face_n, dir) < 0.0f;
     if (!sf.front) {
@@ -38,8 +37,8 @@ SOL_DEV Surface decode_hit(const SceneView& s, V3 org, V3 dir, int slot, float t
     sf.face_n = face_n;
     sf.point = org + dir * t;
     sf.n = normalize(normal);
-    sf.mat = __float_as_int(c.y);
-    sf.light = __float_as_int(c.z);
+    sf.mat = __float_as_int(d.x);
+    sf.light = __float_as_int(d.y);
     sf.su = u;
     sf.sv = v;
     return sf;

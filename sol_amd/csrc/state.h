@@ -39,9 +39,9 @@ struct WavefrontState;  // defined in wavefront.hip
 
 inline solhip_tuning default_tuning() {
     solhip_tuning t{};
-    t.block_size = 64;
+    t.block_size = 256;
     t.lds_stack = 16;
-    t.path_slots = 1u << 20;
+    t.path_slots = 1u << 23;
     t.tile_w = 32;
     t.tile_h = 32;
     t.graph_iters = 16;
@@ -59,6 +59,8 @@ struct Context {
     TraversalScratch scratch{};
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     WavefrontState* wave = nullptr;
+    int live_scenes = 0;             // scenes uploaded and not yet destroyed
+    bool destroy_requested = false;  // solhip_destroy was called while scenes were alive
 
     int ensure_scratch(int threads);
     LaunchShape shape(int block, int lds) const;

@@ -17,8 +17,8 @@
 
 namespace solhip {
 
-// Maximum number of top nodes copied to LDS (8 KB).
-constexpr int kTopNodesMax = 127;
+// Maximum number of top nodes copied to LDS (4 KB: six levels).
+constexpr int kTopNodesMax = 63;
 
 struct HitRecord {
     int slot;   // leaf-order triangle index, -1 = miss

@@ -13,6 +13,9 @@
 
 namespace solhip {
 
+// Workgroup size of the sort, shade and start kernels.
+constexpr int kWaveBlock = 512;
+
 enum : int {
     kQueueDiffuse = 0,
     kQueuePhong = 1,
@@ -26,57 +29,68 @@ enum : int {
     kCntExtend1 = 1,    // extension queue 1 length
     kCntShade0 = 2,     // shade queue lengths (3)
     kCntShadow = 5,     // shadow queue length
-    kCntRegen = 6,      // terminated-path queue length
-    kCntWork = 8,       // per-kernel work-fetch counters: extend, shade x3, shadow, regen
+    kCntTraceWork = 8,  // work-fetch counter of the traversal kernel
     kNumWaveCounters = 16
 };
 
+// One bounce of all paths in flight is four launches: trace (closest hit of the extension rays
+// and any hit of the previous bounce's shadow rays), reset, sort (hit decoding, emission, material
+// queues) and shade (one kernel per material queue: NEE, Russian roulette, BSDF sampling).
+// Per-path state is one 64-byte record per slot (random access costs one cache line); the data
+// that flows from one kernel to the next travels in queue entries written at compacted positions
+// (coalesced): extension rays (32 B), hits (16 B), shade entries (hit record, 64 B), shadow rays
+// (48 B). Appends are compacted per workgroup (one atomic per 256 entries and queue).
+// A path that ends writes its radiance to its sample's entry of the sample buffer and the slot
+// starts the next work item at once (in the kernel that ended it); a shadow ray still in flight
+// for the ended path adds its contribution to that sample entry instead of the path record.
+//
+// path record (4 x float4):   [0] throughput xyz, pdf of the previous bounce
+//                             [1] radiance xyz, depth (int bits)
+//                             [2] PCG state lo, hi, work item lo, hi (uint bits)
+//                             [3] scene index (uint bits), unused
+// extension entry (2 x float4): [0] org xyz, tmin  [1] dir xyz, slot
+// shade entry (4 x float4):   [0] point xyz, material | front bit  [1] shading normal xyz, slot
+//                             [2] geometric normal xyz, -           [3] ray direction xyz, -
+// shadow entry (3 x float4):  [0] org xyz, target  [1] dir xyz, item hi  [2] contribution xyz, item lo
+//                             target = slot, or kSampleTarget | 0 when the path has ended (the
+//                             contribution then goes to the sample entry of the item)
 struct WaveBuf {
     uint32_t slots;
-    // path state [slots]
-    float *tx, *ty, *tz;         // throughput
-    float *cx, *cy, *cz;         // radiance gathered so far
-    float* pdf;                  // pdf of the previous bounce (0 after a specular one)
-    unsigned long long* rng;     // PCG state
-    unsigned long long* item;    // global work item
-    uint32_t* depth;             // bounces done
-    uint32_t* scene;             // scene index (batched runs)
-    // hit record [slots], written by extend, read by shade
-    float *hpx, *hpy, *hpz;      // point
-    float *hnx, *hny, *hnz;      // shading normal
-    float *hfx, *hfy, *hfz;      // geometric normal (flipped to the ray side)
-    float *hdx, *hdy, *hdz;      // ray direction
-    int* hmat;                   // material index, bit 30 set for a front-side hit
-    // extension queues [2][slots]: org, dir, tmin, slot
-    float* eq[2][7];
-    uint32_t* es[2];
-    // shade queues [3][slots]: slot
-    uint32_t* mq[kNumShadeQueues];
-    // shadow queue [slots]: org, dir, contribution, slot
-    float* sq[9];
-    uint32_t* ss;
-    // terminated paths [slots]
-    uint32_t* rq;
-    unsigned* counts;            // kNumWaveCounters
-    unsigned long long* items;   // [0] next global item, [1] chunk end, [2] chunk begin
-    unsigned long long* stats;   // [0] extension rays [1] shadow rays [2] samples
-    float* samples;              // 3 planes of sample_capacity floats
+    float4* path;                     // [slots * 4]
+    float4* hits;                     // [slots]: closest hit of extension entry i: leaf-order
+                                      // triangle (int bits, -1 = miss), t, u, v
+    float4* eq[2];                    // [slots * 2] each
+    float4* mq[kNumShadeQueues];      // [slots * 4] each
+    float4* sq;                       // [slots * 3]
+    unsigned* counts;                 // kNumWaveCounters
+    unsigned long long* items;        // [0] next global item, [1] chunk end, [2] chunk begin
+    unsigned long long* stats;        // [0] extension rays [1] shadow rays [2] samples
+    float4* samples;                  // [sample_capacity]: radiance of each item of the chunk
     unsigned long long sample_capacity;
 };
 
+// Jobs are images; a job's work items are ordered band by band (tile_h rows), tile by tile
+// (tile_w columns) inside a band, sample by sample inside a tile, pixel by pixel (row-major)
+// inside a tile. Edge bands and tiles are smaller. Items of a job are numbered from job_item0.
 struct WaveTables {
     const SceneView* scenes;          // per scene
+    uint32_t num_jobs;
+    const unsigned long long* job_item0;    // num_jobs + 1 entries (last = total items)
+    const unsigned long long* job_sample0;  // absolute sample index of the job's first sample
     const uint32_t* job_scene;
     float* const* job_accum;          // device image of each job
     const uint32_t* job_w;
     const uint32_t* job_h;
+    const uint32_t* job_spp;
+    const uint32_t* job_tw;
+    const uint32_t* job_th;
+    // tiles, for the per-pixel reduction: the same order as the items
     const uint32_t* tile_job;
     const uint32_t* tile_x0;
     const uint32_t* tile_y0;
     const uint32_t* tile_w;
     const uint32_t* tile_h;
     const unsigned long long* tile_item0;   // first global item, plus one entry = total items
-    const unsigned long long* tile_sample0; // absolute sample index of the tile's first sample
     uint32_t num_tiles;
 };
 

@@ -1,12 +1,19 @@
-// Wavefront kernels: extend (closest hit + hit decoding + emission + material sort), shade (one
-// kernel per material queue: NEE, Russian roulette, BSDF sampling), shadow (any hit), regen
-// (retire finished paths, start new ones from the global work queue), reduce (per-pixel sums in
-// sample order). Queue appends are compacted per wave: a ballot gives each lane its rank, one
-// lane reserves the wave's slots with a single atomic.
+// Wavefront kernels.
+//   k_trace  closest hit of the extension rays (hit written in place) and any hit of the previous
+//            bounce's shadow rays (unoccluded contribution added to the path or its sample);
+//            traversal only, work fetched by waves in shrinking chunks (guided self-scheduling).
+//   k_sort   hit decoding, emission, material queues; ended paths write their sample and the slot
+//            starts the next work item.
+//   k_shade  one kernel per material queue: NEE, Russian roulette, BSDF sampling.
+//   k_start  fills every slot at the beginning of a chunk of work.
+//   k_reduce per-pixel sums of the chunk's samples in sample order.
+// Appends are compacted per workgroup: ballots give each lane its rank in its wave, the wave
+// counts are scanned in LDS and one lane per queue reserves the workgroup's entries with a single
+// device atomic (device-scope atomics are slow on a multi-XCD part: few of them, never per lane).
 //
 // The per-bounce arithmetic is the shared code of path.h, in the order of trace_path
-// (src/algorithms/path_tracer.cpp:31-120 of sol): the emission of hit k is added in extend(k),
-// the NEE contribution of vertex k in shadow(k), before extend(k+1).
+// (src/algorithms/path_tracer.cpp:31-120 of sol): NEE of vertex k is added by trace(k+1), before
+// the emission of hit k+1 is added by sort(k+1).
 #include "launch.h"
 #include "path.h"
 #include "traverse.h"
@@ -17,26 +24,22 @@ namespace solhip {
 
 namespace {
 
-constexpr int kFrontBit = 1 << 30;
+constexpr uint32_t kFrontBit = 1u << 30;
+constexpr uint32_t kSampleTarget = 0xFFFFFFFFu;
+constexpr int kBlock = kWaveBlock;
+constexpr uint32_t kHole = 0xFFFFFFFFu;     // slot field of an unused extension entry
+constexpr int kWaves = kBlock / 64;
 
-SOL_DEV unsigned long long wave_grab64(unsigned* counter, unsigned chunk) {
-    unsigned base = 0;
-    if (__lane_id() == 0) base = atomicAdd(counter, chunk);
-    return __shfl(base, 0);
+SOL_DEV uint32_t as_u(float f) { return __float_as_uint(f); }
+SOL_DEV float as_f(uint32_t u) { return __uint_as_float(u); }
+SOL_DEV float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+SOL_DEV V3 xyz(float4 v) { return V3{v.x, v.y, v.z}; }
+SOL_DEV unsigned long long item_of(float4 p2) {
+    return static_cast<unsigned long long>(as_u(p2.z)) | (static_cast<unsigned long long>(as_u(p2.w)) << 32);
 }
-
-// Wave-compacted append: returns this lane's slot in the queue (or -1 if !pred).
-SOL_DEV int wave_append(unsigned* count, bool pred) {
-    unsigned long long mask = __ballot(pred);
-    if (mask == 0) return -1;
+SOL_DEV unsigned long long lanes_below() {
     int lane = __lane_id();
-    int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(count, static_cast<unsigned>(__popcll(mask)));
-    base = __shfl(base, leader);
-    if (!pred) return -1;
-    unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-    return static_cast<int>(base + static_cast<unsigned>(__popcll(below)));
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
 }
 
 SOL_DEV void wave_add_stat(unsigned long long* counter, unsigned long long v) {
@@ -44,11 +47,98 @@ SOL_DEV void wave_add_stat(unsigned long long* counter, unsigned long long v) {
     if (__lane_id() == 0 && v) atomicAdd(counter, v);
 }
 
+// Workgroup-wide queue reservations of one round (one entry per thread). Each thread belongs to
+// at most one of NQ plain queues (`dest`), may continue its path (`cont`: one extension entry) or
+// retire it (`retire`). Retired paths take the next work items in thread order; every retired
+// path also gets an extension entry after the round's `cont` entries, used for the new path if
+// an item was left and marked as a hole (slot kHole) otherwise - holes only appear when the
+// chunk runs out of items. All reservations are issued at once by different lanes: two barriers
+// and one device atomic per non-empty queue per round. The LDS block is double-buffered by round
+// parity, so no third barrier is needed before the next round overwrites it.
+template <int NQ>
+struct RoundLds {
+    unsigned wc[NQ + 2][kWaves];    // per-wave counts: plain queues, cont, retire
+    unsigned pre[NQ + 2][kWaves];   // their exclusive prefix sums over the waves
+    unsigned base[NQ];
+    unsigned ext_base;
+    unsigned n_cont;
+    unsigned n_ret;
+    unsigned long long item_base;
+};
+
+struct RoundPlan {
+    int pos;                     // position in the plain queue, -1 if none
+    int ext_pos;                 // position in the extension queue (cont or retired), -1 if none
+    bool start;                  // the retired path got a new work item
+    unsigned long long item;     // that item
+};
+
+template <int NQ>
+SOL_DEV RoundPlan plan_round(RoundLds<NQ>& lds, unsigned* const* qcounters, unsigned* ext_counter,
+                             unsigned long long* items, unsigned long long items_end, int dest, bool cont,
+                             bool retire) {
+    const int wave = threadIdx.x >> 6;
+    const bool lane0 = __lane_id() == 0;
+    const unsigned long long below = lanes_below();
+    unsigned rank = 0;
+#pragma unroll
+    for (int d = 0; d < NQ; ++d) {
+        unsigned long long m = __ballot(dest == d);
+        if (lane0) lds.wc[d][wave] = static_cast<unsigned>(__popcll(m));
+        if (dest == d) rank = static_cast<unsigned>(__popcll(m & below));
+    }
+    unsigned long long mc = __ballot(cont), mr = __ballot(retire);
+    if (lane0) {
+        lds.wc[NQ][wave] = static_cast<unsigned>(__popcll(mc));
+        lds.wc[NQ + 1][wave] = static_cast<unsigned>(__popcll(mr));
+    }
+    unsigned rank_c = static_cast<unsigned>(__popcll(mc & below));
+    unsigned rank_r = static_cast<unsigned>(__popcll(mr & below));
+    __syncthreads();
+    if (threadIdx.x < NQ + 2) {
+        const int d = threadIdx.x;
+        unsigned run = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            lds.pre[d][w] = run;
+            run += lds.wc[d][w];
+        }
+        if (d < NQ) {
+            lds.base[d] = run ? atomicAdd(qcounters[d], run) : 0u;
+        } else if (d == NQ) {
+            // extension entries: the cont lanes, then one per retired lane
+            unsigned n_ret = 0;
+            for (int w = 0; w < kWaves; ++w) n_ret += lds.wc[NQ + 1][w];
+            unsigned n = run + n_ret;
+            lds.ext_base = n ? atomicAdd(ext_counter, n) : 0u;
+            lds.n_cont = run;
+        } else {
+            lds.item_base = run ? atomicAdd(items, static_cast<unsigned long long>(run)) : 0ull;
+            lds.n_ret = run;
+        }
+    }
+    __syncthreads();
+    RoundPlan p;
+    p.pos = dest >= 0 ? static_cast<int>(lds.base[dest] + lds.pre[dest][wave] + rank) : -1;
+    const unsigned ridx = lds.pre[NQ + 1][wave] + rank_r;
+    const unsigned long long ib = lds.item_base;
+    const unsigned long long avail = ib < items_end ? items_end - ib : 0ull;
+    p.start = retire && ridx < avail;
+    p.item = ib + ridx;
+    p.ext_pos = cont ? static_cast<int>(lds.ext_base + lds.pre[NQ][wave] + rank_c)
+                     : (retire ? static_cast<int>(lds.ext_base + lds.n_cont + ridx) : -1);
+    return p;
+}
+
+// A retired lane whose extension entry stays unused marks it as a hole (trace and sort skip it).
+SOL_DEV void write_hole(const WaveBuf& b, int nxt, int ext_pos) {
+    b.eq[nxt][2 * ext_pos + 1] = make_float4(0.0f, 0.0f, 0.0f, as_f(kHole));
+}
+
 template <bool MULTI>
 SOL_DEV const SceneView& scene_of(const SceneView& single, const WaveTables& t, const WaveBuf& b, uint32_t slot,
                                   SceneView& tmp) {
     if (!MULTI) return single;
-    tmp = t.scenes[b.scene[slot]];
+    tmp = t.scenes[as_u(b.path[slot * 4 + 3].x)];
     return tmp;
 }
 
@@ -56,122 +146,287 @@ SOL_DEV int queue_of(int type) {
     return type == SOL_BSDF_DIFFUSE ? kQueueDiffuse : (type == SOL_BSDF_PHONG ? kQueuePhong : kQueueSpecular);
 }
 
+// ---- work items ------------------------------------------------------------------------------------
+
+SOL_DEV uint32_t find_job(const WaveTables& t, unsigned long long item) {
+    uint32_t lo = 0, hi = t.num_jobs;   // job_item0[lo] <= item < job_item0[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (t.job_item0[mid] <= item) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Pixel and sample of a work item (the order of WaveTables). A job holds fewer than 2^32 items.
+SOL_DEV void decode_item(const WaveTables& t, unsigned long long item, uint32_t& job, uint32_t& x, uint32_t& y,
+                         unsigned long long& sample) {
+    job = find_job(t, item);
+    uint32_t local = static_cast<uint32_t>(item - t.job_item0[job]);
+    uint32_t w = t.job_w[job], h = t.job_h[job], tw = t.job_tw[job], th = t.job_th[job], spp = t.job_spp[job];
+    uint32_t bands = (h + th - 1) / th;
+    uint32_t band_items = w * th * spp;
+    uint32_t band = min(local / band_items, bands - 1);
+    uint32_t q = local - band * band_items;
+    uint32_t bh = min(th, h - band * th);
+    uint32_t tiles_x = (w + tw - 1) / tw;
+    uint32_t tile_items = tw * bh * spp;
+    uint32_t tile = min(q / tile_items, tiles_x - 1);
+    uint32_t r = q - tile * tile_items;
+    uint32_t tw2 = min(tw, w - tile * tw);
+    uint32_t per_sample = tw2 * bh;
+    uint32_t s = r / per_sample;
+    uint32_t pix = r - s * per_sample;
+    x = tile * tw + pix % tw2;
+    y = band * th + pix / tw2;
+    sample = t.job_sample0[job] + s;
+}
+
+// Path start of one lane: seed, camera ray, fresh path record.
+SOL_DEV void init_path(const WaveTables& t, const WaveBuf& b, unsigned long long item, uint32_t slot, V3& org,
+                       V3& dir) {
+    uint32_t job, x, y;
+    unsigned long long sample;
+    decode_item(t, item, job, x, y, sample);
+    uint32_t sc = t.job_scene[job];
+    const sol_camera cam = t.scenes[sc].camera;
+    Pcg rng;
+    rng.seed(pixel_seed(sample, x, y));
+    camera_ray(cam, x, y, t.job_w[job], t.job_h[job], rng, org, dir);
+    float4* p = b.path + slot * 4;
+    p[0] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    p[1] = make_float4(0.0f, 0.0f, 0.0f, as_f(0u));
+    p[2] = make_float4(as_f(static_cast<uint32_t>(rng.state)), as_f(static_cast<uint32_t>(rng.state >> 32)),
+                       as_f(static_cast<uint32_t>(item)), as_f(static_cast<uint32_t>(item >> 32)));
+    p[3] = make_float4(as_f(sc), 0.0f, 0.0f, 0.0f);
+}
+
+// The radiance of an ended path goes to its sample entry.
+SOL_DEV void retire(const WaveBuf& b, float4 p1, float4 p2) {
+    b.samples[item_of(p2) - b.items[2]] = make_float4(p1.x, p1.y, p1.z, 0.0f);
+}
+
 template <int BLOCK>
-SOL_DEV Stack make_stack_w(int* lds_stack, const TraversalScratch& sc) {
+SOL_DEV Stack make_stack_w(int* lds_stack, const TraversalScratch& sc, int lds_entries) {
     int gtid = blockIdx.x * BLOCK + threadIdx.x;
     Stack st;
     st.lds = lds_stack + threadIdx.x;
     st.spill = sc.spill + gtid;
     st.spill_stride = sc.max_threads;
-    st.capacity = 0;
+    st.capacity = lds_entries + sc.spill_entries;
     st.overflow = sc.overflow;
     return st;
 }
 
-// ---- extend ----------------------------------------------------------------------------------------
+// ---- trace -----------------------------------------------------------------------------------------
+
+// Guided self-scheduling: a wave takes (items left) / (4 x waves) items, a multiple of 64 in
+// [64, 1024], so that the launch ends with small chunks and starts with few atomics.
+SOL_DEV unsigned wave_fetch(unsigned* counter, unsigned total, unsigned waves, unsigned last, unsigned& len) {
+    unsigned left = total > last ? total - last : 0;
+    unsigned chunk = (left / (4 * waves)) & ~63u;
+    chunk = chunk < 64 ? 64 : (chunk > 1024 ? 1024 : chunk);
+    unsigned base = 0;
+    if (__lane_id() == 0) base = atomicAdd(counter, chunk);
+    base = __shfl(base, 0);
+    len = chunk;
+    return base;
+}
 
 template <int BLOCK, int LDS, bool MULTI>
-__global__ __launch_bounds__(BLOCK) void k_extend(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
-                                                  int cur, TraversalScratch sc) {
+__global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
+                                                 int cur, TraversalScratch sc) {
     __shared__ float4 lds_nodes[kTopNodesMax * 4];
     __shared__ int lds_stack[LDS * BLOCK];
+    const unsigned n_ext = b.counts[kCntExtend0 + cur];
+    const unsigned n_shadow = b.counts[kCntShadow];
+    const unsigned total = n_ext + n_shadow;
+    if (total == 0) return;
     if (!MULTI) load_top_nodes<BLOCK>(single, lds_nodes);
-    Stack st = make_stack_w<BLOCK>(lds_stack, sc);
-    st.capacity = LDS + sc.spill_entries;
-    const unsigned count = b.counts[kCntExtend0 + cur];
-    const float* const* q = b.eq[cur];
-    unsigned processed = 0;
+    Stack st = make_stack_w<BLOCK>(lds_stack, sc, LDS);
+    const unsigned waves = gridDim.x * (BLOCK / 64);
+    const float4* q = b.eq[cur];
+    unsigned n_closest = 0, n_any = 0, last = 0;
     while (true) {
-        unsigned base = static_cast<unsigned>(wave_grab64(&b.counts[kCntWork + 0], 64));
-        if (base >= count) break;
-        unsigned i = base + __lane_id();
-        bool valid = i < count;
-        int dest = -1;  // 0..2 shade queue, 3 terminated
-        uint32_t slot = 0;
-        if (valid) {
-            ++processed;
-            slot = b.es[cur][i];
-            V3 org{q[0][i], q[1][i], q[2][i]};
-            V3 dir{q[3][i], q[4][i], q[5][i]};
-            float tmin = q[6][i];
-            SceneView tmp;
-            const SceneView& s = scene_of<MULTI>(single, tabs, b, slot, tmp);
-            RayPre r = make_ray_pre(org, dir);
-            HitRecord hit;
-            bool found = MULTI ? traverse<false, BLOCK, LDS>(s, s.nodes, st, r, tmin, FLT_MAX, hit)
-                               : traverse<false, BLOCK, LDS>(s, lds_nodes, st, r, tmin, FLT_MAX, hit);
-            if (!found) {
-                dest = 3;
+        unsigned len;
+        unsigned base = wave_fetch(&b.counts[kCntTraceWork], total, waves, last, len);
+        if (base >= total) break;
+        last = base;
+        unsigned end = min(base + len, total);
+        for (unsigned c = base; c < end; c += 64) {
+            unsigned i = c + __lane_id();
+            if (i >= end) continue;
+            if (i < n_ext) {
+                float4 e0 = q[2 * i], e1 = q[2 * i + 1];
+                if (as_u(e1.w) == kHole) continue;
+                ++n_closest;
+                SceneView tmp;
+                const SceneView& s = scene_of<MULTI>(single, tabs, b, as_u(e1.w), tmp);
+                RayPre r = make_ray_pre(xyz(e0), xyz(e1));
+                HitRecord hit;
+                bool found = MULTI ? traverse<false, BLOCK, LDS>(s, s.nodes, st, r, e0.w, FLT_MAX, hit)
+                                   : traverse<false, BLOCK, LDS>(s, lds_nodes, st, r, e0.w, FLT_MAX, hit);
+                b.hits[i] = found ? make_float4(as_f(static_cast<uint32_t>(hit.slot)), hit.t, hit.u, hit.v)
+                                  : make_float4(as_f(0xFFFFFFFFu), 0.0f, 0.0f, 0.0f);
             } else {
-                Surface sf = decode_hit(s, org, dir, hit.slot, hit.t, hit.u, hit.v);
-                if (sf.light >= 0 && sf.front) {
-                    V3 T{b.tx[slot], b.ty[slot], b.tz[slot]};
-                    V3 C{b.cx[slot], b.cy[slot], b.cz[slot]};
-                    V3 Le;
-                    float w;
-                    C = C + hit_emission(s, cfg.flags, sf, dir, hit.t, b.pdf[slot], 1.0f / static_cast<float>(s.num_lights),
-                                         T, Le, w);
-                    b.cx[slot] = C.x;
-                    b.cy[slot] = C.y;
-                    b.cz[slot] = C.z;
+                ++n_any;
+                const float4* e = b.sq + 3 * (i - n_ext);
+                float4 e0 = e[0], e1 = e[1];
+                uint32_t target = as_u(e0.w);
+                SceneView sv;
+                if (MULTI) {
+                    if (target == kSampleTarget) {
+                        unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) | as_u(e[2].w);
+                        sv = tabs.scenes[tabs.job_scene[find_job(tabs, item)]];
+                    } else {
+                        sv = tabs.scenes[as_u(b.path[target * 4 + 3].x)];
+                    }
                 }
-                if (sf.mat < 0) {
-                    dest = 3;
-                } else {
-                    b.hpx[slot] = sf.point.x; b.hpy[slot] = sf.point.y; b.hpz[slot] = sf.point.z;
-                    b.hnx[slot] = sf.n.x; b.hny[slot] = sf.n.y; b.hnz[slot] = sf.n.z;
-                    b.hfx[slot] = sf.face_n.x; b.hfy[slot] = sf.face_n.y; b.hfz[slot] = sf.face_n.z;
-                    b.hdx[slot] = dir.x; b.hdy[slot] = dir.y; b.hdz[slot] = dir.z;
-                    b.hmat[slot] = sf.mat | (sf.front ? kFrontBit : 0);
-                    dest = queue_of(s.materials[sf.mat].type);
+                const SceneView& s = MULTI ? sv : single;
+                RayPre r = make_ray_pre(xyz(e0), xyz(e1));
+                HitRecord unused;
+                bool occluded = MULTI ? traverse<true, BLOCK, LDS>(s, s.nodes, st, r, cfg.ray_offset, 1.0f - cfg.ray_offset, unused)
+                                      : traverse<true, BLOCK, LDS>(s, lds_nodes, st, r, cfg.ray_offset, 1.0f - cfg.ray_offset, unused);
+                if (!occluded) {
+                    float4 ct = e[2];
+                    if (target == kSampleTarget) {
+                        unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) | as_u(ct.w);
+                        float4* d = b.samples + (item - b.items[2]);
+                        float4 v = *d;
+                        *d = make_float4(v.x + ct.x, v.y + ct.y, v.z + ct.z, 0.0f);
+                    } else {
+                        float4 p1 = b.path[target * 4 + 1];
+                        b.path[target * 4 + 1] = make_float4(p1.x + ct.x, p1.y + ct.y, p1.z + ct.z, p1.w);
+                    }
                 }
             }
         }
-        for (int qi = 0; qi < kNumShadeQueues; ++qi) {
-            int at = wave_append(&b.counts[kCntShade0 + qi], dest == qi);
-            if (at >= 0) b.mq[qi][at] = slot;
-        }
-        int at = wave_append(&b.counts[kCntRegen], dest == 3);
-        if (at >= 0) b.rq[at] = slot;
     }
-    wave_add_stat(&b.stats[0], processed);
+    wave_add_stat(&b.stats[0], n_closest);
+    wave_add_stat(&b.stats[1], n_any);
+}
+
+// ---- sort ------------------------------------------------------------------------------------------
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_sort(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
+                                                 int cur) {
+    __shared__ RoundLds<kNumShadeQueues> lds[2];
+    const unsigned count = b.counts[kCntExtend0 + cur];
+    const int nxt = 1 - cur;
+    unsigned* qcnt[kNumShadeQueues] = {&b.counts[kCntShade0], &b.counts[kCntShade0 + 1], &b.counts[kCntShade0 + 2]};
+    const float4* q = b.eq[cur];
+    const unsigned long long items_end = b.items[1];
+    unsigned started = 0, parity = 0;
+    for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
+        unsigned i = round * kBlock + threadIdx.x;
+        bool valid = i < count;
+        int dest = -1;
+        bool dead = false;
+        uint32_t slot = 0;
+        float4 rec0, rec1, rec2, rec3, e1;
+        if (valid) {
+            e1 = q[2 * i + 1];
+            slot = as_u(e1.w);
+            valid = slot != kHole;
+        }
+        if (valid) {
+            float4 e0 = q[2 * i], h = b.hits[i];
+            int tri = static_cast<int>(as_u(h.x));
+            if (tri < 0) {
+                dead = true;
+            } else {
+                SceneView tmp;
+                const SceneView& s = scene_of<MULTI>(single, tabs, b, slot, tmp);
+                V3 org = xyz(e0), dir = xyz(e1);
+                Surface sf = decode_hit(s, org, dir, tri, h.y, h.z, h.w);
+                if (sf.light >= 0 && sf.front) {
+                    float4 p0 = b.path[slot * 4 + 0], p1 = b.path[slot * 4 + 1];
+                    V3 Le;
+                    float w;
+                    V3 C = xyz(p1) + hit_emission(s, cfg.flags, sf, dir, h.y, p0.w,
+                                                  1.0f / static_cast<float>(s.num_lights), xyz(p0), Le, w);
+                    b.path[slot * 4 + 1] = f4(C, p1.w);
+                }
+                if (sf.mat < 0) {
+                    dead = true;
+                } else {
+                    rec0 = f4(sf.point, as_f(static_cast<uint32_t>(sf.mat) | (sf.front ? kFrontBit : 0u)));
+                    rec1 = f4(sf.n, as_f(slot));
+                    rec2 = f4(sf.face_n, 0.0f);
+                    rec3 = f4(dir, 0.0f);
+                    dest = queue_of(s.materials[sf.mat].type);
+                }
+            }
+            if (dead) retire(b, b.path[slot * 4 + 1], b.path[slot * 4 + 2]);
+        }
+        RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, &b.counts[kCntExtend0 + nxt], &b.items[0],
+                                                     items_end, dest, false, dead);
+        if (plan.pos >= 0) {
+            float4* e = b.mq[dest] + 4 * plan.pos;
+            e[0] = rec0;
+            e[1] = rec1;
+            e[2] = rec2;
+            e[3] = rec3;
+        }
+        if (plan.start) {
+            ++started;
+            V3 org, dir;
+            init_path(tabs, b, plan.item, slot, org, dir);
+            float4* e = b.eq[nxt] + 2 * plan.ext_pos;
+            e[0] = f4(org, 0.0f);
+            e[1] = f4(dir, as_f(slot));
+        } else if (dead) {
+            write_hole(b, nxt, plan.ext_pos);
+        }
+    }
+    wave_add_stat(&b.stats[2], started);
 }
 
 // ---- shade -----------------------------------------------------------------------------------------
 
+// One kernel per queue; the material type of the queue is a compile-time constant (glass and
+// mirror share the specular queue), so each kernel carries only its BSDF's code.
 template <int QUEUE, bool MULTI>
-__global__ __launch_bounds__(256) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
-                                               int nxt) {
+__global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
+                                                  int nxt) {
+    __shared__ RoundLds<1> lds[2];
     const unsigned count = b.counts[kCntShade0 + QUEUE];
-    while (true) {
-        unsigned base = static_cast<unsigned>(wave_grab64(&b.counts[kCntWork + 1 + QUEUE], 64));
-        if (base >= count) break;
-        unsigned i = base + __lane_id();
+    unsigned* scnt[1] = {&b.counts[kCntShadow]};
+    const float4* mq = b.mq[QUEUE];
+    const unsigned long long items_end = b.items[1];
+    unsigned started = 0, parity = 0;
+    for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
+        unsigned i = round * kBlock + threadIdx.x;
         bool valid = i < count;
         uint32_t slot = 0;
         bool shoot = false, cont = false;
         V3 point{0, 0, 0}, sdir{0, 0, 0}, contrib{0, 0, 0}, ndir{0, 0, 0};
+        unsigned long long item = 0;
         if (valid) {
-            slot = b.mq[QUEUE][i];
+            float4 r0 = mq[4 * i], r1 = mq[4 * i + 1], r2 = mq[4 * i + 2], r3 = mq[4 * i + 3];
+            slot = as_u(r1.w);
             SceneView tmp;
             const SceneView& s = scene_of<MULTI>(single, tabs, b, slot, tmp);
             Surface sf;
-            sf.point = V3{b.hpx[slot], b.hpy[slot], b.hpz[slot]};
-            sf.n = V3{b.hnx[slot], b.hny[slot], b.hnz[slot]};
-            sf.face_n = V3{b.hfx[slot], b.hfy[slot], b.hfz[slot]};
-            int hm = b.hmat[slot];
+            uint32_t hm = as_u(r0.w);
+            sf.point = xyz(r0);
+            sf.n = xyz(r1);
+            sf.face_n = xyz(r2);
             sf.front = (hm & kFrontBit) != 0;
-            sf.mat = hm & ~kFrontBit;
+            sf.mat = static_cast<int>(hm & ~kFrontBit);
             sf.light = -1;
             point = sf.point;
-            V3 dir{b.hdx[slot], b.hdy[slot], b.hdz[slot]};
-            V3 out = -dir;
-            const sol_material m = s.materials[sf.mat];
-            V3 T{b.tx[slot], b.ty[slot], b.tz[slot]};
+            V3 out = -xyz(r3);
+            sol_material m = s.materials[sf.mat];
+            if (QUEUE == kQueueDiffuse) m.type = SOL_BSDF_DIFFUSE;
+            if (QUEUE == kQueuePhong) m.type = SOL_BSDF_PHONG;
+            float4 p0 = b.path[slot * 4 + 0], p1 = b.path[slot * 4 + 1], p2 = b.path[slot * 4 + 2];
+            item = item_of(p2);
+            V3 T = xyz(p0);
             Pcg rng;
-            rng.state = b.rng[slot];
-            uint32_t depth = b.depth[slot];
-            bool skip_nee = (cfg.flags & SOL_PT_DISABLE_NEE) || is_specular(m.type);
+            rng.state = static_cast<unsigned long long>(as_u(p2.x)) | (static_cast<unsigned long long>(as_u(p2.y)) << 32);
+            uint32_t depth = as_u(p1.w);
+            bool skip_nee = QUEUE == kQueueSpecular || (cfg.flags & SOL_PT_DISABLE_NEE);
             if (!skip_nee) {
                 NeeSample ns;
                 shoot = nee_sample(s, cfg.flags, sf, m, out, T, rng, 1.0f / static_cast<float>(s.num_lights), ns);
@@ -182,151 +437,78 @@ __global__ __launch_bounds__(256) void k_shade(SceneView single, WaveTables tabs
             BsdfSample bs;
             if (russian_roulette(cfg, depth, T, rng, q) && bsdf_sample(m, sf, ortho_basis(sf.n), out, rng, bs)) {
                 T = T * (bs.color * (bs.cos / (bs.pdf * q)));
-                b.tx[slot] = T.x; b.ty[slot] = T.y; b.tz[slot] = T.z;
-                b.pdf[slot] = skip_nee ? 0.0f : bs.pdf;
-                b.depth[slot] = depth + 1;
-                cont = depth + 1 < cfg.max_path_len;
+                depth += 1;
+                cont = depth < cfg.max_path_len;
                 ndir = bs.in;
+                if (cont) {
+                    b.path[slot * 4 + 0] = f4(T, skip_nee ? 0.0f : bs.pdf);
+                    b.path[slot * 4 + 1] = make_float4(p1.x, p1.y, p1.z, as_f(depth));
+                    b.path[slot * 4 + 2] = make_float4(as_f(static_cast<uint32_t>(rng.state)),
+                                                       as_f(static_cast<uint32_t>(rng.state >> 32)), p2.z, p2.w);
+                }
             }
-            b.rng[slot] = rng.state;
+            if (!cont) retire(b, p1, p2);
         }
-        int at = wave_append(&b.counts[kCntShadow], shoot);
-        if (at >= 0) {
-            b.sq[0][at] = point.x; b.sq[1][at] = point.y; b.sq[2][at] = point.z;
-            b.sq[3][at] = sdir.x; b.sq[4][at] = sdir.y; b.sq[5][at] = sdir.z;
-            b.sq[6][at] = contrib.x; b.sq[7][at] = contrib.y; b.sq[8][at] = contrib.z;
-            b.ss[at] = slot;
+        bool retired = valid && !cont;
+        RoundPlan plan = plan_round<1>(lds[parity], scnt, &b.counts[kCntExtend0 + nxt], &b.items[0], items_end,
+                                       shoot ? 0 : -1, cont, retired);
+        if (plan.pos >= 0) {
+            float4* e = b.sq + 3 * plan.pos;
+            e[0] = f4(point, as_f(cont ? slot : kSampleTarget));
+            e[1] = f4(sdir, as_f(static_cast<uint32_t>(item >> 32)));
+            e[2] = f4(contrib, as_f(static_cast<uint32_t>(item)));
         }
-        at = wave_append(&b.counts[kCntExtend0 + nxt], cont);
-        if (at >= 0) {
-            float* const* e = b.eq[nxt];
-            e[0][at] = point.x; e[1][at] = point.y; e[2][at] = point.z;
-            e[3][at] = ndir.x; e[4][at] = ndir.y; e[5][at] = ndir.z;
-            e[6][at] = cfg.ray_offset;
-            b.es[nxt][at] = slot;
-        }
-        at = wave_append(&b.counts[kCntRegen], valid && !cont);
-        if (at >= 0) b.rq[at] = slot;
-    }
-}
-
-// ---- shadow ----------------------------------------------------------------------------------------
-
-template <int BLOCK, int LDS, bool MULTI>
-__global__ __launch_bounds__(BLOCK) void k_shadow(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
-                                                  TraversalScratch sc) {
-    __shared__ float4 lds_nodes[kTopNodesMax * 4];
-    __shared__ int lds_stack[LDS * BLOCK];
-    if (!MULTI) load_top_nodes<BLOCK>(single, lds_nodes);
-    Stack st = make_stack_w<BLOCK>(lds_stack, sc);
-    st.capacity = LDS + sc.spill_entries;
-    const unsigned count = b.counts[kCntShadow];
-    unsigned processed = 0;
-    while (true) {
-        unsigned base = static_cast<unsigned>(wave_grab64(&b.counts[kCntWork + 4], 64));
-        if (base >= count) break;
-        unsigned i = base + __lane_id();
-        if (i >= count) continue;
-        ++processed;
-        uint32_t slot = b.ss[i];
-        SceneView tmp;
-        const SceneView& s = scene_of<MULTI>(single, tabs, b, slot, tmp);
-        V3 org{b.sq[0][i], b.sq[1][i], b.sq[2][i]};
-        V3 dir{b.sq[3][i], b.sq[4][i], b.sq[5][i]};
-        RayPre r = make_ray_pre(org, dir);
-        HitRecord unused;
-        bool occluded = MULTI ? traverse<true, BLOCK, LDS>(s, s.nodes, st, r, cfg.ray_offset, 1.0f - cfg.ray_offset, unused)
-                              : traverse<true, BLOCK, LDS>(s, lds_nodes, st, r, cfg.ray_offset, 1.0f - cfg.ray_offset, unused);
-        if (!occluded) {
-            b.cx[slot] = b.cx[slot] + b.sq[6][i];
-            b.cy[slot] = b.cy[slot] + b.sq[7][i];
-            b.cz[slot] = b.cz[slot] + b.sq[8][i];
-        }
-    }
-    wave_add_stat(&b.stats[1], processed);
-}
-
-// ---- regen -----------------------------------------------------------------------------------------
-
-// Tile owning a global item: the last tile whose first item is <= item.
-SOL_DEV uint32_t find_tile(const WaveTables& t, unsigned long long item) {
-    uint32_t lo = 0, hi = t.num_tiles;   // tile_item0[lo] <= item < tile_item0[hi]
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (t.tile_item0[mid] <= item) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_regen(WaveTables tabs, WaveBuf b, sol_pt_config cfg, int nxt, int initial) {
-    const unsigned count = initial ? b.slots : b.counts[kCntRegen];
-    const unsigned long long chunk_begin = b.items[2], chunk_end = b.items[1];
-    unsigned started = 0;
-    while (true) {
-        unsigned base = static_cast<unsigned>(wave_grab64(&b.counts[kCntWork + 5], 64));
-        if (base >= count) break;
-        unsigned i = base + __lane_id();
-        bool valid = i < count;
-        uint32_t slot = valid ? (initial ? i : b.rq[i]) : 0;
-        if (valid && !initial) {
-            unsigned long long k = b.item[slot] - chunk_begin;
-            b.samples[k] = b.cx[slot];
-            b.samples[b.sample_capacity + k] = b.cy[slot];
-            b.samples[2 * b.sample_capacity + k] = b.cz[slot];
-        }
-        // one new item per lane that retired a path
-        unsigned long long mask = __ballot(valid);
-        unsigned long long first = 0;
-        int lane = __lane_id();
-        int leader = __ffsll(static_cast<long long>(mask)) - 1;
-        if (mask && lane == leader) first = atomicAdd(&b.items[0], static_cast<unsigned long long>(__popcll(mask)));
-        first = __shfl(first, leader < 0 ? 0 : leader);
-        unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-        unsigned long long item = first + __popcll(below);
-        bool start = valid && item < chunk_end;
-        V3 org{0, 0, 0}, dir{0, 0, 0};
-        if (start) {
-            ++started;
-            uint32_t tile = find_tile(tabs, item);
-            uint32_t tw = tabs.tile_w[tile], th = tabs.tile_h[tile];
-            unsigned long long local = item - tabs.tile_item0[tile];
-            unsigned long long per_sample = static_cast<unsigned long long>(tw) * th;
-            unsigned long long s = local / per_sample;
-            uint32_t pix = static_cast<uint32_t>(local - s * per_sample);
-            uint32_t job = tabs.tile_job[tile];
-            uint32_t x = tabs.tile_x0[tile] + pix % tw;
-            uint32_t y = tabs.tile_y0[tile] + pix / tw;
-            uint32_t sc = tabs.job_scene[job];
-            const sol_camera cam = tabs.scenes[sc].camera;
-            Pcg rng;
-            rng.seed(pixel_seed(tabs.tile_sample0[tile] + s, x, y));
-            camera_ray(cam, x, y, tabs.job_w[job], tabs.job_h[job], rng, org, dir);
-            b.rng[slot] = rng.state;
-            b.item[slot] = item;
-            b.tx[slot] = 1.0f; b.ty[slot] = 1.0f; b.tz[slot] = 1.0f;
-            b.cx[slot] = 0.0f; b.cy[slot] = 0.0f; b.cz[slot] = 0.0f;
-            b.pdf[slot] = 0.0f;
-            b.depth[slot] = 0;
-            if (MULTI) b.scene[slot] = sc;
-        }
-        int at = wave_append(&b.counts[kCntExtend0 + nxt], start);
-        if (at >= 0) {
-            float* const* e = b.eq[nxt];
-            e[0][at] = org.x; e[1][at] = org.y; e[2][at] = org.z;
-            e[3][at] = dir.x; e[4][at] = dir.y; e[5][at] = dir.z;
-            e[6][at] = 0.0f;
-            b.es[nxt][at] = slot;
+        if (plan.ext_pos >= 0) {
+            if (cont || plan.start) {
+                V3 org = point, dir = ndir;
+                float tmin = cfg.ray_offset;
+                if (plan.start) {
+                    ++started;
+                    init_path(tabs, b, plan.item, slot, org, dir);
+                    tmin = 0.0f;
+                }
+                float4* e = b.eq[nxt] + 2 * plan.ext_pos;
+                e[0] = f4(org, tmin);
+                e[1] = f4(dir, as_f(slot));
+            } else {
+                write_hole(b, nxt, plan.ext_pos);
+            }
         }
     }
     wave_add_stat(&b.stats[2], started);
 }
 
-// Zeroes the counters the next iteration appends to (kernel, not a memset: it is captured in
-// graphs).
-__global__ void k_reset(WaveBuf b, int nxt) {
+// ---- start / reset / reduce ------------------------------------------------------------------------
+
+// Starts a path in every slot (beginning of a chunk of work).
+__global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, int nxt) {
+    __shared__ RoundLds<1> lds[2];
+    const unsigned long long items_end = b.items[1];
+    unsigned started = 0, parity = 0;
+    for (unsigned round = blockIdx.x; round * kBlock < b.slots; round += gridDim.x, parity ^= 1) {
+        unsigned slot = round * kBlock + threadIdx.x;
+        bool want = slot < b.slots;
+        RoundPlan plan = plan_round<1>(lds[parity], nullptr, &b.counts[kCntExtend0 + nxt], &b.items[0], items_end,
+                                       -1, false, want);
+        if (plan.start) {
+            ++started;
+            V3 org, dir;
+            init_path(tabs, b, plan.item, slot, org, dir);
+            float4* e = b.eq[nxt] + 2 * plan.ext_pos;
+            e[0] = f4(org, 0.0f);
+            e[1] = f4(dir, as_f(slot));
+        } else if (want) {
+            write_hole(b, nxt, plan.ext_pos);
+        }
+    }
+    wave_add_stat(&b.stats[2], started);
+}
+
+// After trace(k) has consumed extension queue `cur` and the shadow queue: clears what sort(k) and
+// shade(k) append to (kernel, not a memset: it is captured in graphs).
+__global__ void k_reset(WaveBuf b, int cur) {
     int i = threadIdx.x;
-    if (i < kNumWaveCounters && i != kCntExtend0 + (1 - nxt)) b.counts[i] = 0;
+    if (i < kNumWaveCounters && i != kCntExtend0 + cur) b.counts[i] = 0;
 }
 
 __global__ void k_set_chunk(WaveBuf b, unsigned long long begin, unsigned long long end) {
@@ -350,10 +532,10 @@ __global__ __launch_bounds__(256) void k_reduce(WaveTables tabs, WaveBuf b, uint
     unsigned long long spp = (tabs.tile_item0[tile + 1] - tabs.tile_item0[tile]) / per_sample;
     float r = 0.0f, g = 0.0f, bl = 0.0f;
     for (unsigned long long s = 0; s < spp; ++s) {
-        unsigned long long k = k0 + s * per_sample + pix;
-        r = r + b.samples[k];
-        g = g + b.samples[b.sample_capacity + k];
-        bl = bl + b.samples[2 * b.sample_capacity + k];
+        float4 v = b.samples[k0 + s * per_sample + pix];
+        r = r + v.x;
+        g = g + v.y;
+        bl = bl + v.z;
     }
     uint32_t job = tabs.tile_job[tile];
     uint32_t w = tabs.job_w[job], h = tabs.job_h[job];
@@ -366,15 +548,9 @@ __global__ __launch_bounds__(256) void k_reduce(WaveTables tabs, WaveBuf b, uint
 }
 
 template <int BLOCK, int LDS, bool MULTI>
-void launch_extend_t(const WaveLaunch& L, int cur) {
-    hipLaunchKernelGGL((k_extend<BLOCK, LDS, MULTI>), dim3(L.grid_traverse), dim3(BLOCK), 0, L.stream, L.single, L.tabs,
+void launch_trace_t(const WaveLaunch& L, int cur) {
+    hipLaunchKernelGGL((k_trace<BLOCK, LDS, MULTI>), dim3(L.grid_traverse), dim3(BLOCK), 0, L.stream, L.single, L.tabs,
                        L.buf, L.cfg, cur, L.scratch);
-}
-
-template <int BLOCK, int LDS, bool MULTI>
-void launch_shadow_t(const WaveLaunch& L) {
-    hipLaunchKernelGGL((k_shadow<BLOCK, LDS, MULTI>), dim3(L.grid_traverse), dim3(BLOCK), 0, L.stream, L.single, L.tabs,
-                       L.buf, L.cfg, L.scratch);
 }
 
 } // namespace
@@ -393,20 +569,25 @@ void launch_shadow_t(const WaveLaunch& L) {
         default: return hipErrorInvalidValue;                                                        \
     }
 
-hipError_t wave_launch_extend(const WaveLaunch& L, int cur) {
-    if (L.multi) { SOLHIP_WAVE_DISPATCH(launch_extend_t, true, L, cur) }
-    else { SOLHIP_WAVE_DISPATCH(launch_extend_t, false, L, cur) }
+hipError_t wave_launch_trace(const WaveLaunch& L, int cur) {
+    (void)hipGetLastError();
+    if (L.multi) { SOLHIP_WAVE_DISPATCH(launch_trace_t, true, L, cur) }
+    else { SOLHIP_WAVE_DISPATCH(launch_trace_t, false, L, cur) }
     return hipGetLastError();
 }
 
-hipError_t wave_launch_shadow(const WaveLaunch& L) {
-    if (L.multi) { SOLHIP_WAVE_DISPATCH(launch_shadow_t, true, L) }
-    else { SOLHIP_WAVE_DISPATCH(launch_shadow_t, false, L) }
+hipError_t wave_launch_sort(const WaveLaunch& L, int cur) {
+    (void)hipGetLastError();
+    if (L.multi)
+        hipLaunchKernelGGL((k_sort<true>), dim3(L.grid_shade), dim3(kBlock), 0, L.stream, L.single, L.tabs, L.buf, L.cfg, cur);
+    else
+        hipLaunchKernelGGL((k_sort<false>), dim3(L.grid_shade), dim3(kBlock), 0, L.stream, L.single, L.tabs, L.buf, L.cfg, cur);
     return hipGetLastError();
 }
 
 hipError_t wave_launch_shade(const WaveLaunch& L, int queue, int nxt) {
-    dim3 g(L.grid_shade), bs(256);
+    dim3 g(L.grid_shade), bs(kBlock);
+    (void)hipGetLastError();
     if (L.multi) {
         if (queue == 0) hipLaunchKernelGGL((k_shade<0, true>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
         if (queue == 1) hipLaunchKernelGGL((k_shade<1, true>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
@@ -419,22 +600,20 @@ hipError_t wave_launch_shade(const WaveLaunch& L, int queue, int nxt) {
     return hipGetLastError();
 }
 
-hipError_t wave_launch_regen(const WaveLaunch& L, int nxt, bool initial) {
-    if (L.multi)
-        hipLaunchKernelGGL((k_regen<true>), dim3(L.grid_shade), dim3(256), 0, L.stream, L.tabs, L.buf, L.cfg, nxt,
-                           initial ? 1 : 0);
-    else
-        hipLaunchKernelGGL((k_regen<false>), dim3(L.grid_shade), dim3(256), 0, L.stream, L.tabs, L.buf, L.cfg, nxt,
-                           initial ? 1 : 0);
+hipError_t wave_launch_start(const WaveLaunch& L, int nxt) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_start, dim3(L.grid_shade), dim3(kBlock), 0, L.stream, L.tabs, L.buf, nxt);
     return hipGetLastError();
 }
 
-hipError_t wave_launch_reset(const WaveLaunch& L, int nxt) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, L.stream, L.buf, nxt);
+hipError_t wave_launch_reset(const WaveLaunch& L, int cur) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, L.stream, L.buf, cur);
     return hipGetLastError();
 }
 
 hipError_t wave_launch_set_chunk(const WaveLaunch& L, unsigned long long begin, unsigned long long end) {
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(64), 0, L.stream, L.buf, begin, end);
     return hipGetLastError();
 }
@@ -442,6 +621,7 @@ hipError_t wave_launch_set_chunk(const WaveLaunch& L, unsigned long long begin, 
 hipError_t wave_launch_reduce(const WaveLaunch& L, uint32_t first_tile, uint32_t num_tiles, uint32_t max_tile_pixels,
                               unsigned long long chunk_begin) {
     dim3 g(num_tiles, (max_tile_pixels + 255) / 256);
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_reduce, g, dim3(256), 0, L.stream, L.tabs, L.buf, first_tile, chunk_begin);
     return hipGetLastError();
 }

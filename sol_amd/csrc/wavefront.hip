@@ -16,8 +16,10 @@ namespace {
 #define HIP_OK(expr)                                                                             \
     do {                                                                                         \
         hipError_t e__ = (expr);                                                                 \
-        if (e__ != hipSuccess)                                                                   \
+        if (e__ != hipSuccess) {                                                                 \
+            (void)hipGetLastError();                                                             \
             return set_error(SOLHIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e__));   \
+        }                                                                                        \
     } while (0)
 
 // Items (path samples) whose colours the sample buffer can hold at once: 12 bytes each.
@@ -40,8 +42,8 @@ struct WavefrontState {
     DeviceBuffer arena, counters, samples;
     unsigned long long sample_capacity = 0;
     WaveBuf buf{};
-    DeviceBuffer t_scenes, t_job_scene, t_job_accum, t_job_w, t_job_h, t_tile_job, t_tile_x0, t_tile_y0, t_tile_w,
-        t_tile_h, t_tile_item0, t_tile_sample0;
+    DeviceBuffer t_scenes, t_job_item0, t_job_sample0, t_job_scene, t_job_accum, t_job_w, t_job_h, t_job_spp,
+        t_job_tw, t_job_th, t_tile_job, t_tile_x0, t_tile_y0, t_tile_w, t_tile_h, t_tile_item0;
     uint64_t generation = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -56,40 +58,21 @@ struct WavefrontState {
 
     int ensure_slots(uint32_t n) {
         if (n == slots && arena.ptr) return SOLHIP_OK;
-        // per slot: path 7 f + 2 u64 + 2 u32, hit 13, extend 2 x 8, shade 3, shadow 10, regen 1
-        const int words32 = 7 + 2 + 13 + 16 + 3 + 10 + 1;
-        size_t bytes = 0;
-        bytes += align_up(sizeof(float) * n) * words32 + align_up(sizeof(unsigned long long) * n) * 2 + 4096;
+        // per slot: path record 64 B, hit 16 B, two extension queues 2 x 32 B, three shade queues
+        // 3 x 64 B, shadow queue 48 B
+        const size_t per_slot_f4 = 4 + 1 + 2 * 2 + 3 * 4 + 3;
+        size_t bytes = align_up(sizeof(float4) * n) * per_slot_f4 + 4096;
         HIP_OK(arena.alloc(bytes));
         HIP_OK(counters.alloc(256 * 3));
         char* p = arena.as<char>();
-        auto f32 = [&]() { float* r = reinterpret_cast<float*>(p); p += align_up(sizeof(float) * n); return r; };
-        auto u32 = [&]() { uint32_t* r = reinterpret_cast<uint32_t*>(p); p += align_up(sizeof(uint32_t) * n); return r; };
-        auto u64 = [&]() {
-            unsigned long long* r = reinterpret_cast<unsigned long long*>(p);
-            p += align_up(sizeof(unsigned long long) * n);
-            return r;
-        };
+        auto f4s = [&](size_t k) { float4* r = reinterpret_cast<float4*>(p); p += align_up(sizeof(float4) * n * k); return r; };
         WaveBuf& b = buf;
         b.slots = n;
-        b.tx = f32(); b.ty = f32(); b.tz = f32();
-        b.cx = f32(); b.cy = f32(); b.cz = f32();
-        b.pdf = f32();
-        b.rng = u64(); b.item = u64();
-        b.depth = u32(); b.scene = u32();
-        b.hpx = f32(); b.hpy = f32(); b.hpz = f32();
-        b.hnx = f32(); b.hny = f32(); b.hnz = f32();
-        b.hfx = f32(); b.hfy = f32(); b.hfz = f32();
-        b.hdx = f32(); b.hdy = f32(); b.hdz = f32();
-        b.hmat = reinterpret_cast<int*>(u32());
-        for (int q = 0; q < 2; ++q) {
-            for (int k = 0; k < 7; ++k) b.eq[q][k] = f32();
-            b.es[q] = u32();
-        }
-        for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = u32();
-        for (int k = 0; k < 9; ++k) b.sq[k] = f32();
-        b.ss = u32();
-        b.rq = u32();
+        b.path = f4s(4);
+        b.hits = f4s(1);
+        for (int q = 0; q < 2; ++q) b.eq[q] = f4s(2);
+        for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = f4s(4);
+        b.sq = f4s(3);
         char* c = counters.as<char>();
         b.counts = reinterpret_cast<unsigned*>(c);
         b.items = reinterpret_cast<unsigned long long*>(c + 256);
@@ -101,9 +84,9 @@ struct WavefrontState {
 
     int ensure_samples(unsigned long long items) {
         if (items <= sample_capacity && samples.ptr) return SOLHIP_OK;
-        HIP_OK(samples.alloc(sizeof(float) * 3 * items));
+        HIP_OK(samples.alloc(sizeof(float4) * items));
         sample_capacity = items;
-        buf.samples = samples.as<float>();
+        buf.samples = samples.as<float4>();
         buf.sample_capacity = items;
         ++generation;
         return SOLHIP_OK;
@@ -137,21 +120,21 @@ void Context::release_wavefront() {
 
 namespace {
 
-// One bounce of every active path: reset, extend, shade (three queues), shadow, regen.
+// One bounce of every path in flight: trace (closest hits of queue `cur`, any hits of the shadow
+// queue), reset, sort, shade (three queues). Paths that end in sort or shade start their slot's
+// next work item there.
 hipError_t enqueue_iteration(const WaveLaunch& L, int cur, hipEvent_t* ev) {
     int nxt = 1 - cur;
     hipError_t e;
-    if ((e = wave_launch_reset(L, nxt)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[0], L.stream)) != hipSuccess) return e;
-    if ((e = wave_launch_extend(L, cur)) != hipSuccess) return e;
+    if ((e = wave_launch_trace(L, cur)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[1], L.stream)) != hipSuccess) return e;
+    if ((e = wave_launch_reset(L, cur)) != hipSuccess) return e;
+    if ((e = wave_launch_sort(L, cur)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev[2], L.stream)) != hipSuccess) return e;
     for (int q = 0; q < kNumShadeQueues; ++q)
         if ((e = wave_launch_shade(L, q, nxt)) != hipSuccess) return e;
-    if (ev && (e = hipEventRecord(ev[2], L.stream)) != hipSuccess) return e;
-    if ((e = wave_launch_shadow(L)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[3], L.stream)) != hipSuccess) return e;
-    if ((e = wave_launch_regen(L, nxt, false)) != hipSuccess) return e;
-    if (ev && (e = hipEventRecord(ev[4], L.stream)) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -159,6 +142,7 @@ std::string graph_key(const WaveLaunch& L, const WavefrontState& st, int iters) 
     std::string k(reinterpret_cast<const char*>(&L.cfg), sizeof(L.cfg));
     k.append(reinterpret_cast<const char*>(&L.single), sizeof(L.single));
     k.append(reinterpret_cast<const char*>(&L.tabs), sizeof(L.tabs));
+    k.append(reinterpret_cast<const char*>(&L.buf), sizeof(L.buf));
     int meta[6] = {L.block, L.lds_stack, L.grid_traverse, L.grid_shade, L.multi ? 1 : 0, iters};
     k.append(reinterpret_cast<const char*>(meta), sizeof(meta));
     k.append(reinterpret_cast<const char*>(&st.generation), sizeof(st.generation));
@@ -175,54 +159,67 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     int rc = st.ensure_slots(tu.path_slots);
     if (rc) return rc;
 
-    // Tiles: sample passes small enough that one tile's samples fit the sample buffer.
+    // Jobs in passes of samples: a pass keeps a job under 2^32 items and one tile's samples
+    // under the sample-buffer capacity (far above any ordinary sample count); each pass is
+    // accumulated into the image on its own.
+    const unsigned long long cap = kDefaultSampleCapacity;
+    std::vector<unsigned long long> job_item0, job_sample0;
+    std::vector<uint32_t> job_scene, job_w, job_h, job_spp, job_tw, job_th;
+    std::vector<float*> job_accum;
     std::vector<uint32_t> tile_job, tile_x0, tile_y0, tile_w, tile_h;
-    std::vector<unsigned long long> tile_item0, tile_sample0;
+    std::vector<unsigned long long> tile_item0;
     unsigned long long total = 0;
     uint32_t max_tile_pixels = 0;
-    const unsigned long long cap = kDefaultSampleCapacity;
-    for (uint32_t j = 0; j < jobs.size(); ++j) {
-        const Job& job = jobs[j];
-        for (uint32_t y0 = 0; y0 < job.h; y0 += tu.tile_h) {
-            for (uint32_t x0 = 0; x0 < job.w; x0 += tu.tile_w) {
-                uint32_t tw = std::min(tu.tile_w, job.w - x0), th = std::min(tu.tile_h, job.h - y0);
-                unsigned long long px = static_cast<unsigned long long>(tw) * th;
-                uint32_t max_pass = static_cast<uint32_t>(std::max<unsigned long long>(1, cap / px));
-                // Samples beyond one pass are split into further tiles over the same pixels, each
-                // accumulated on its own (only for sample counts far above the capacity).
-                for (uint32_t s0 = 0; s0 < job.sample_count; s0 += max_pass) {
-                    uint32_t n = std::min(max_pass, job.sample_count - s0);
-                    tile_job.push_back(j);
+    for (const Job& j : jobs) {
+        uint32_t tw = std::min(tu.tile_w, j.w), th = std::min(tu.tile_h, j.h);
+        unsigned long long px = static_cast<unsigned long long>(j.w) * j.h;
+        unsigned long long max_pass = std::min<unsigned long long>(cap / (static_cast<unsigned long long>(tw) * th),
+                                                                   0xFFFFFFFFull / px);
+        if (max_pass == 0) return set_error(SOLHIP_E_INVALID, "image too large for one pass");
+        for (uint32_t s0 = 0; s0 < j.sample_count;) {
+            uint32_t spp = static_cast<uint32_t>(std::min<unsigned long long>(max_pass, j.sample_count - s0));
+            uint32_t jid = static_cast<uint32_t>(job_item0.size());
+            job_item0.push_back(total);
+            job_sample0.push_back(j.sample_index + s0);
+            job_scene.push_back(j.scene);
+            job_w.push_back(j.w);
+            job_h.push_back(j.h);
+            job_spp.push_back(spp);
+            job_tw.push_back(tw);
+            job_th.push_back(th);
+            job_accum.push_back(j.accum);
+            // tiles in item order: bands of th rows, tiles of tw columns
+            for (uint32_t y0 = 0; y0 < j.h; y0 += th) {
+                uint32_t bh = std::min(th, j.h - y0);
+                for (uint32_t x0 = 0; x0 < j.w; x0 += tw) {
+                    uint32_t w2 = std::min(tw, j.w - x0);
+                    tile_job.push_back(jid);
                     tile_x0.push_back(x0);
                     tile_y0.push_back(y0);
-                    tile_w.push_back(tw);
-                    tile_h.push_back(th);
+                    tile_w.push_back(w2);
+                    tile_h.push_back(bh);
                     tile_item0.push_back(total);
-                    tile_sample0.push_back(job.sample_index + s0);
-                    total += px * n;
-                    max_tile_pixels = std::max<uint32_t>(max_tile_pixels, static_cast<uint32_t>(px));
+                    total += static_cast<unsigned long long>(w2) * bh * spp;
+                    max_tile_pixels = std::max(max_tile_pixels, w2 * bh);
                 }
             }
+            s0 += spp;
         }
     }
+    job_item0.push_back(total);
     tile_item0.push_back(total);
     const uint32_t num_tiles = static_cast<uint32_t>(tile_job.size());
+    const uint32_t num_jobs = static_cast<uint32_t>(job_scene.size());
 
-    std::vector<uint32_t> job_scene, job_w, job_h;
-    std::vector<float*> job_accum;
-    for (const Job& j : jobs) {
-        job_scene.push_back(j.scene);
-        job_w.push_back(j.w);
-        job_h.push_back(j.h);
-        job_accum.push_back(j.accum);
-    }
     hipStream_t s = ctx.stream;
-    if ((rc = st.upload(st.t_scenes, views, s)) || (rc = st.upload(st.t_job_scene, job_scene, s)) ||
+    if ((rc = st.upload(st.t_scenes, views, s)) || (rc = st.upload(st.t_job_item0, job_item0, s)) ||
+        (rc = st.upload(st.t_job_sample0, job_sample0, s)) || (rc = st.upload(st.t_job_scene, job_scene, s)) ||
         (rc = st.upload(st.t_job_accum, job_accum, s)) || (rc = st.upload(st.t_job_w, job_w, s)) ||
-        (rc = st.upload(st.t_job_h, job_h, s)) || (rc = st.upload(st.t_tile_job, tile_job, s)) ||
-        (rc = st.upload(st.t_tile_x0, tile_x0, s)) || (rc = st.upload(st.t_tile_y0, tile_y0, s)) ||
-        (rc = st.upload(st.t_tile_w, tile_w, s)) || (rc = st.upload(st.t_tile_h, tile_h, s)) ||
-        (rc = st.upload(st.t_tile_item0, tile_item0, s)) || (rc = st.upload(st.t_tile_sample0, tile_sample0, s)))
+        (rc = st.upload(st.t_job_h, job_h, s)) || (rc = st.upload(st.t_job_spp, job_spp, s)) ||
+        (rc = st.upload(st.t_job_tw, job_tw, s)) || (rc = st.upload(st.t_job_th, job_th, s)) ||
+        (rc = st.upload(st.t_tile_job, tile_job, s)) || (rc = st.upload(st.t_tile_x0, tile_x0, s)) ||
+        (rc = st.upload(st.t_tile_y0, tile_y0, s)) || (rc = st.upload(st.t_tile_w, tile_w, s)) ||
+        (rc = st.upload(st.t_tile_h, tile_h, s)) || (rc = st.upload(st.t_tile_item0, tile_item0, s)))
         return rc;
 
     // Chunks of whole tiles whose items fit the sample buffer.
@@ -245,24 +242,31 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     WaveLaunch L{};
     L.single = views[0];
     L.tabs.scenes = st.t_scenes.as<SceneView>();
+    L.tabs.num_jobs = num_jobs;
+    L.tabs.job_item0 = st.t_job_item0.as<unsigned long long>();
+    L.tabs.job_sample0 = st.t_job_sample0.as<unsigned long long>();
     L.tabs.job_scene = st.t_job_scene.as<uint32_t>();
     L.tabs.job_accum = st.t_job_accum.as<float*>();
     L.tabs.job_w = st.t_job_w.as<uint32_t>();
     L.tabs.job_h = st.t_job_h.as<uint32_t>();
+    L.tabs.job_spp = st.t_job_spp.as<uint32_t>();
+    L.tabs.job_tw = st.t_job_tw.as<uint32_t>();
+    L.tabs.job_th = st.t_job_th.as<uint32_t>();
     L.tabs.tile_job = st.t_tile_job.as<uint32_t>();
     L.tabs.tile_x0 = st.t_tile_x0.as<uint32_t>();
     L.tabs.tile_y0 = st.t_tile_y0.as<uint32_t>();
     L.tabs.tile_w = st.t_tile_w.as<uint32_t>();
     L.tabs.tile_h = st.t_tile_h.as<uint32_t>();
     L.tabs.tile_item0 = st.t_tile_item0.as<unsigned long long>();
-    L.tabs.tile_sample0 = st.t_tile_sample0.as<unsigned long long>();
     L.tabs.num_tiles = num_tiles;
     L.buf = st.buf;
+    // no more slots than the largest chunk has items: a small render does not walk idle slots
+    L.buf.slots = static_cast<uint32_t>(std::min<unsigned long long>(st.slots, (biggest + 255) & ~255ull));
     L.cfg = cfg;
     L.block = static_cast<int>(tu.block_size);
     L.lds_stack = static_cast<int>(tu.lds_stack);
     L.grid_traverse = persistent_grid(ctx.info.compute_units, L.block);
-    L.grid_shade = persistent_grid(ctx.info.compute_units, 256);
+    L.grid_shade = persistent_grid(ctx.info.compute_units, kWaveBlock);
     L.multi = multi;
     L.stream = s;
     if ((rc = ctx.ensure_scratch(L.grid_traverse * L.block))) return rc;
@@ -271,7 +275,12 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
         return set_error(SOLHIP_E_TOO_DEEP, "BVH deeper than the traversal stack");
 
     const int iters = static_cast<int>(std::max<uint32_t>(2, tu.graph_iters & ~1u));
-    const bool use_graph = !ctx.profiling;
+    // Profiling records HIP events between the kernels of every iteration. Events recorded by a
+    // graph cannot be timed on this runtime, so a profiled run launches the same iterations
+    // eagerly; otherwise the loop replays the captured graph.
+    const bool profile = ctx.profiling;
+    if (profile && (rc = st.ensure_events(static_cast<size_t>(iters) * 4))) return rc;
+    const bool use_graph = !profile;
     if (use_graph) {
         std::string key = graph_key(L, st, iters);
         if (key != st.graph_key || !st.graph_exec) {
@@ -291,8 +300,6 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
             HIP_OK(hipGraphInstantiate(&st.graph_exec, g, nullptr, nullptr, 0));
             st.graph_key = key;
         }
-    } else {
-        if ((rc = st.ensure_events(static_cast<size_t>(iters) * 5))) return rc;
     }
 
     HIP_OK(hipMemsetAsync(st.buf.stats, 0, sizeof(unsigned long long) * 4, s));
@@ -300,32 +307,34 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     uint64_t replays = 0, launches = 0;
     for (const Chunk& c : chunks) {
         HIP_OK(wave_launch_set_chunk(L, c.begin, c.end));
-        HIP_OK(wave_launch_regen(L, 0, true));
+        HIP_OK(wave_launch_start(L, 0));
         launches += 2;
         while (true) {
             if (use_graph) {
                 HIP_OK(hipGraphLaunch(st.graph_exec, s));
             } else {
-                for (int it = 0; it < iters; ++it) HIP_OK(enqueue_iteration(L, it & 1, &st.events[it * 5]));
+                for (int it = 0; it < iters; ++it)
+                    HIP_OK(enqueue_iteration(L, it & 1, profile ? &st.events[it * 4] : nullptr));
             }
             ++replays;
-            launches += static_cast<uint64_t>(iters) * 7;
-            unsigned pending = 0;
-            HIP_OK(hipMemcpyAsync(&pending, st.buf.counts + kCntExtend0, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            launches += static_cast<uint64_t>(iters) * 6;
+            // After an even number of iterations the next extension queue is queue 0; shadow rays
+            // of the last shade still need a trace.
+            unsigned counts[kNumWaveCounters];
+            HIP_OK(hipMemcpyAsync(counts, st.buf.counts, sizeof(counts), hipMemcpyDeviceToHost, s));
             HIP_OK(hipStreamSynchronize(s));
-            if (!use_graph) {
+            if (profile) {
                 for (int it = 0; it < iters; ++it) {
-                    float ms[4];
-                    for (int k = 0; k < 4; ++k)
-                        HIP_OK(hipEventElapsedTime(&ms[k], st.events[it * 5 + k], st.events[it * 5 + k + 1]));
-                    ctx.stats.extend_ms += ms[0];
-                    ctx.stats.shade_ms += ms[1];
-                    ctx.stats.shadow_ms += ms[2];
-                    ctx.stats.regen_ms += ms[3];
-                    ctx.stats.extend_launches += 1;
+                    float ms[3];
+                    for (int k = 0; k < 3; ++k)
+                        HIP_OK(hipEventElapsedTime(&ms[k], st.events[it * 4 + k], st.events[it * 4 + k + 1]));
+                    ctx.stats.trace_ms += ms[0];
+                    ctx.stats.sort_ms += ms[1];
+                    ctx.stats.shade_ms += ms[2];
+                    ctx.stats.trace_launches += 1;
                 }
             }
-            if (pending == 0) break;
+            if (counts[kCntExtend0] == 0 && counts[kCntShadow] == 0) break;
         }
         HIP_OK(wave_launch_reduce(L, c.first, c.count, max_tile_pixels, c.begin));
         launches += 1;

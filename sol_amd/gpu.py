@@ -5,7 +5,9 @@ include/sol/algorithms/path_tracer.h:23-54 of sol): render(image, sample_index, 
 adds the samples [sample_index, sample_index + sample_count) of every pixel into a planar RGB
 float32 image of shape (3, height, width), pixel (0, 0) top-left.
 """
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -38,6 +40,19 @@ class PathTracerConfig:
         return c
 
 
+_live_scenes = weakref.WeakSet()
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    # Scenes before contexts, while the HIP runtime is still up.
+    for s in list(_live_scenes):
+        s.close()
+    for c in list(_live_contexts):
+        c.close()
+
+
 def new_image(width, height):
     """An empty planar RGB image, sol::Image(width, height, 3)."""
     return np.zeros((3, height, width), np.float32)
@@ -50,6 +65,7 @@ class Context:
         h = C.c_void_p()
         F.check_hip(F.hip.solhip_create(device, C.byref(h)), "solhip_create")
         self._h = h
+        _live_contexts.add(self)
 
     def close(self):
         if self._h and self._h.value:
@@ -118,6 +134,7 @@ class GpuScene:
         h = C.c_void_p()
         F.check_hip(F.hip.solhip_scene_upload(ctx.handle, C.byref(desc), C.byref(h)), "solhip_scene_upload")
         self._h = h
+        _live_scenes.add(self)
 
     def close(self):
         if self._h and self._h.value:

@@ -291,6 +291,8 @@ FlatScene flatten_scene(const Scene& scene, uint32_t top_levels) {
             s.n2[k] = f.normals[i2 * 3 + k];
         }
         g.prim = static_cast<int32_t>(t);
+        Vec3f v0(g.v0[0], g.v0[1], g.v0[2]), v1(g.v1[0], g.v1[1], g.v1[2]), v2(g.v2[0], g.v2[1], g.v2[2]);
+        copy3(s.face_n, cross(v1 - v0, v2 - v0));
         s.material = f.tri_material[t];
         s.light = f.tri_light[t];
     }

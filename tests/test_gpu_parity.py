@@ -166,7 +166,7 @@ def test_wavefront_profiling_mode_matches_graph_mode(ctx, cornell):
     st = ctx.stats()
     ctx.set_profiling(False)
     np.testing.assert_array_equal(a, b)
-    assert st.extend_ms > 0 and st.extend_launches > 0
+    assert st.trace_ms > 0 and st.trace_launches > 0
     assert st.samples == 48 * 40 * 4 and st.extension_rays >= st.samples
 
 
