@@ -9,8 +9,11 @@ and arithmetic, so their images are identical (equal RMSE by construction; check
 Also reported:
   roofline      the traversal kernel k_trace (closest hits of the extension rays and any hits of the
                 shadow rays), timed with HIP events around each launch over a second timed pass
-                of the same frames; its algorithmic bytes per ray come from the oracle's
-                instrumented traversal of the same BVH.
+                of the same frames; its algorithmic bytes per ray use the inner-node visits and
+                triangle tests per ray of tests/golden/traversal_counts.json (counted once on the
+                same BVH and rays by tests/golden/make_traversal_counts.py). "traffic" is the HBM
+                bytes per launch read from the committed rocprofv3 counter summary of this
+                workload, if present. "copy_gbs" is a device-to-device copy timed here.
   cpu_baseline  the CPU oracle (C++ restatement of sol's path tracer, OpenMP) on the host's cores,
                 on a bounded sample of the same frame (all pixels, fewer samples per pixel).
 """
@@ -29,6 +32,37 @@ HBM_PEAK_GBS = 8000.0
 # hit) or 4 B (any hit), 64 B per inner-node visit (both child boxes + child words), 48 B per
 # triangle test.
 B_RAY, B_HIT, B_FLAG, B_NODE, B_TRI = 32, 16, 4, 64, 48
+COUNTS = os.path.join(ROOT, "tests", "golden", "traversal_counts.json")
+TRAFFIC = os.path.join(ROOT, "profiles", "k_trace_sphere_field_1080p.pmc.txt")
+COPY_BYTES = 1 << 31
+
+
+def copy_bandwidth(torch):
+    """A plain streaming kernel over COPY_BYTES (reads COPY_BYTES, writes COPY_BYTES: torch.neg into
+    a second buffer), best of 5, in GB/s. The same kernel calibrates the HBM byte counters."""
+    src = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for _ in range(5):
+        e0.record()
+        torch.neg(src, out=dst)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    return 2 * COPY_BYTES / (best * 1e-3) / 1e9
+
+
+def read_traffic():
+    """HBM bytes per k_trace launch from the committed counter summary ("traffic_bytes_per_launch: N")."""
+    if not os.path.exists(TRAFFIC):
+        return None
+    with open(TRAFFIC) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def log(msg):
@@ -43,11 +77,12 @@ def main():
     ap.add_argument("--spp", type=int, default=SPP)
     ap.add_argument("--no-megakernel", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-spp", type=int, default=16)
+    ap.add_argument("--cpu-spp", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each frame renders on one "
-              "MI355X; see DESIGN.md for how the path shards)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. A frame renders on one MI355X; "
+              "across GPUs the work splits by sample ranges or image tiles with one final image sum, which this "
+              "build does not run", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -114,29 +149,32 @@ def main():
         rmse_wm = float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)) / args.steps / args.spp)
         log(f"wavefront vs megakernel: identical pixels {identical:.6f}, RMSE between them {rmse_wm:.3e}")
 
-    # Algorithmic bytes per extension ray from the oracle's instrumented traversal of the same BVH
-    # (every 8th row, 1 spp: the same rays the GPU traces for those samples).
-    from tests import oracle_lib
-    osc = oracle_lib.OracleScene(scene, use_product_bvh=True)
-    ost = oracle_lib.OracleStats()
-    probe = sol_amd.new_image(WIDTH, HEIGHT)
-    for y in range(0, HEIGHT, 8):
-        osc.render(probe, 0, 1, cfg, rows=(y, y + 1), stats=ost)
-    # k_trace processes the extension rays (closest hit, 16 B result) and the shadow rays (any
-    # hit, 4 B result) of one bounce of every path in flight.
-    ext_rays, shadow_rays = ost.extension_rays, ost.shadow_rays
-    b_ext = B_RAY + B_HIT + B_NODE * ost.ext_inner_visits / ext_rays + B_TRI * ost.ext_tri_tests / ext_rays
-    b_shadow = B_RAY + B_FLAG + B_NODE * ost.shadow_inner_visits / shadow_rays + B_TRI * ost.shadow_tri_tests / shadow_rays
+    # Algorithmic bytes per ray: inner-node visits and triangle tests per ray of the same BVH and
+    # rays (committed fixture). k_trace processes the extension rays (closest hit, 16 B result)
+    # and the shadow rays (any hit, 4 B result) of one bounce of every path in flight.
+    with open(COUNTS) as f:
+        counts = json.load(f)["sphere_field_1080p"]
+    sig = counts["bvh"]
+    if (sig["triangles"], sig["nodes"], sig["depth"]) != (scene.num_tris, bst.num_nodes, bst.depth):
+        log(f"warning: BVH differs from the one traversal_counts.json was counted on ({sig}); regenerate it")
+    ce, cs = counts["extension"], counts["shadow"]
+    b_ext = B_RAY + B_HIT + B_NODE * ce["inner_visits_per_ray"] + B_TRI * ce["tri_tests_per_ray"]
+    b_shadow = B_RAY + B_FLAG + B_NODE * cs["inner_visits_per_ray"] + B_TRI * cs["tri_tests_per_ray"]
     launches = max(1, wst.trace_launches)
     bytes_per_launch = (b_ext * wst.extension_rays + b_shadow * wst.shadow_rays) / launches
     launch_ms = wst.trace_ms / launches
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
     log(f"trace kernel: {launches} launches, {launch_ms:.3f} ms avg, {bytes_per_launch / 1e6:.1f} MB algorithmic per launch "
-        f"(extension {b_ext:.0f} B/ray: {ost.ext_inner_visits / ext_rays:.1f} nodes {ost.ext_tri_tests / ext_rays:.2f} tris; "
+        f"(extension {b_ext:.0f} B/ray: {ce['inner_visits_per_ray']:.1f} nodes {ce['tri_tests_per_ray']:.2f} tris; "
         f"shadow {b_shadow:.0f} B/ray) -> {achieved:.0f} GB/s")
+    copy_gbs = copy_bandwidth(torch)
+    traffic = read_traffic()
+    log(f"device copy: {copy_gbs:.0f} GB/s; k_trace HBM traffic per launch: {traffic}")
 
     cpu = None
     if not args.no_cpu:
+        from tests import oracle_lib
+        osc = oracle_lib.OracleScene(scene, use_product_bvh=True)
         threads = min(16, os.cpu_count() or 1)
         cimg = sol_amd.new_image(WIDTH, HEIGHT)
         t = time.perf_counter()
@@ -169,7 +207,8 @@ def main():
         "mrays_s": round(mrays, 1),
         "device": info.name.decode(errors="replace"),
         "roofline": {"bound": "hbm", "kernel": "k_trace", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic) if traffic else None, "copy_gbs": round(copy_gbs, 1),
                      "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(launch_ms, 4),
                      "launches": int(launches), "profiled_ms_per_step": round(prof_s * 1e3, 2),
                      "kernel_ms_per_step": {"trace": round(wst.trace_ms / args.steps, 2),
