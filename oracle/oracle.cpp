@@ -567,7 +567,26 @@ struct Tally {
     uint64_t ext_rays = 0, shadow_rays = 0, bsdf_hits = 0;
 };
 
-Color trace_path(const oracle_scene& s, const sol_pt_config& cfg, Pcg& sampler, Ray ray, Tally& tally) {
+// What one surface vertex of a path contributes, recorded for the adjoint (oracle_backward).
+// Radiance of the path: C = sum_k T_k * (emission_k + nee_k), T_0 = 1, T_{k+1} = T_k * f_k * a_k.
+struct Vertex {
+    int em_light = -1;       // light hit here whose emission applies, -1 = none
+    float em_w = 0.0f;       // its MIS weight: emission_k = Le * em_w
+    Color em_le{0, 0, 0};
+    int mat = -1;            // material of the vertex (-1: none, path ends)
+    int nee_light = -1;      // light of an unoccluded NEE sample, -1 = none
+    Color nee_le{0, 0, 0};
+    Color nee_f{0, 0, 0};    // BSDF value towards the light
+    float nee_rc = 0.0f;     // Phong reflect cosine towards the light
+    float nee_s = 0.0f;      // nee_k = Le * f * nee_s
+    bool cont = false;       // the path continues from here
+    Color f{0, 0, 0};        // BSDF value of the sampled direction
+    float a = 0.0f;          // cos / (pdf * survival probability)
+    float rc = 0.0f;         // Phong reflect cosine of the sampled direction
+};
+
+Color trace_path(const oracle_scene& s, const sol_pt_config& cfg, Pcg& sampler, Ray ray, Tally& tally,
+                 std::vector<Vertex>* rec = nullptr) {
     const bool disable_mis = cfg.flags & SOL_PT_DISABLE_MIS;
     const bool disable_nee = cfg.flags & SOL_PT_DISABLE_NEE;
     const bool disable_rr = cfg.flags & SOL_PT_DISABLE_RR;
@@ -583,6 +602,11 @@ Color trace_path(const oracle_scene& s, const sol_pt_config& cfg, Pcg& sampler, 
         if (!intersect_closest(s, ray, hit, tally.ext)) break;
         Vec3 out_dir = -ray.dir;
         const SurfaceInfo& si = hit.surf_info;
+        Vertex* vx = nullptr;
+        if (rec) {
+            rec->emplace_back();
+            vx = &rec->back();
+        }
 
         if (hit.light >= 0 && si.is_front_side) {
             float pdf_prev_bounce_area = pdf_prev_bounce * dot(out_dir, si.normal()) / (ray.tmax * ray.tmax);
@@ -592,11 +616,18 @@ Color trace_path(const oracle_scene& s, const sol_pt_config& cfg, Pcg& sampler, 
                                    : 1.0f;
             if (disable_mis || disable_nee) mis_weight = pdf_prev_bounce != 0 ? 0.0f : 1.0f;
             color = color + throughput * e.intensity * mis_weight;
+            const sol_light& hl = s.lights[hit.light];
+            if (vx && hl.type == SOL_LIGHT_TRIANGLE && dot(out_dir, vec(hl.normal)) > 0.0f) {
+                vx->em_light = hit.light;
+                vx->em_w = mis_weight;
+                vx->em_le = e.intensity;
+            }
         }
 
         if (hit.bsdf < 0) break;
         const sol_material& bsdf = s.materials[hit.bsdf];
         ++tally.bsdf_hits;
+        if (vx) vx->mat = hit.bsdf;
 
         bool skip_nee = disable_nee || bsdf_type(bsdf) == BsdfType::Specular;
         if (!skip_nee) {
@@ -619,8 +650,16 @@ Color trace_path(const oracle_scene& s, const sol_pt_config& cfg, Pcg& sampler, 
                     float geom_term = ls.cos * inv_light_dist * inv_light_dist;
                     float mis_weight = has_area ? balance_heuristic(pdf_light, pdf_bounce * geom_term) : 1.0f;
                     if (disable_mis) mis_weight = 1.0f;
-                    color = color + ls.intensity * throughput * bsdf_eval(bsdf, in_dir, si, out_dir) *
-                                        (geom_term * cos_surf * mis_weight / pdf_light);
+                    Color f = bsdf_eval(bsdf, in_dir, si, out_dir);
+                    float scale = geom_term * cos_surf * mis_weight / pdf_light;
+                    color = color + ls.intensity * throughput * f * scale;
+                    if (vx) {
+                        vx->nee_light = static_cast<int>(light_index);
+                        vx->nee_le = ls.intensity;
+                        vx->nee_f = f;
+                        vx->nee_rc = reflect_cosine(in_dir, si.normal(), out_dir);
+                        vx->nee_s = scale;
+                    }
                 }
             }
         }
@@ -633,7 +672,14 @@ Color trace_path(const oracle_scene& s, const sol_pt_config& cfg, Pcg& sampler, 
 
         BsdfSample bs;
         if (!bsdf_sample(bsdf, sampler, si, out_dir, bs)) break;
-        throughput = throughput * (bs.color * (bs.cos / (bs.pdf * survival_prob)));
+        float a = bs.cos / (bs.pdf * survival_prob);
+        throughput = throughput * (bs.color * a);
+        if (vx) {
+            vx->cont = true;
+            vx->f = bs.color;
+            vx->a = a;
+            vx->rc = reflect_cosine(bs.in_dir, si.normal(), out_dir);
+        }
         ray = Ray{si.point, bs.in_dir, cfg.ray_offset, FLT_MAX};
         pdf_prev_bounce = skip_nee ? 0.0f : bs.pdf;
     }
@@ -791,6 +837,90 @@ int oracle_render(const oracle_scene* s, const sol_pt_config* cfg, uint32_t w, u
     }
     for (const Tally& t : tallies) accumulate_stats(stats, t, 0);
     if (stats) stats->samples += uint64_t(rows) * w * sample_count;
+    return 0;
+}
+
+// Gradients of sum(d_image * image) with respect to the diffuse albedo and Phong log-exponent of
+// every material and the emission of every light, with sampling decisions, MIS weights and Russian
+// roulette held constant. Reverse mode over each recorded path: the suffix sums
+//   S_k = emission_k + nee_k + f_k * a_k * S_{k+1}
+// give what vertex k's sampled BSDF value multiplies, and the forward throughput T_k what the
+// parameters of vertex k multiply:
+//   d/dtheta (material of k) += T_k * (df_nee/dtheta * Le * nee_s + df_k/dtheta * a_k * S_{k+1})
+//   d/dLe    (emission at k) += T_k * em_w;   d/dLe (NEE light of k) += T_k * f_nee * nee_s
+// with df/dkd_c = 1/pi (diffuse) and df/dlog ns = ks rc^ns ns ((ns + 2) ln rc + 1) / (2 pi) (Phong).
+// Accumulated in double per thread, then added into the float outputs.
+int oracle_backward(const oracle_scene* s, const sol_pt_config* cfg, uint32_t w, uint32_t h, uint64_t sample_index,
+                    uint32_t sample_count, const float* d_image, float* d_materials, float* d_lights, int threads) {
+    if (!s || !cfg || !d_image || !d_lights || !w || !h) return -1;
+    const size_t nm = s->materials.size(), nl = s->lights.size();
+    if (nm && !d_materials) return -1;
+    if (threads <= 0) threads = omp_get_max_threads();
+    const size_t plane = size_t(w) * h;
+    std::vector<std::vector<double>> acc(threads, std::vector<double>(nm * 4 + nl * 3, 0.0));
+    #pragma omp parallel for schedule(dynamic) collapse(2) num_threads(threads)
+    for (long long y = 0; y < static_cast<long long>(h); ++y) {
+        for (long long x = 0; x < static_cast<long long>(w); ++x) {
+            size_t p = size_t(y) * w + x;
+            const Color delta{d_image[p], d_image[plane + p], d_image[2 * plane + p]};
+            if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) continue;
+            std::vector<double>& g = acc[omp_get_thread_num()];
+            auto add3 = [&](size_t base, Color v) {
+                g[base] += double(delta.x) * v.x;
+                g[base + 1] += double(delta.y) * v.y;
+                g[base + 2] += double(delta.z) * v.z;
+            };
+            auto add_material = [&](int mi, Color scaled_T, Color le_s, float rc) {
+                // scaled_T * df/dtheta * le_s, per parameter of material mi
+                const sol_material& m = s->materials[mi];
+                if (m.type == SOL_BSDF_DIFFUSE) {
+                    add3(size_t(mi) * 4, scaled_T * le_s * kInvPi);
+                } else if (m.type == SOL_BSDF_PHONG && rc > 0.0f) {
+                    float dlog = fpow(rc, m.ns) * m.ns * ((m.ns + 2.0f) * std::log(rc) + 1.0f) * (0.5f * kInvPi);
+                    Color v = scaled_T * vec(m.ks) * le_s * dlog;
+                    g[size_t(mi) * 4 + 3] += double(delta.x) * v.x + double(delta.y) * v.y + double(delta.z) * v.z;
+                }
+            };
+            Tally tally;
+            std::vector<Vertex> path;
+            std::vector<Color> suffix;
+            for (uint32_t i = 0; i < sample_count; ++i) {
+                Pcg sampler(pixel_seed(sample_index + i, x, y));
+                float jx = sampler();
+                float jy = sampler();
+                float u = (static_cast<float>(x) + jx) * (2.0f / static_cast<float>(w)) - 1.0f;
+                float v = 1.0f - (static_cast<float>(y) + jy) * (2.0f / static_cast<float>(h));
+                Ray ray{s->eye, normalize(s->cam_dir + s->cam_right * u + s->cam_up * v), 0.0f, FLT_MAX};
+                path.clear();
+                trace_path(*s, *cfg, sampler, ray, tally, &path);
+                const size_t K = path.size();
+                suffix.assign(K + 1, Color{0, 0, 0});   // suffix[k] = S_k
+                for (size_t k = K; k-- > 0;) {
+                    const Vertex& vx = path[k];
+                    Color S = vx.em_le * vx.em_w + vx.nee_le * vx.nee_f * vx.nee_s;
+                    if (vx.cont) S = S + vx.f * vx.a * suffix[k + 1];
+                    suffix[k] = S;
+                }
+                Color T{1, 1, 1};
+                for (size_t k = 0; k < K; ++k) {
+                    const Vertex& vx = path[k];
+                    if (vx.em_light >= 0) add3(nm * 4 + size_t(vx.em_light) * 3, T * vx.em_w);
+                    if (vx.nee_light >= 0) {
+                        add3(nm * 4 + size_t(vx.nee_light) * 3, T * vx.nee_f * vx.nee_s);
+                        add_material(vx.mat, T, vx.nee_le * vx.nee_s, vx.nee_rc);
+                    }
+                    if (vx.cont) {
+                        add_material(vx.mat, T, suffix[k + 1] * vx.a, vx.rc);
+                        T = T * vx.f * vx.a;
+                    }
+                }
+            }
+        }
+    }
+    for (const auto& g : acc) {
+        for (size_t i = 0; i < nm * 4; ++i) d_materials[i] += static_cast<float>(g[i]);
+        for (size_t i = 0; i < nl * 3; ++i) d_lights[i] += static_cast<float>(g[nm * 4 + i]);
+    }
     return 0;
 }
 

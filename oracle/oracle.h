@@ -58,6 +58,12 @@ int oracle_render(const oracle_scene* s, const sol_pt_config* cfg, uint32_t w, u
                   uint32_t sample_count, float* accum, int threads, uint32_t row_begin, uint32_t row_end,
                   oracle_stats* stats);
 
+/* Adjoint of oracle_render: adds d(sum(d_image * image))/d(parameter) into d_materials
+   (num_materials x 4: kd r, g, b, log ns) and d_lights (num_lights x 3: intensity r, g, b), with
+   sampling, MIS weights and Russian roulette held constant. */
+int oracle_backward(const oracle_scene* s, const sol_pt_config* cfg, uint32_t w, uint32_t h, uint64_t sample_index,
+                    uint32_t sample_count, const float* d_image, float* d_materials, float* d_lights, int threads);
+
 uint32_t oracle_pixel_seed(uint64_t sample_index, uint64_t x, uint64_t y);
 /* First n outputs of PcgGenerator(seed) and their floats. */
 void oracle_pcg(uint64_t seed, uint32_t n, uint32_t* out_u32, float* out_float);

@@ -354,6 +354,57 @@ int solhip_render(solhip_scene* s, const sol_pt_config* cfg, int renderer, uint3
     return ctx->check_overflow();
 }
 
+int solhip_backward(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, uint32_t height, uint64_t sample_index,
+                    uint32_t sample_count, const float* d_image, float* d_materials, float* d_lights, int device_ptrs) {
+    if (!s || !cfg || !d_image || !d_lights || (s->num_materials && !d_materials))
+        return fail(SOLHIP_E_INVALID, "null argument");
+    if (width == 0 || height == 0) return fail(SOLHIP_E_INVALID, "empty image");
+    if (cfg->min_survival_prob < 0 || cfg->max_survival_prob > 1 || cfg->min_survival_prob > cfg->max_survival_prob)
+        return fail(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    if (sample_count == 0) return SOLHIP_OK;
+    Context* ctx = s->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t plane = static_cast<size_t>(width) * height;
+    const size_t n_mat = static_cast<size_t>(s->num_materials) * 4, n_light = static_cast<size_t>(s->num_lights) * 3;
+    const float* img = d_image;
+    float* gm = d_materials;
+    float* gl = d_lights;
+    DeviceBuffer tmp_img, tmp_grad;
+    if (!device_ptrs) {
+        HIP_TRY(tmp_img.alloc(sizeof(float) * 3 * plane));
+        HIP_TRY(tmp_grad.alloc(sizeof(float) * (n_mat + n_light + 1)));
+        HIP_TRY(hipMemcpyAsync(tmp_img.ptr, d_image, sizeof(float) * 3 * plane, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemsetAsync(tmp_grad.ptr, 0, sizeof(float) * (n_mat + n_light + 1), ctx->stream));
+        img = static_cast<const float*>(tmp_img.ptr);
+        gm = static_cast<float*>(tmp_grad.ptr);
+        gl = gm + n_mat;
+    }
+    LaunchShape shape = ctx->shape(ctx->tuning.block_size, ctx->tuning.lds_stack);
+    int st = ctx->ensure_scratch(shape.grid * shape.block);
+    if (st) return st;
+    HIP_TRY(launch_zero_u64(ctx->counters, 4, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
+    HIP_TRY(launch_adjoint(s->view, *cfg, width, height, sample_index, sample_count, img, s->num_materials, gm, gl,
+                           ctx->counters, ctx->counters + 1, ctx->scratch, shape, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev_end, ctx->stream));
+    unsigned long long c[4];
+    st = ctx->read_counters(c, 4);
+    if (st) return st;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+    ctx->stats.render_ms += ms;
+    ctx->stats.extension_rays += c[1];
+    ctx->stats.shadow_rays += c[2];
+    ctx->stats.launches += 1;
+    if (!device_ptrs) {
+        std::vector<float> g(n_mat + n_light);
+        HIP_TRY(hipMemcpy(g.data(), gm, sizeof(float) * (n_mat + n_light), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n_mat; ++i) d_materials[i] += g[i];
+        for (size_t i = 0; i < n_light; ++i) d_lights[i] += g[n_mat + i];
+    }
+    return ctx->check_overflow();
+}
+
 } // extern "C"
 
 namespace solhip {

@@ -34,6 +34,13 @@ hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint3
                              unsigned long long* work_counter, StatCounters stats, const TraversalScratch& scratch,
                              const LaunchShape& shape, hipStream_t stream);
 
+// Adjoint path replay: adds dLoss/dparameter into d_mat (num_materials x 4) and d_light
+// (num_lights x 3) for the loss gradient d_image (planar RGB, width x height).
+hipError_t launch_adjoint(const SceneView& s, const sol_pt_config& cfg, uint32_t width, uint32_t height,
+                          uint64_t sample_index, uint32_t sample_count, const float* d_image, uint32_t num_materials,
+                          float* d_mat, float* d_light, unsigned long long* work_counter, StatCounters stats,
+                          const TraversalScratch& scratch, const LaunchShape& shape, hipStream_t stream);
+
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t stream);
 
 // Workgroups for a persistent launch: enough for every CU at full occupancy.

@@ -48,7 +48,8 @@ SOL_DEV bool is_specular(int type) { return type == SOL_BSDF_GLASS || type == SO
 
 // Emission of a front-side hit on a triangle light, weighted against the BSDF sample that
 // found it (path_tracer.cpp:49-60; lights.cpp:94-106). Returns T * Le * w, the radiance added
-// to the path (0 if none); Le_out and w_out receive the emission and the MIS weight.
+// to the path (0 if none); Le_out and w_out receive the emission and the MIS weight (w_out = 0
+// when the light emits nothing towards the ray, so that T * w_out is dradiance / dLe).
 SOL_DEV V3 hit_emission(const SceneView& s, uint32_t flags, const Surface& sf, V3 dir, float t, float pdf_prev,
                         float pick_prob, V3 T, V3& Le_out, float& w_out) {
     Le_out = V3{0.0f, 0.0f, 0.0f};
@@ -59,17 +60,19 @@ SOL_DEV V3 hit_emission(const SceneView& s, uint32_t flags, const Surface& sf, V
     const sol_light& L = s.lights[sf.light];
     V3 Le{0.0f, 0.0f, 0.0f};
     float pdf_from = 1.0f;
+    bool emits = false;
     if (L.type == SOL_LIGHT_TRIANGLE) {
         float c = dot(out, load3(L.normal));
         if (c > 0.0f) {
             Le = load3(L.intensity);
             pdf_from = L.inv_area;
+            emits = true;
         }
     }
     float w = pdf_prev != 0.0f ? balance_heuristic(pdf_prev_area, pdf_from * pick_prob) : 1.0f;
     if (flags & (SOL_PT_DISABLE_MIS | SOL_PT_DISABLE_NEE)) w = pdf_prev != 0.0f ? 0.0f : 1.0f;
     Le_out = Le;
-    w_out = w;
+    w_out = emits ? w : 0.0f;
     return T * Le * w;
 }
 
@@ -182,6 +185,7 @@ struct NeeSample {
     int light;
     V3 f;            // BSDF value towards the light
     float rc;        // Phong reflect cosine towards the light
+    float scale;     // G cos_surf w / pdf_L: contrib = Le T f scale
 };
 
 // Light pick + light sample + everything of path_tracer.cpp:66-97 that does not depend on the
@@ -229,7 +233,8 @@ SOL_DEV bool nee_sample(const SceneView& s, uint32_t flags, const Surface& sf, c
     V3 f = bsdf_eval(m, in, sf, out);
     ns.f = f;
     ns.rc = m.type == SOL_BSDF_PHONG ? reflect_cosine(in, sf.n, out) : 0.0f;
-    ns.contrib = Le * T * f * (G * cos_surf * w / pdf_L);
+    ns.scale = G * cos_surf * w / pdf_L;
+    ns.contrib = Le * T * f * ns.scale;
     return true;
 }
 

@@ -403,11 +403,6 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
     return ctx->check_overflow();
 }
 
-int solhip_backward(solhip_scene*, const sol_pt_config*, uint32_t, uint32_t, uint64_t, uint32_t, const float*, float*,
-                    float*, int) {
-    return set_error(SOLHIP_E_INVALID, "adjoint not built");
-}
-
 int solhip_autotune(solhip_scene*, const sol_pt_config*, uint32_t, uint32_t, uint32_t, solhip_tuning*) {
     return set_error(SOLHIP_E_INVALID, "autotuner not built");
 }

@@ -203,6 +203,14 @@ class GpuScene:
                     "solhip_backward")
         return dm, dl
 
+    def backward_device(self, d_image_ptr, width, height, sample_index, sample_count, d_materials_ptr, d_lights_ptr,
+                        config=None):
+        """solhip_backward on device buffers: d_image (3, height, width), d_materials (M, 4) and
+        d_lights (L, 3) float32; the gradients are added to the last two."""
+        cfg = (config or PathTracerConfig()).to_c()
+        F.check_hip(F.hip.solhip_backward(self._h, C.byref(cfg), width, height, sample_index, sample_count,
+                                          d_image_ptr, d_materials_ptr, d_lights_ptr, 1), "solhip_backward")
+
     def autotune(self, width, height, budget_spp=1, config=None):
         cfg = (config or PathTracerConfig()).to_c()
         t = F.Tuning()

@@ -27,6 +27,8 @@ lib.oracle_scene_set_lights.argtypes = [_P, C.POINTER(F.Light), C.c_uint32]
 lib.oracle_intersect.argtypes = [_P, _P, C.c_uint64, C.c_int, C.c_int, _P, C.POINTER(OracleStats)]
 lib.oracle_render.argtypes = [_P, C.POINTER(F.PtConfig), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, _P, C.c_int,
                               C.c_uint32, C.c_uint32, C.POINTER(OracleStats)]
+lib.oracle_backward.argtypes = [_P, C.POINTER(F.PtConfig), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, _P, _P, _P,
+                                C.c_int]
 lib.oracle_pixel_seed.restype = C.c_uint32
 lib.oracle_pixel_seed.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
 lib.oracle_pcg.restype = None
@@ -89,6 +91,17 @@ class OracleScene:
         assert lib.oracle_render(self._h, C.byref(cfg), image.shape[2], image.shape[1], sample_index, sample_count,
                                  image.ctypes.data, threads, rows[0], rows[1], C.byref(st)) == 0
         return st
+
+
+def backward(oscene, d_image, sample_index, sample_count, config, num_materials, num_lights, threads=0):
+    """oracle_backward: (d_materials (M, 4), d_lights (L, 3)) of sum(d_image * image)."""
+    d_image = np.ascontiguousarray(d_image, np.float32)
+    dm = np.zeros((num_materials, 4), np.float32)
+    dl = np.zeros((num_lights, 3), np.float32)
+    cfg = config.to_c()
+    assert lib.oracle_backward(oscene._h, C.byref(cfg), d_image.shape[2], d_image.shape[1], sample_index, sample_count,
+                               d_image.ctypes.data, dm.ctypes.data, dl.ctypes.data, threads) == 0
+    return dm, dl
 
 
 def pcg(seed, n):
