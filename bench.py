@@ -16,6 +16,12 @@ Also reported:
                 workload, if present. "copy_gbs" is a device-to-device copy timed here.
   cpu_baseline  the CPU oracle (C++ restatement of sol's path tracer, OpenMP) on the host's cores,
                 on a bounded sample of the same frame (all pixels, fewer samples per pixel).
+  batched       64 uneven sphere-field scenes (sol_amd.workloads.batch_specs) rendered by one
+                solhip_render_batch call after solhip_autotune on the largest job, against the same
+                jobs rendered one by one with the megakernel.
+  adjoint       solhip_backward on the sphere field with 256 material groups at 512x512, 32 spp
+                (Russian roulette off), with a central-difference check (h = 1e-3, common random
+                numbers) of the largest albedo, Phong log-exponent and emission gradients.
 """
 import argparse
 import json
@@ -69,6 +75,98 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def bench_batched(sol_amd, F, torch, steps):
+    from sol_amd.workloads import BATCH_DEPTH, batch_specs
+    specs = batch_specs()
+    t = time.perf_counter()
+    scenes = [sp.scene() for sp in specs]
+    build_s = time.perf_counter() - t
+    ctx = sol_amd.Context(0)
+    gs = [ctx.upload(sc) for sc in scenes]
+    cfg = sol_amd.PathTracerConfig(max_path_len=BATCH_DEPTH)
+    big = max(range(len(specs)), key=lambda i: specs[i].width * specs[i].height * specs[i].spp)
+    t = time.perf_counter()
+    tuned = gs[big].autotune(specs[big].width, specs[big].height, budget_spp=2, config=cfg)
+    tune_s = time.perf_counter() - t
+    bufs = [torch.zeros((3, sp.height, sp.width), dtype=torch.float32, device="cuda") for sp in specs]
+    jobs = [(g, sp.width, sp.height, 0, sp.spp, b.data_ptr()) for sp, g, b in zip(specs, gs, bufs)]
+    samples = sum(sp.width * sp.height * sp.spp for sp in specs)
+    torch.cuda.synchronize()
+    ctx.render_batch(jobs, cfg)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        ctx.render_batch(jobs, cfg)
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - t) / steps
+    t = time.perf_counter()
+    for sp, g, b in zip(specs, gs, bufs):
+        g.render_device(b.data_ptr(), sp.width, sp.height, 0, sp.spp, cfg, renderer=F.RENDERER_MEGAKERNEL)
+    torch.cuda.synchronize()
+    mega_s = time.perf_counter() - t
+    out = {"scenes": len(specs), "triangles": int(sum(sc.num_tris for sc in scenes)), "samples": samples,
+           "msamples_s": round(samples / batch_s / 1e6, 2), "ms": round(batch_s * 1e3, 2),
+           "megakernel_one_by_one_msamples_s": round(samples / mega_s / 1e6, 2),
+           "tuned": {k: getattr(tuned, k) for k in ("block_size", "lds_stack", "path_slots", "tile_w", "tile_h",
+                                                   "graph_iters")},
+           "build_s": round(build_s, 1), "autotune_s": round(tune_s, 1)}
+    log(f"batched: {out}")
+    for g in gs:
+        g.close()
+    ctx.close()
+    return out
+
+
+def bench_adjoint(sol_amd, torch):
+    import numpy as np
+    from sol_amd.diff import DiffRenderer
+    w = h = 512
+    spp = 32
+    scene = sol_amd.Scene.sphere_field(num_groups=256, aspect=1.0)
+    ctx = sol_amd.Context(0)
+    cfg = sol_amd.PathTracerConfig(max_path_len=DEPTH, disable_rr=True)
+    dr = DiffRenderer(ctx.upload(scene), w, h, 0, spp, cfg)
+    albedo, log_ns, le = dr.parameters()
+    rng = np.random.default_rng(0)
+    target = dr.render_with(np.clip(albedo + rng.normal(0, 0.1, albedo.shape), 0.02, 0.98).astype(np.float32),
+                            (log_ns + rng.normal(0, 0.1, log_ns.shape)).astype(np.float32),
+                            np.maximum(le + rng.normal(0, 0.1, le.shape), 0).astype(np.float32)).double()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    img = dr.render_with(albedo, log_ns, le)
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t
+    d_image = (2 * (img.double() - target)).float()
+    dr.gradients(d_image, albedo, log_ns, le)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    da, dn, de = dr.gradients(d_image, albedo, log_ns, le)
+    torch.cuda.synchronize()
+    bwd_s = time.perf_counter() - t
+    da, dn, de = da.cpu().numpy(), dn.cpu().numpy(), de.cpu().numpy()
+
+    def loss(a, n, e):
+        return float(((dr.render_with(a, n, e).double() - target) ** 2).sum())
+
+    checks = []
+    for k, grad in ((0, da), (1, dn), (2, de)):
+        idx = np.unravel_index(int(np.argmax(np.abs(grad))), grad.shape)
+        plus = [albedo.copy(), log_ns.copy(), le.copy()]
+        minus = [albedo.copy(), log_ns.copy(), le.copy()]
+        plus[k][idx] += 1e-3
+        minus[k][idx] -= 1e-3
+        fd = (loss(*plus) - loss(*minus)) / 2e-3
+        checks.append(abs(float(grad[idx]) - fd) / abs(fd))
+    samples = w * h * spp
+    out = {"workload": "sphere field, 256 material groups, 512x512, 32 spp, depth 16, Russian roulette off",
+           "backward_ms": round(bwd_s * 1e3, 2), "backward_msamples_s": round(samples / bwd_s / 1e6, 2),
+           "forward_ms": round(fwd_s * 1e3, 2), "fd_params_checked": len(checks),
+           "fd_max_rel_err": float(f"{max(checks):.3g}")}
+    log(f"adjoint: {out}")
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +176,8 @@ def main():
     ap.add_argument("--no-megakernel", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. A frame renders on one MI355X; "
@@ -171,6 +271,9 @@ def main():
     traffic = read_traffic()
     log(f"device copy: {copy_gbs:.0f} GB/s; k_trace HBM traffic per launch: {traffic}")
 
+    batched = None if args.no_batch else bench_batched(sol_amd, F, torch, args.steps)
+    adjoint = None if args.no_adjoint else bench_adjoint(sol_amd, torch)
+
     cpu = None
     if not args.no_cpu:
         from tests import oracle_lib
@@ -215,6 +318,8 @@ def main():
                                             "sort": round(wst.sort_ms / args.steps, 2),
                                             "shade": round(wst.shade_ms / args.steps, 2)}},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "adjoint": adjoint,
         "setup_s": round(time.time() - t0, 1),
     }
     print(json.dumps(out), flush=True)

@@ -403,8 +403,83 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
     return ctx->check_overflow();
 }
 
-int solhip_autotune(solhip_scene*, const sol_pt_config*, uint32_t, uint32_t, uint32_t, solhip_tuning*) {
-    return set_error(SOLHIP_E_INVALID, "autotuner not built");
+// Launch-parameter search for the wavefront renderer on one scene and image size: coordinate
+// descent from the current parameters over paths in flight, traversal workgroup size, LDS stack
+// depth, bounce iterations per graph and tile shape. Every candidate renders budget_spp samples
+// per pixel twice into a scratch image (the first render builds the slot arena and captures the
+// graph) and the second is timed with HIP events. A candidate replaces the current best if it is
+// at least 1 % faster. The statistics and the profiling switch of the context are restored.
+int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, uint32_t height, uint32_t budget_spp,
+                    solhip_tuning* out) {
+    if (!s || !cfg) return set_error(SOLHIP_E_INVALID, "null argument");
+    if (width == 0 || height == 0) return set_error(SOLHIP_E_INVALID, "empty image");
+    if (cfg->min_survival_prob < 0 || cfg->max_survival_prob > 1 || cfg->min_survival_prob > cfg->max_survival_prob)
+        return set_error(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    if (budget_spp == 0) budget_spp = 1;
+    Context& ctx = *s->ctx;
+    HIP_OK(hipSetDevice(ctx.device));
+    const size_t plane = static_cast<size_t>(width) * height;
+    DeviceBuffer img;
+    HIP_OK(img.alloc(sizeof(float) * 3 * plane));
+    const solhip_stats saved_stats = ctx.stats;
+    const bool saved_profiling = ctx.profiling;
+    ctx.profiling = false;
+    const solhip_tuning start = ctx.tuning;
+
+    auto trial = [&](const solhip_tuning& t, float& ms) -> int {
+        ctx.tuning = t;
+        int rc = render_wavefront(*s, *cfg, width, height, 0, budget_spp, img.as<float>());
+        if (rc) return rc;
+        HIP_OK(hipEventRecord(ctx.ev_begin, ctx.stream));
+        rc = render_wavefront(*s, *cfg, width, height, 0, budget_spp, img.as<float>());
+        if (rc) return rc;
+        HIP_OK(hipEventRecord(ctx.ev_end, ctx.stream));
+        HIP_OK(hipEventSynchronize(ctx.ev_end));
+        HIP_OK(hipEventElapsedTime(&ms, ctx.ev_begin, ctx.ev_end));
+        return SOLHIP_OK;
+    };
+
+    // Paths in flight above the work of one render only add idle slots.
+    const unsigned long long items = static_cast<unsigned long long>(plane) * budget_spp;
+    std::vector<uint32_t> slot_cands;
+    for (uint32_t c : {1u << 21, 1u << 22, 1u << 23, 1u << 24}) {
+        uint32_t v = static_cast<uint32_t>(std::min<unsigned long long>(c, std::max<unsigned long long>(4096, items)));
+        if (std::find(slot_cands.begin(), slot_cands.end(), v) == slot_cands.end()) slot_cands.push_back(v);
+    }
+    solhip_tuning best = start;
+    float best_ms = 0.0f;
+    int rc = trial(best, best_ms);
+    for (int dim = 0; rc == SOLHIP_OK && dim < 5; ++dim) {
+        std::vector<solhip_tuning> cands;
+        solhip_tuning t = best;
+        switch (dim) {
+            case 0: for (uint32_t v : slot_cands) { t.path_slots = v; cands.push_back(t); } break;
+            case 1: for (uint32_t v : {128u, 256u}) { t.block_size = v; cands.push_back(t); } break;
+            case 2: for (uint32_t v : {8u, 16u}) { t.lds_stack = v; cands.push_back(t); } break;
+            case 3: for (uint32_t v : {8u, 16u, 32u}) { t.graph_iters = v; cands.push_back(t); } break;
+            default:
+                for (auto wh : {std::pair<uint32_t, uint32_t>{16, 16}, {32, 32}, {64, 16}, {128, 8}}) {
+                    t.tile_w = wh.first;
+                    t.tile_h = wh.second;
+                    cands.push_back(t);
+                }
+        }
+        for (const solhip_tuning& c : cands) {
+            if (std::memcmp(&c, &best, sizeof(c)) == 0) continue;
+            float ms = 0.0f;
+            if ((rc = trial(c, ms)) != SOLHIP_OK) break;
+            if (ms < 0.99f * best_ms) {
+                best = c;
+                best_ms = ms;
+            }
+        }
+    }
+    ctx.tuning = rc == SOLHIP_OK ? best : start;
+    ctx.stats = saved_stats;
+    ctx.profiling = saved_profiling;
+    if (rc) return rc;
+    if (out) *out = best;
+    return SOLHIP_OK;
 }
 
 } // extern "C"

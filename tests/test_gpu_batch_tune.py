@@ -1,0 +1,69 @@
+"""Batched rendering of the uneven multi-scene workload and the launch-parameter autotuner.
+
+Images do not depend on launch parameters or on batching: every sample follows its own
+(pixel, sample index) random stream and the per-pixel sums run in sample order, so a batch equals
+the single renders and a tuned context renders the same image bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+import sol_amd
+from sol_amd import _ffi as F
+from sol_amd.workloads import BATCH_DEPTH, batch_specs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return sol_amd.Context(0)
+
+
+def test_batch_workload_equals_single_renders(ctx):
+    specs = batch_specs(count=8, seed=7, max_spheres=300, sides=(64, 128, 256), spps=(1, 4))
+    cfg = sol_amd.PathTracerConfig(max_path_len=BATCH_DEPTH)
+    gs = [ctx.upload(sp.scene()) for sp in specs]
+    want = []
+    for sp, g in zip(specs, gs):
+        img = sol_amd.new_image(sp.width, sp.height)
+        g.render(img, 0, sp.spp, cfg, renderer=F.RENDERER_MEGAKERNEL)
+        want.append(img)
+    bufs = [torch.zeros((3, sp.height, sp.width), dtype=torch.float32, device="cuda") for sp in specs]
+    torch.cuda.synchronize()
+    ctx.render_batch([(g, sp.width, sp.height, 0, sp.spp, b.data_ptr()) for sp, g, b in zip(specs, gs, bufs)], cfg)
+    for sp, b, wimg in zip(specs, bufs, want):
+        got = b.cpu().numpy()
+        assert got.any()
+        np.testing.assert_array_equal(got, wimg)
+
+
+def test_autotune_returns_valid_parameters_and_keeps_the_image(ctx):
+    scene = sol_amd.Scene.sphere_field(num_spheres=40, subdivision=2, seed=3, extent=4.0, radius_max=1.0)
+    g = ctx.upload(scene)
+    cfg = sol_amd.PathTracerConfig(max_path_len=8)
+    before = sol_amd.new_image(160, 90)
+    g.render(before, 0, 4, cfg)
+    ctx.reset_stats()
+    t = g.autotune(160, 90, budget_spp=2, config=cfg)
+    assert t.block_size in (128, 256) and t.lds_stack in (8, 16)
+    assert 4096 <= t.path_slots <= (1 << 24) and t.graph_iters in (8, 16, 32)
+    assert (t.tile_w, t.tile_h) in ((16, 16), (32, 32), (64, 16), (128, 8))
+    cur = ctx.tuning()
+    assert (cur.block_size, cur.lds_stack, cur.path_slots, cur.graph_iters, cur.tile_w, cur.tile_h) == \
+        (t.block_size, t.lds_stack, t.path_slots, t.graph_iters, t.tile_w, t.tile_h)
+    assert ctx.stats().samples == 0          # trial renders are not counted
+    after = sol_amd.new_image(160, 90)
+    g.render(after, 0, 4, cfg)
+    np.testing.assert_array_equal(after, before)
+    ctx.set_tuning(**{k: 0 for k in ("block_size", "lds_stack", "path_slots", "tile_w", "tile_h", "graph_iters")})
+
+
+def test_tuning_rejects_unsupported_values(ctx):
+    with pytest.raises(F.SolError):
+        ctx.set_tuning(block_size=96)
+    with pytest.raises(F.SolError):
+        ctx.set_tuning(path_slots=10)
+    g = ctx.upload(sol_amd.Scene.cornell_box())
+    with pytest.raises(F.SolError):
+        g.autotune(0, 16)
