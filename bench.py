@@ -16,6 +16,9 @@ Also reported:
                 workload, if present. "copy_gbs" is a device-to-device copy timed here.
   cpu_baseline  the CPU oracle (C++ restatement of sol's path tracer, OpenMP) on the host's cores,
                 on a bounded sample of the same frame (all pixels, fewer samples per pixel).
+  intersect     the intersection microbenchmark: 1M-triangle soup, 4,194,304 closest-hit rays and
+                4,194,304 any-hit rays (device buffers), Mrays/s; the oracle on 262,144 of each.
+  cornell       the Cornell box at 256x256, 16 spp, depth 8: both GPU renderers and the oracle.
   batched       64 uneven sphere-field scenes (sol_amd.workloads.batch_specs) rendered by one
                 solhip_render_batch call after solhip_autotune on the largest job, against the same
                 jobs rendered one by one with the megakernel.
@@ -73,6 +76,73 @@ def read_traffic():
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def bench_intersect(sol_amd, torch, with_cpu):
+    n = 1 << 22
+    t = time.perf_counter()
+    scene = sol_amd.Scene.triangle_soup(1_000_000)
+    build_s = time.perf_counter() - t
+    ctx = sol_amd.Context(0)
+    g = ctx.upload(scene)
+    out = {"triangles": scene.num_tris, "rays": n, "build_s": round(build_s, 2)}
+    for kind, name in ((0, "closest"), (1, "any")):
+        rays = sol_amd.gen_rays(kind, n)
+        d_rays = torch.from_numpy(rays).cuda()
+        d_out = torch.empty((n, 4) if kind == 0 else (n,), dtype=torch.float32 if kind == 0 else torch.int32,
+                            device="cuda")
+        torch.cuda.synchronize()
+        g.intersect_device(d_rays.data_ptr(), n, d_out.data_ptr(), any_hit=kind == 1)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            g.intersect_device(d_rays.data_ptr(), n, d_out.data_ptr(), any_hit=kind == 1)
+        torch.cuda.synchronize()
+        out[f"{name}_mrays_s"] = round(n * reps / (time.perf_counter() - t) / 1e6, 1)
+        if with_cpu:
+            from tests import oracle_lib
+            osc = oracle_lib.OracleScene(scene, use_product_bvh=True)
+            m = 1 << 18
+            t = time.perf_counter()
+            osc.intersect(rays[:m], any_hit=kind == 1)
+            out[f"cpu_{name}_mrays_s"] = round(m / (time.perf_counter() - t) / 1e6, 2)
+    log(f"intersect: {out}")
+    g.close()
+    ctx.close()
+    return out
+
+
+def bench_cornell(sol_amd, F, torch, with_cpu):
+    w = h = 256
+    spp = 16
+    scene = sol_amd.Scene.cornell_box(aspect=1.0)
+    ctx = sol_amd.Context(0)
+    g = ctx.upload(scene)
+    cfg = sol_amd.PathTracerConfig(max_path_len=8)
+    img = torch.zeros((3, h, w), dtype=torch.float32, device="cuda")
+    out = {"width": w, "height": h, "spp": spp, "max_path_len": 8}
+    for name, r in (("wavefront", F.RENDERER_WAVEFRONT), ("megakernel", F.RENDERER_MEGAKERNEL)):
+        g.render_device(img.data_ptr(), w, h, 0, spp, cfg, renderer=r)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            g.render_device(img.data_ptr(), w, h, 0, spp, cfg, renderer=r)
+        torch.cuda.synchronize()
+        out[f"{name}_msamples_s"] = round(w * h * spp * reps / (time.perf_counter() - t) / 1e6, 1)
+    if with_cpu:
+        from tests import oracle_lib
+        threads = min(16, os.cpu_count() or 1)
+        cimg = sol_amd.new_image(w, h)
+        t = time.perf_counter()
+        oracle_lib.OracleScene(scene).render(cimg, 0, spp, cfg, threads=threads)
+        out["cpu_msamples_s"] = round(w * h * spp / (time.perf_counter() - t) / 1e6, 2)
+        out["cpu_threads"] = threads
+    log(f"cornell: {out}")
+    g.close()
+    ctx.close()
+    return out
 
 
 def bench_batched(sol_amd, F, torch, steps):
@@ -177,6 +247,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the intersection and Cornell box legs")
     ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -271,6 +342,8 @@ def main():
     traffic = read_traffic()
     log(f"device copy: {copy_gbs:.0f} GB/s; k_trace HBM traffic per launch: {traffic}")
 
+    intersect = None if args.no_small else bench_intersect(sol_amd, torch, not args.no_cpu)
+    cornell = None if args.no_small else bench_cornell(sol_amd, F, torch, not args.no_cpu)
     batched = None if args.no_batch else bench_batched(sol_amd, F, torch, args.steps)
     adjoint = None if args.no_adjoint else bench_adjoint(sol_amd, torch)
 
@@ -318,6 +391,8 @@ def main():
                                             "sort": round(wst.sort_ms / args.steps, 2),
                                             "shade": round(wst.shade_ms / args.steps, 2)}},
         "cpu_baseline": cpu,
+        "intersect": intersect,
+        "cornell": cornell,
         "batched": batched,
         "adjoint": adjoint,
         "setup_s": round(time.time() - t0, 1),

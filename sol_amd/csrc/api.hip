@@ -197,6 +197,10 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     if (!d->num_lights || !d->lights) return fail(SOLHIP_E_NO_LIGHTS, "scene without light");
     if (d->num_materials && !d->materials) return fail(SOLHIP_E_INVALID, "null material table");
     if (d->num_tris >= (1u << 28)) return fail(SOLHIP_E_INVALID, "too many triangles");
+    // traversal addresses nodes and triangles with 32-bit byte offsets
+    if (static_cast<uint64_t>(d->num_tris) * sizeof(sol_gpu_tri) > 0xFFFFFFFFull ||
+        static_cast<uint64_t>(d->num_nodes) * sizeof(sol_gpu_node) > 0xFFFFFFFFull)
+        return fail(SOLHIP_E_INVALID, "scene larger than 4 GiB of nodes or triangles");
     if (d->bvh_depth > static_cast<uint32_t>(8 + kSpillEntries))
         return fail(SOLHIP_E_TOO_DEEP, "BVH depth " + std::to_string(d->bvh_depth) + " exceeds the traversal stack");
     for (uint32_t i = 0; i < d->num_tris; ++i) {

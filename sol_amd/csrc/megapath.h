@@ -22,13 +22,7 @@ SOL_DEV void wave_add(unsigned long long* counter, unsigned long long v) {
 template <int BLOCK, int LDS>
 SOL_DEV Stack make_stack(int* lds_stack, const TraversalScratch& sc) {
     int gtid = blockIdx.x * BLOCK + threadIdx.x;
-    Stack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.spill = sc.spill + gtid;
-    st.spill_stride = sc.max_threads;
-    st.capacity = LDS + sc.spill_entries;
-    st.overflow = sc.overflow;
-    return st;
+    return Stack::make(lds_stack, threadIdx.x, sc.spill, gtid, sc.max_threads, LDS + sc.spill_entries, sc.overflow);
 }
 
 // One sample of PathTracer::trace_path (src/algorithms/path_tracer.cpp:31-120 of sol): returns its

@@ -20,18 +20,38 @@ namespace solhip {
 // Maximum number of top nodes copied to LDS (4 KB: six levels).
 constexpr int kTopNodesMax = 63;
 
+// Word 3 of a raw buffer descriptor on gfx9-family parts (32-bit data format, no swizzle).
+constexpr int kBufferFlags = 0x00020000;
+
 struct HitRecord {
     int slot;   // leaf-order triangle index, -1 = miss
     int prim;   // original triangle index
     float t, u, v;
 };
 
+typedef __attribute__((address_space(3))) int lds_int;     // an int in LDS
+typedef __attribute__((address_space(1))) int global_int;  // an int in device memory
+
 struct Stack {
-    int* lds;          // this lane's first entry
-    int* spill;        // this lane's first spill entry
-    int spill_stride;  // distance between consecutive spill entries
-    int capacity;      // LDS + spill entries
-    int* overflow;     // set to 1 if a push would exceed the capacity
+    lds_int* lds;        // this lane's first entry
+    global_int* spill;   // this lane's first spill entry
+    int spill_stride;    // distance between consecutive spill entries
+    int capacity;        // LDS + spill entries
+    global_int* overflow;  // set to 1 if a push would exceed the capacity
+
+    // Entries live in LDS (lds_stack, entry e of lane l at lds_stack[e * block + l]) and spill to
+    // the lane's column of the global area. Explicit address spaces keep the two paths apart
+    // (ds_read / global_load rather than flat loads).
+    SOL_DEV static Stack make(int* lds_stack, int lane_in_block, int* spill_base, int lane_global, int spill_stride,
+                              int capacity, int* overflow) {
+        Stack st;
+        st.lds = (lds_int*)(lds_stack + lane_in_block);
+        st.spill = (global_int*)(spill_base + lane_global);
+        st.spill_stride = spill_stride;
+        st.capacity = capacity;
+        st.overflow = (global_int*)(overflow);
+        return st;
+    }
 };
 
 template <int BLOCK>
@@ -41,10 +61,166 @@ SOL_DEV void load_top_nodes(const SceneView& s, float4* lds_nodes) {
     __syncthreads();
 }
 
-template <bool ANY, int BLOCK, int LDS_STACK>
-SOL_DEV bool traverse(const SceneView& s, const float4* lds_nodes, const Stack& st, const RayPre& r,
-                      float tmin, float tmax, HitRecord& hit) {
-    const int top = static_cast<int>(s.top_nodes < static_cast<uint32_t>(kTopNodesMax) ? s.top_nodes : kTopNodesMax);
+// Node and triangle fetch of a scene that is the same for the whole wave: the top levels from the
+// workgroup's LDS copy (ds_read), the rest through buffer loads with 32-bit byte offsets from a
+// scalar buffer descriptor (the hardware returns zeros beyond num_records, never faults).
+struct FetchUniform {
+    __amdgpu_buffer_rsrc_t nodes, tris;
+    const float4* lds;
+    int top;
+
+    SOL_DEV static __amdgpu_buffer_rsrc_t rsrc(const void* p, uint32_t bytes) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, static_cast<int>(bytes), kBufferFlags);
+    }
+    SOL_DEV FetchUniform(const SceneView& s, const float4* lds_nodes)
+        : nodes(rsrc(s.nodes, s.num_nodes * 64u)), tris(rsrc(s.tris, s.num_tris * 48u)), lds(lds_nodes),
+          top(static_cast<int>(s.top_nodes < static_cast<uint32_t>(kTopNodesMax) ? s.top_nodes : kTopNodesMax)) {}
+
+    SOL_DEV static float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(off), 0, 0);
+        return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+    }
+    SOL_DEV static float4 ld3(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b96(r, static_cast<int>(off), 0, 0);
+        return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), 0.0f);
+    }
+    SOL_DEV void node(int cur, float4& a, float4& b, float4& c, float4& d) const {
+        if (cur < top) {
+            a = lds[cur * 4 + 0];
+            b = lds[cur * 4 + 1];
+            c = lds[cur * 4 + 2];
+            d = lds[cur * 4 + 3];
+        } else {
+            uint32_t o = static_cast<uint32_t>(cur) * 64u;
+            a = ld4(nodes, o);
+            b = ld4(nodes, o + 16);
+            c = ld4(nodes, o + 32);
+            d = ld4(nodes, o + 48);
+        }
+    }
+    // v0 and the original triangle index (w), v1, v2
+    SOL_DEV void tri(uint32_t i, float4& p0, float4& p1, float4& p2) const {
+        uint32_t o = i * 48u;
+        p0 = ld4(tris, o);
+        p1 = ld3(tris, o + 16);
+        p2 = ld3(tris, o + 32);
+    }
+};
+
+// Node and triangle fetch of a scene that may differ between the lanes of a wave (batches):
+// plain global loads (pointers in the global address space, so no flat loads).
+struct FetchGlobal {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    typedef const __attribute__((address_space(1))) v4f* gptr;
+    gptr nodes, tris;
+
+    SOL_DEV explicit FetchGlobal(const SceneView& s) : nodes((gptr)(s.nodes)), tris((gptr)(s.tris)) {}
+    SOL_DEV static float4 f4(v4f v) { return make_float4(v.x, v.y, v.z, v.w); }
+    SOL_DEV void node(int cur, float4& a, float4& b, float4& c, float4& d) const {
+        gptr p = nodes + cur * 4;
+        a = f4(p[0]);
+        b = f4(p[1]);
+        c = f4(p[2]);
+        d = f4(p[3]);
+    }
+    SOL_DEV void tri(uint32_t i, float4& p0, float4& p1, float4& p2) const {
+        gptr p = tris + i * 3;
+        p0 = f4(p[0]);
+        p1 = f4(p[1]);
+        p2 = f4(p[2]);
+    }
+};
+
+// State of one lane's traversal between steps.
+struct TraversalState {
+    int node;        // next node (>= 0 inner, < 0 leaf reference)
+    int sp;          // stack depth
+    float tmax;      // shrinks with closest hits
+    HitRecord hit;
+
+    SOL_DEV void start(float t_max) {
+        node = 0;
+        sp = 0;
+        tmax = t_max;
+        hit.slot = -1;
+        hit.prim = -1;
+    }
+};
+
+enum : int { kStepContinue = 0, kStepDone = 1, kStepOccluded = 2 };
+
+template <int BLOCK, int LDS_STACK>
+SOL_DEV bool stack_pop(const Stack& st, TraversalState& ts) {
+    if (ts.sp == 0) return false;
+    --ts.sp;
+    if (ts.sp < LDS_STACK) ts.node = st.lds[ts.sp * BLOCK];
+    else ts.node = st.spill[(ts.sp - LDS_STACK) * st.spill_stride];
+    return true;
+}
+
+// One round of a lane's traversal: descend through inner nodes (both child boxes; the nearer hit
+// child next, the farther pushed; pop when neither is hit) until a leaf, test the leaf's
+// triangles, pop. Returns kStepDone when the stack is empty (closest hit in ts.hit, if any),
+// kStepOccluded when an any-hit query found a hit, else kStepContinue. Descending first and
+// testing leaves after keeps the lanes of a wave in the same code (inner nodes, then leaves).
+template <int BLOCK, int LDS_STACK, typename Fetch>
+SOL_DEV int traverse_step(const Fetch& f, const Stack& st, const RayPre& r, float tmin, bool any, TraversalState& ts) {
+    while (ts.node >= 0) {
+        float4 a, b, c, d;
+        f.node(ts.node, a, b, c, d);
+        float t0, t1;
+        bool h0 = intersect_box(r, a.x, a.y, a.z, a.w, c.x, c.y, tmin, ts.tmax, t0);
+        bool h1 = intersect_box(r, b.x, b.y, b.z, b.w, c.z, c.w, tmin, ts.tmax, t1);
+        int c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
+        if (h0 && h1) {
+            int near = c0, far = c1;
+            if (t1 < t0) { near = c1; far = c0; }
+            if (ts.sp < LDS_STACK) {
+                st.lds[ts.sp * BLOCK] = far;
+            } else if (ts.sp < st.capacity) {
+                st.spill[(ts.sp - LDS_STACK) * st.spill_stride] = far;
+            } else {
+                *st.overflow = 1;
+                --ts.sp;
+            }
+            ++ts.sp;
+            ts.node = near;
+        } else if (h0) {
+            ts.node = c0;
+        } else if (h1) {
+            ts.node = c1;
+        } else if (!stack_pop<BLOCK, LDS_STACK>(st, ts)) {
+            return kStepDone;
+        }
+    }
+    uint32_t first, count;
+    decode_leaf(ts.node, first, count);
+    for (uint32_t i = 0; i < count; ++i) {
+        float4 p0, p1, p2;
+        f.tri(first + i, p0, p1, p2);
+        float t, u, v;
+        if (intersect_triangle(r, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, t, u, v)) {
+            int prim = __float_as_int(p0.w);
+            if (any) {
+                if (t > tmin && t < ts.tmax) return kStepOccluded;
+            } else if (t > tmin && (t < ts.tmax || (t == ts.tmax && ts.hit.prim >= 0 && prim < ts.hit.prim))) {
+                ts.tmax = t;
+                ts.hit.slot = static_cast<int>(first + i);
+                ts.hit.prim = prim;
+                ts.hit.t = t;
+                ts.hit.u = u;
+                ts.hit.v = v;
+            }
+        }
+    }
+    return stack_pop<BLOCK, LDS_STACK>(st, ts) ? kStepContinue : kStepDone;
+}
+
+// A whole traversal in one loop (one node or one leaf per iteration). The kernels that run whole
+// paths per lane use this form: it compiles to fewer registers there than looping over
+// traverse_step.
+template <bool ANY, int BLOCK, int LDS_STACK, typename Fetch>
+SOL_DEV bool traverse_with(const Fetch& f, const Stack& st, const RayPre& r, float tmin, float tmax, HitRecord& hit) {
     int sp = 0;
     int cur = 0;
     hit.slot = -1;
@@ -52,18 +228,7 @@ SOL_DEV bool traverse(const SceneView& s, const float4* lds_nodes, const Stack& 
     while (true) {
         if (cur >= 0) {
             float4 a, b, c, d;
-            if (cur < top) {
-                a = lds_nodes[cur * 4 + 0];
-                b = lds_nodes[cur * 4 + 1];
-                c = lds_nodes[cur * 4 + 2];
-                d = lds_nodes[cur * 4 + 3];
-            } else {
-                const float4* p = s.nodes + cur * 4;
-                a = p[0];
-                b = p[1];
-                c = p[2];
-                d = p[3];
-            }
+            f.node(cur, a, b, c, d);
             float t0, t1;
             bool h0 = intersect_box(r, a.x, a.y, a.z, a.w, c.x, c.y, tmin, tmax, t0);
             bool h1 = intersect_box(r, b.x, b.y, b.z, b.w, c.z, c.w, tmin, tmax, t1);
@@ -89,8 +254,8 @@ SOL_DEV bool traverse(const SceneView& s, const float4* lds_nodes, const Stack& 
             uint32_t first, count;
             decode_leaf(cur, first, count);
             for (uint32_t i = 0; i < count; ++i) {
-                const float4* tp = s.tris + (first + i) * 3;
-                float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                float4 p0, p1, p2;
+                f.tri(first + i, p0, p1, p2);
                 float t, u, v;
                 if (intersect_triangle(r, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, t, u, v)) {
                     int prim = __float_as_int(p0.w);
@@ -109,9 +274,24 @@ SOL_DEV bool traverse(const SceneView& s, const float4* lds_nodes, const Stack& 
         }
         if (sp == 0) break;
         --sp;
-        cur = sp < LDS_STACK ? st.lds[sp * BLOCK] : st.spill[(sp - LDS_STACK) * st.spill_stride];
+        if (sp < LDS_STACK) cur = st.lds[sp * BLOCK];
+        else cur = st.spill[(sp - LDS_STACK) * st.spill_stride];
     }
     return ANY ? false : hit.prim >= 0;
+}
+
+// Traversal of a wave-uniform scene whose top levels are staged in lds_nodes.
+template <bool ANY, int BLOCK, int LDS_STACK>
+SOL_DEV bool traverse(const SceneView& s, const float4* lds_nodes, const Stack& st, const RayPre& r, float tmin,
+                      float tmax, HitRecord& hit) {
+    return traverse_with<ANY, BLOCK, LDS_STACK>(FetchUniform(s, lds_nodes), st, r, tmin, tmax, hit);
+}
+
+// Traversal of a per-lane scene (no LDS staging).
+template <bool ANY, int BLOCK, int LDS_STACK>
+SOL_DEV bool traverse_lane(const SceneView& s, const Stack& st, const RayPre& r, float tmin, float tmax,
+                           HitRecord& hit) {
+    return traverse_with<ANY, BLOCK, LDS_STACK>(FetchGlobal(s), st, r, tmin, tmax, hit);
 }
 
 } // namespace solhip

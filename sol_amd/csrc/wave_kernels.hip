@@ -14,6 +14,8 @@
 // The per-bounce arithmetic is the shared code of path.h, in the order of trace_path
 // (src/algorithms/path_tracer.cpp:31-120 of sol): NEE of vertex k is added by trace(k+1), before
 // the emission of hit k+1 is added by sort(k+1).
+#include <algorithm>
+
 #include "launch.h"
 #include "path.h"
 #include "traverse.h"
@@ -208,13 +210,8 @@ SOL_DEV void retire(const WaveBuf& b, float4 p1, float4 p2) {
 template <int BLOCK>
 SOL_DEV Stack make_stack_w(int* lds_stack, const TraversalScratch& sc, int lds_entries) {
     int gtid = blockIdx.x * BLOCK + threadIdx.x;
-    Stack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.spill = sc.spill + gtid;
-    st.spill_stride = sc.max_threads;
-    st.capacity = lds_entries + sc.spill_entries;
-    st.overflow = sc.overflow;
-    return st;
+    return Stack::make(lds_stack, threadIdx.x, sc.spill, gtid, sc.max_threads, lds_entries + sc.spill_entries,
+                       sc.overflow);
 }
 
 // ---- trace -----------------------------------------------------------------------------------------
@@ -232,6 +229,15 @@ SOL_DEV unsigned wave_fetch(unsigned* counter, unsigned total, unsigned waves, u
     return base;
 }
 
+// Lanes whose ray has ended take new entries as soon as this many lanes of the wave are idle, so
+// that a wave does not wait for its slowest ray before fetching more work.
+constexpr int kRefillLanes = 16;
+
+// Persistent traversal with per-lane refill: every lane runs its own ray one traversal step at a
+// time; when kRefillLanes lanes are idle the wave hands them the next entries of its chunk
+// (fetching a new chunk when the current one runs out). Entries [0, n_ext) are the closest-hit
+// queries of extension queue `cur` (hit written in place), [n_ext, n_ext + n_shadow) the any-hit
+// queries of the shadow queue (unoccluded contribution added to the path or its sample).
 template <int BLOCK, int LDS, bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                  int cur, TraversalScratch sc) {
@@ -242,52 +248,99 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
     const unsigned total = n_ext + n_shadow;
     if (total == 0) return;
     if (!MULTI) load_top_nodes<BLOCK>(single, lds_nodes);
-    Stack st = make_stack_w<BLOCK>(lds_stack, sc, LDS);
+    const Stack st = make_stack_w<BLOCK>(lds_stack, sc, LDS);
     const unsigned waves = gridDim.x * (BLOCK / 64);
     const float4* q = b.eq[cur];
-    unsigned n_closest = 0, n_any = 0, last = 0;
+    const FetchUniform fu(single, lds_nodes);
+    FetchGlobal fg(single);
+    int idx = -1;        // entry this lane traces, -1 = idle
+    bool any = false;
+    float tmin = 0.0f;
+    RayPre r;
+    TraversalState ts;
+    unsigned c_next = 0, c_end = 0, last = 0;   // the wave's current chunk [c_next, c_end)
+    bool exhausted = false;
+    unsigned n_closest = 0, n_any = 0;
     while (true) {
-        unsigned len;
-        unsigned base = wave_fetch(&b.counts[kCntTraceWork], total, waves, last, len);
-        if (base >= total) break;
-        last = base;
-        unsigned end = min(base + len, total);
-        for (unsigned c = base; c < end; c += 64) {
-            unsigned i = c + __lane_id();
-            if (i >= end) continue;
-            if (i < n_ext) {
-                float4 e0 = q[2 * i], e1 = q[2 * i + 1];
-                if (as_u(e1.w) == kHole) continue;
-                ++n_closest;
-                SceneView tmp;
-                const SceneView& s = scene_of<MULTI>(single, tabs, b, as_u(e1.w), tmp);
-                RayPre r = make_ray_pre(xyz(e0), xyz(e1));
-                HitRecord hit;
-                bool found = MULTI ? traverse<false, BLOCK, LDS>(s, s.nodes, st, r, e0.w, FLT_MAX, hit)
-                                   : traverse<false, BLOCK, LDS>(s, lds_nodes, st, r, e0.w, FLT_MAX, hit);
-                b.hits[i] = found ? make_float4(as_f(static_cast<uint32_t>(hit.slot)), hit.t, hit.u, hit.v)
-                                  : make_float4(as_f(0xFFFFFFFFu), 0.0f, 0.0f, 0.0f);
-            } else {
-                ++n_any;
-                const float4* e = b.sq + 3 * (i - n_ext);
-                float4 e0 = e[0], e1 = e[1];
-                uint32_t target = as_u(e0.w);
-                SceneView sv;
-                if (MULTI) {
-                    if (target == kSampleTarget) {
-                        unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) | as_u(e[2].w);
-                        sv = tabs.scenes[tabs.job_scene[find_job(tabs, item)]];
-                    } else {
-                        sv = tabs.scenes[as_u(b.path[target * 4 + 3].x)];
-                    }
+        const unsigned long long idle = __ballot(idx < 0);
+        const unsigned n_idle = static_cast<unsigned>(__popcll(idle));
+        if (!exhausted && n_idle >= static_cast<unsigned>(kRefillLanes)) {
+            const unsigned rank = static_cast<unsigned>(__popcll(idle & lanes_below()));
+            const unsigned avail = c_end - c_next;
+            unsigned base2 = 0, end2 = 0;
+            if (avail < n_idle) {
+                unsigned len;
+                base2 = wave_fetch(&b.counts[kCntTraceWork], total, waves, last, len);
+                if (base2 >= total) {
+                    exhausted = true;
+                } else {
+                    last = base2;
+                    end2 = min(base2 + len, total);
                 }
-                const SceneView& s = MULTI ? sv : single;
-                RayPre r = make_ray_pre(xyz(e0), xyz(e1));
-                HitRecord unused;
-                bool occluded = MULTI ? traverse<true, BLOCK, LDS>(s, s.nodes, st, r, cfg.ray_offset, 1.0f - cfg.ray_offset, unused)
-                                      : traverse<true, BLOCK, LDS>(s, lds_nodes, st, r, cfg.ray_offset, 1.0f - cfg.ray_offset, unused);
-                if (!occluded) {
-                    float4 ct = e[2];
+            }
+            if (idx < 0) {
+                unsigned e = 0xFFFFFFFFu;
+                if (rank < avail) e = c_next + rank;
+                else if (base2 + (rank - avail) < end2) e = base2 + (rank - avail);
+                if (e < n_ext) {
+                    float4 e0 = q[2 * e], e1 = q[2 * e + 1];
+                    uint32_t slot = as_u(e1.w);
+                    if (slot != kHole) {
+                        idx = static_cast<int>(e);
+                        any = false;
+                        tmin = e0.w;
+                        r = make_ray_pre(xyz(e0), xyz(e1));
+                        ts.start(FLT_MAX);
+                        if (MULTI) fg = FetchGlobal(tabs.scenes[as_u(b.path[slot * 4 + 3].x)]);
+                        ++n_closest;
+                    }
+                } else if (e != 0xFFFFFFFFu) {
+                    const float4* se = b.sq + 3 * (e - n_ext);
+                    float4 e0 = se[0], e1 = se[1];
+                    idx = static_cast<int>(e);
+                    any = true;
+                    tmin = cfg.ray_offset;
+                    r = make_ray_pre(xyz(e0), xyz(e1));
+                    ts.start(1.0f - cfg.ray_offset);
+                    if (MULTI) {
+                        uint32_t target = as_u(e0.w);
+                        if (target == kSampleTarget) {
+                            unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) | as_u(se[2].w);
+                            fg = FetchGlobal(tabs.scenes[tabs.job_scene[find_job(tabs, item)]]);
+                        } else {
+                            fg = FetchGlobal(tabs.scenes[as_u(b.path[target * 4 + 3].x)]);
+                        }
+                    }
+                    ++n_any;
+                }
+            }
+            if (avail >= n_idle) {
+                c_next += n_idle;
+            } else if (end2 > base2) {
+                c_next = min(base2 + (n_idle - avail), end2);
+                c_end = end2;
+            } else {
+                c_next = c_end;
+            }
+        }
+        if (__ballot(idx >= 0) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (idx >= 0) {
+            int status = MULTI ? traverse_step<BLOCK, LDS>(fg, st, r, tmin, any, ts)
+                               : traverse_step<BLOCK, LDS>(fu, st, r, tmin, any, ts);
+            if (status != kStepContinue) {
+                const unsigned e = static_cast<unsigned>(idx);
+                if (!any) {
+                    b.hits[e] = ts.hit.prim >= 0
+                                    ? make_float4(as_f(static_cast<uint32_t>(ts.hit.slot)), ts.hit.t, ts.hit.u, ts.hit.v)
+                                    : make_float4(as_f(0xFFFFFFFFu), 0.0f, 0.0f, 0.0f);
+                } else if (status == kStepDone) {
+                    // unoccluded: add the light sample's contribution
+                    const float4* se = b.sq + 3 * (e - n_ext);
+                    float4 e0 = se[0], e1 = se[1], ct = se[2];
+                    uint32_t target = as_u(e0.w);
                     if (target == kSampleTarget) {
                         unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) | as_u(ct.w);
                         float4* d = b.samples + (item - b.items[2]);
@@ -298,6 +351,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
                         b.path[target * 4 + 1] = make_float4(p1.x + ct.x, p1.y + ct.y, p1.z + ct.z, p1.w);
                     }
                 }
+                idx = -1;
             }
         }
     }
@@ -569,6 +623,35 @@ void launch_trace_t(const WaveLaunch& L, int cur) {
         default: return hipErrorInvalidValue;                                                        \
     }
 
+namespace {
+template <typename K>
+hipError_t resident_grid(K kernel, int compute_units, int cap, int& grid) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel),
+                                                               kBlock, 0);
+    if (e != hipSuccess) return e;
+    grid = std::min(cap, compute_units * std::max(1, per_cu));
+    return hipSuccess;
+}
+} // namespace
+
+hipError_t wave_set_grids(WaveLaunch& L, int compute_units) {
+    hipError_t e;
+    const int cap = L.grid_shade;
+    if (L.multi) {
+        if ((e = resident_grid(k_sort<true>, compute_units, cap, L.grid_sort)) != hipSuccess) return e;
+        if ((e = resident_grid(k_shade<0, true>, compute_units, cap, L.grid_kshade[0])) != hipSuccess) return e;
+        if ((e = resident_grid(k_shade<1, true>, compute_units, cap, L.grid_kshade[1])) != hipSuccess) return e;
+        if ((e = resident_grid(k_shade<2, true>, compute_units, cap, L.grid_kshade[2])) != hipSuccess) return e;
+    } else {
+        if ((e = resident_grid(k_sort<false>, compute_units, cap, L.grid_sort)) != hipSuccess) return e;
+        if ((e = resident_grid(k_shade<0, false>, compute_units, cap, L.grid_kshade[0])) != hipSuccess) return e;
+        if ((e = resident_grid(k_shade<1, false>, compute_units, cap, L.grid_kshade[1])) != hipSuccess) return e;
+        if ((e = resident_grid(k_shade<2, false>, compute_units, cap, L.grid_kshade[2])) != hipSuccess) return e;
+    }
+    return resident_grid(k_start, compute_units, cap, L.grid_start);
+}
+
 hipError_t wave_launch_trace(const WaveLaunch& L, int cur) {
     (void)hipGetLastError();
     if (L.multi) { SOLHIP_WAVE_DISPATCH(launch_trace_t, true, L, cur) }
@@ -579,14 +662,14 @@ hipError_t wave_launch_trace(const WaveLaunch& L, int cur) {
 hipError_t wave_launch_sort(const WaveLaunch& L, int cur) {
     (void)hipGetLastError();
     if (L.multi)
-        hipLaunchKernelGGL((k_sort<true>), dim3(L.grid_shade), dim3(kBlock), 0, L.stream, L.single, L.tabs, L.buf, L.cfg, cur);
+        hipLaunchKernelGGL((k_sort<true>), dim3(L.grid_sort), dim3(kBlock), 0, L.stream, L.single, L.tabs, L.buf, L.cfg, cur);
     else
-        hipLaunchKernelGGL((k_sort<false>), dim3(L.grid_shade), dim3(kBlock), 0, L.stream, L.single, L.tabs, L.buf, L.cfg, cur);
+        hipLaunchKernelGGL((k_sort<false>), dim3(L.grid_sort), dim3(kBlock), 0, L.stream, L.single, L.tabs, L.buf, L.cfg, cur);
     return hipGetLastError();
 }
 
 hipError_t wave_launch_shade(const WaveLaunch& L, int queue, int nxt) {
-    dim3 g(L.grid_shade), bs(kBlock);
+    dim3 g(L.grid_kshade[queue]), bs(kBlock);
     (void)hipGetLastError();
     if (L.multi) {
         if (queue == 0) hipLaunchKernelGGL((k_shade<0, true>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
@@ -602,7 +685,7 @@ hipError_t wave_launch_shade(const WaveLaunch& L, int queue, int nxt) {
 
 hipError_t wave_launch_start(const WaveLaunch& L, int nxt) {
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_start, dim3(L.grid_shade), dim3(kBlock), 0, L.stream, L.tabs, L.buf, nxt);
+    hipLaunchKernelGGL(k_start, dim3(L.grid_start), dim3(kBlock), 0, L.stream, L.tabs, L.buf, nxt);
     return hipGetLastError();
 }
 

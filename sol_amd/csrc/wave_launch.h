@@ -15,10 +15,17 @@ struct WaveLaunch {
     int block;                // traversal kernels
     int lds_stack;
     int grid_traverse;
-    int grid_shade;
+    int grid_shade;           // workgroups of k_sort, k_shade, k_start before wave_set_grids
+    int grid_sort;            // resident workgroups of each kernel (set by wave_set_grids): these
+    int grid_kshade[3];       // kernels hand out rounds by workgroup index, so every workgroup
+    int grid_start;           // of the grid must be resident at once
     bool multi;
     hipStream_t stream;
 };
+
+// Sizes the grids of the sort, shade and start kernels to the workgroups the device keeps
+// resident (occupancy x compute units), capped at grid_shade.
+hipError_t wave_set_grids(WaveLaunch& L, int compute_units);
 
 hipError_t wave_launch_trace(const WaveLaunch& L, int cur);
 hipError_t wave_launch_sort(const WaveLaunch& L, int cur);

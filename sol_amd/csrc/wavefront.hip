@@ -143,7 +143,9 @@ std::string graph_key(const WaveLaunch& L, const WavefrontState& st, int iters) 
     k.append(reinterpret_cast<const char*>(&L.single), sizeof(L.single));
     k.append(reinterpret_cast<const char*>(&L.tabs), sizeof(L.tabs));
     k.append(reinterpret_cast<const char*>(&L.buf), sizeof(L.buf));
-    int meta[6] = {L.block, L.lds_stack, L.grid_traverse, L.grid_shade, L.multi ? 1 : 0, iters};
+    int meta[11] = {L.block,          L.lds_stack,      L.grid_traverse,  L.grid_shade,
+                    L.grid_sort,      L.grid_kshade[0], L.grid_kshade[1], L.grid_kshade[2],
+                    L.grid_start,     L.multi ? 1 : 0,  iters};
     k.append(reinterpret_cast<const char*>(meta), sizeof(meta));
     k.append(reinterpret_cast<const char*>(&st.generation), sizeof(st.generation));
     k.append(reinterpret_cast<const char*>(&L.scratch), sizeof(L.scratch));
@@ -268,6 +270,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     L.grid_traverse = persistent_grid(ctx.info.compute_units, L.block);
     L.grid_shade = persistent_grid(ctx.info.compute_units, kWaveBlock);
     L.multi = multi;
+    HIP_OK(wave_set_grids(L, ctx.info.compute_units));
     L.stream = s;
     if ((rc = ctx.ensure_scratch(L.grid_traverse * L.block))) return rc;
     L.scratch = ctx.scratch;

@@ -189,3 +189,22 @@ def test_batch_of_scenes_equals_single_renders(ctx):
     ctx.render_batch(jobs, cfg)
     for b, wimg in zip(bufs, want):
         np.testing.assert_array_equal(b.cpu().numpy(), wimg)
+
+
+@pytest.mark.parametrize("kind", ["megakernel", "wavefront"])
+def test_equal_rmse_against_converged_image(ctx, cornell, kind):
+    # Acceptance criterion of the renderer: at equal spp, the GPU image's RMSE against the
+    # converged (4096-spp oracle) image is within 2 % of the oracle's own, and its mean radiance
+    # within 0.1 % of the oracle's.
+    import os
+    ref = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cornell_64_4096spp.npy"))
+    scene, oracle = cornell
+    cfg = _cfg(max_path_len=8)
+    gpu, cpu = sol_amd.new_image(64, 64), sol_amd.new_image(64, 64)
+    sol_amd.GpuPathTracer(scene, cfg, kind=kind, ctx=ctx).render(gpu, 100_000, 16)
+    oracle.render(cpu, 100_000, 16, cfg)
+    rmse_gpu = float(np.sqrt(((gpu / 16 - ref) ** 2).mean()))
+    rmse_cpu = float(np.sqrt(((cpu / 16 - ref) ** 2).mean()))
+    print(f"RMSE vs converged: gpu {rmse_gpu:.6f}, oracle {rmse_cpu:.6f}")
+    assert abs(rmse_gpu - rmse_cpu) <= 0.02 * rmse_cpu
+    assert abs(gpu.mean() - cpu.mean()) <= 1e-3 * cpu.mean()

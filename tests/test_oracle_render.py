@@ -2,6 +2,7 @@
 closed-form direct lighting from a point light, the furnace identity of a closed emitting box,
 and the agreement of the three estimators sol can switch between (MIS, NEE only, BSDF only)."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -140,3 +141,28 @@ def test_oracle_product_bvh_and_own_bvh_render_identically():
     O.OracleScene(scene, use_product_bvh=True).render(a, 0, 4, cfg)
     O.OracleScene(scene, use_product_bvh=False).render(b, 0, 4, cfg)
     np.testing.assert_array_equal(a, b)
+
+
+CONVERGED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cornell_64_4096spp.npy")
+
+
+def test_converged_cornell_fixture_reproduces_and_converges():
+    # The committed 4096-spp image is the oracle's own (rows 0..3 re-rendered bit for bit), and a
+    # 256-spp render from other sample indices agrees with it to Monte Carlo accuracy.
+    ref = np.load(CONVERGED)
+    scene = sol_amd.Scene.cornell_box()
+    osc = O.OracleScene(scene)
+    cfg = sol_amd.PathTracerConfig(max_path_len=8)
+    part = sol_amd.new_image(64, 64)
+    osc.render(part, 0, 4096, cfg, rows=(0, 4))
+    np.testing.assert_array_equal((part / np.float32(4096)).astype(np.float32)[:, :4], ref[:, :4])
+    img = sol_amd.new_image(64, 64)
+    osc.render(img, 100_000, 256, cfg)
+    img /= 256
+    assert abs(img.mean() - ref.mean()) <= 0.01 * ref.mean()
+    rmse_256 = np.sqrt(((img - ref) ** 2).mean())
+    img16 = sol_amd.new_image(64, 64)
+    osc.render(img16, 100_000, 16, cfg)
+    rmse_16 = np.sqrt(((img16 / 16 - ref) ** 2).mean())
+    # Monte Carlo error falls as 1 / sqrt(spp): 16x the samples, about a quarter of the error
+    assert 0.15 < rmse_256 / rmse_16 < 0.4
