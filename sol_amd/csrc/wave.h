@@ -14,7 +14,7 @@
 namespace solhip {
 
 // Workgroup size of the sort, shade and start kernels.
-constexpr int kWaveBlock = 512;
+constexpr int kWaveBlock = 1024;
 
 enum : int {
     kQueueDiffuse = 0,
@@ -23,14 +23,39 @@ enum : int {
     kNumShadeQueues = 3
 };
 
-// Device counters, one unsigned int each.
+// Queues are split into kNumSegments segments, each with its own length counter: a workgroup
+// appends to segment blockIdx % kNumSegments only, so the device atomics of the appends spread
+// over kNumSegments addresses. Readers see a queue as the concatenation of its segments.
+constexpr int kNumSegments = 8;
+
 enum : int {
-    kCntExtend0 = 0,    // extension queue 0 length
-    kCntExtend1 = 1,    // extension queue 1 length
-    kCntShade0 = 2,     // shade queue lengths (3)
-    kCntShadow = 5,     // shadow queue length
-    kCntTraceWork = 8,  // work-fetch counter of the traversal kernel
-    kNumWaveCounters = 16
+    kQExtend0 = 0,      // extension queues (two: one read, one written per bounce)
+    kQExtend1 = 1,
+    kQShade0 = 2,       // shade queues (kNumShadeQueues)
+    kQShadow = 5,       // shadow queue
+    kNumQueues = 6
+};
+
+// Device counters: the length of segment s of queue q is counter q * kNumSegments + s, then the
+// work-fetch counter of the traversal kernel. Counters sit kCounterStride words (256 bytes) apart:
+// device atomics on one cache line serialise, so each counter gets a line of its own.
+constexpr int kCounterStride = 64;
+SOL_HD constexpr int qcount(int queue, int seg) { return (queue * kNumSegments + seg) * kCounterStride; }
+enum : int {
+    kNumWaveCounters = kNumQueues * kNumSegments + 1,
+    kCntTraceWork = kNumQueues * kNumSegments * kCounterStride,
+    kCounterWords = kNumWaveCounters * kCounterStride
+};
+
+// Work items of the current chunk (64-bit words, 256 bytes apart): segment s hands out
+// items[item_next(s)] .. items[item_end(s)] - 1; items[kItemChunkBegin] is the first item of the
+// chunk (index 0 of the sample buffer).
+constexpr int kItemStride = 32;
+SOL_HD constexpr int item_next(int seg) { return seg * kItemStride; }
+SOL_HD constexpr int item_end(int seg) { return (kNumSegments + seg) * kItemStride; }
+enum : int {
+    kItemChunkBegin = 2 * kNumSegments * kItemStride,
+    kItemWords = kItemChunkBegin + 1
 };
 
 // One bounce of all paths in flight is four launches: trace (closest hit of the extension rays
@@ -39,7 +64,8 @@ enum : int {
 // Per-path state is one 64-byte record per slot (random access costs one cache line); the data
 // that flows from one kernel to the next travels in queue entries written at compacted positions
 // (coalesced): extension rays (32 B), hits (16 B), shade entries (hit record, 64 B), shadow rays
-// (48 B). Appends are compacted per workgroup (one atomic per 256 entries and queue).
+// (48 B). Appends are compacted per workgroup (one atomic per round of kWaveBlock entries, queue
+// and segment).
 // A path that ends writes its radiance to its sample's entry of the sample buffer and the slot
 // starts the next work item at once (in the kernel that ended it); a shadow ray still in flight
 // for the ended path adds its contribution to that sample entry instead of the path record.
@@ -54,16 +80,24 @@ enum : int {
 // shadow entry (3 x float4):  [0] org xyz, target  [1] dir xyz, item hi  [2] contribution xyz, item lo
 //                             target = slot, or kSampleTarget | 0 when the path has ended (the
 //                             contribution then goes to the sample entry of the item)
+//
+// Segment capacities (entries), for a launch of n input entries handed out in rounds of R =
+// kWaveBlock by static round robin over a grid that is a multiple of kNumSegments: segment s gets
+// the rounds r with r % kNumSegments == s, at most n / kNumSegments + 2 R entries per launch and
+// output queue. Shade queues and the shadow queue receive from launches whose inputs sum to at
+// most `slots`; an extension queue receives from sort and the three shade launches, whose inputs
+// sum to at most 2 slots.
 struct WaveBuf {
     uint32_t slots;
+    uint32_t cap_e, cap_m, cap_s;     // segment capacities: extension, shade, shadow queues
     float4* path;                     // [slots * 4]
-    float4* hits;                     // [slots]: closest hit of extension entry i: leaf-order
-                                      // triangle (int bits, -1 = miss), t, u, v
-    float4* eq[2];                    // [slots * 2] each
-    float4* mq[kNumShadeQueues];      // [slots * 4] each
-    float4* sq;                       // [slots * 3]
-    unsigned* counts;                 // kNumWaveCounters
-    unsigned long long* items;        // [0] next global item, [1] chunk end, [2] chunk begin
+    float4* hits;                     // [slots]: closest hit of extension entry i (in queue order):
+                                      // leaf-order triangle (int bits, -1 = miss), t, u, v
+    float4* eq[2];                    // [kNumSegments * cap_e * 2] each
+    float4* mq[kNumShadeQueues];      // [kNumSegments * cap_m * 4] each
+    float4* sq;                       // [kNumSegments * cap_s * 3]
+    unsigned* counts;                 // kCounterWords
+    unsigned long long* items;        // kItemWords
     unsigned long long* stats;        // [0] extension rays [1] shadow rays [2] samples
     float4* samples;                  // [sample_capacity]: radiance of each item of the chunk
     unsigned long long sample_capacity;

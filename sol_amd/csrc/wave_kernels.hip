@@ -54,9 +54,11 @@ SOL_DEV void wave_add_stat(unsigned long long* counter, unsigned long long v) {
 // retire it (`retire`). Retired paths take the next work items in thread order; every retired
 // path also gets an extension entry after the round's `cont` entries, used for the new path if
 // an item was left and marked as a hole (slot kHole) otherwise - holes only appear when the
-// chunk runs out of items. All reservations are issued at once by different lanes: two barriers
-// and one device atomic per non-empty queue per round. The LDS block is double-buffered by round
-// parity, so no third barrier is needed before the next round overwrites it.
+// chunk runs out of items. Work items come from the workgroup's segment first; when it runs dry
+// the other segments are tried in turn, so no item is stranded while any path retires. All
+// reservations are issued at once by different lanes: two barriers and one device atomic per
+// non-empty queue per round. The LDS block is double-buffered by round parity, so no third
+// barrier is needed before the next round overwrites it.
 template <int NQ>
 struct RoundLds {
     unsigned wc[NQ + 2][kWaves];    // per-wave counts: plain queues, cont, retire
@@ -64,21 +66,44 @@ struct RoundLds {
     unsigned base[NQ];
     unsigned ext_base;
     unsigned n_cont;
-    unsigned n_ret;
-    unsigned long long item_base;
+    unsigned n_pieces;              // work items granted: n_pieces ranges [piece_base, + piece_count)
+    unsigned piece_count[kNumSegments];
+    unsigned long long piece_base[kNumSegments];
 };
 
 struct RoundPlan {
-    int pos;                     // position in the plain queue, -1 if none
-    int ext_pos;                 // position in the extension queue (cont or retired), -1 if none
+    int pos;                     // position in the plain queue (in the segment), -1 if none
+    int ext_pos;                 // position in the extension queue (in the segment), -1 if none
     bool start;                  // the retired path got a new work item
     unsigned long long item;     // that item
 };
 
+// Reserves up to `need` work items, own segment first (items[item_next(s)] counts up to
+// items[item_end(s)]). Counters only grow, so a relaxed read at or past the end means the
+// segment is exhausted and it is skipped without an atomic.
+template <int NQ>
+SOL_DEV void grant_items(RoundLds<NQ>& lds, unsigned long long* items, int seg, unsigned need) {
+    unsigned n = 0;
+    for (int k = 0; k < kNumSegments && need > 0; ++k) {
+        int sk = (seg + k) % kNumSegments;
+        const unsigned long long end = items[item_end(sk)];
+        if (k > 0 && __hip_atomic_load(&items[item_next(sk)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= end)
+            continue;
+        unsigned long long base = atomicAdd(&items[item_next(sk)], static_cast<unsigned long long>(need));
+        if (base < end) {
+            unsigned got = static_cast<unsigned>(min(static_cast<unsigned long long>(need), end - base));
+            lds.piece_base[n] = base;
+            lds.piece_count[n] = got;
+            ++n;
+            need -= got;
+        }
+    }
+    lds.n_pieces = n;
+}
+
 template <int NQ>
 SOL_DEV RoundPlan plan_round(RoundLds<NQ>& lds, unsigned* const* qcounters, unsigned* ext_counter,
-                             unsigned long long* items, unsigned long long items_end, int dest, bool cont,
-                             bool retire) {
+                             unsigned long long* items, int seg, int dest, bool cont, bool retire) {
     const int wave = threadIdx.x >> 6;
     const bool lane0 = __lane_id() == 0;
     const unsigned long long below = lanes_below();
@@ -114,27 +139,64 @@ SOL_DEV RoundPlan plan_round(RoundLds<NQ>& lds, unsigned* const* qcounters, unsi
             lds.ext_base = n ? atomicAdd(ext_counter, n) : 0u;
             lds.n_cont = run;
         } else {
-            lds.item_base = run ? atomicAdd(items, static_cast<unsigned long long>(run)) : 0ull;
-            lds.n_ret = run;
+            grant_items(lds, items, seg, run);
         }
     }
     __syncthreads();
     RoundPlan p;
     p.pos = dest >= 0 ? static_cast<int>(lds.base[dest] + lds.pre[dest][wave] + rank) : -1;
     const unsigned ridx = lds.pre[NQ + 1][wave] + rank_r;
-    const unsigned long long ib = lds.item_base;
-    const unsigned long long avail = ib < items_end ? items_end - ib : 0ull;
-    p.start = retire && ridx < avail;
-    p.item = ib + ridx;
+    p.start = false;
+    p.item = 0;
+    if (retire) {
+        unsigned off = ridx;
+        const unsigned np = lds.n_pieces;
+        for (unsigned k = 0; k < np; ++k) {
+            unsigned c = lds.piece_count[k];
+            if (off < c) {
+                p.start = true;
+                p.item = lds.piece_base[k] + off;
+                break;
+            }
+            off -= c;
+        }
+    }
     p.ext_pos = cont ? static_cast<int>(lds.ext_base + lds.pre[NQ][wave] + rank_c)
                      : (retire ? static_cast<int>(lds.ext_base + lds.n_cont + ridx) : -1);
     return p;
 }
 
 // A retired lane whose extension entry stays unused marks it as a hole (trace and sort skip it).
-SOL_DEV void write_hole(const WaveBuf& b, int nxt, int ext_pos) {
-    b.eq[nxt][2 * ext_pos + 1] = make_float4(0.0f, 0.0f, 0.0f, as_f(kHole));
+SOL_DEV void write_hole(const WaveBuf& b, int nxt, unsigned entry) {
+    b.eq[nxt][2 * entry + 1] = make_float4(0.0f, 0.0f, 0.0f, as_f(kHole));
 }
+
+// The segment this workgroup appends to (grids are multiples of kNumSegments).
+SOL_DEV int my_segment() { return static_cast<int>(blockIdx.x % kNumSegments); }
+
+// A queue read as the concatenation of its segments: entry(i) is the storage index of the i-th
+// entry.
+struct SegQueue {
+    unsigned pre[kNumSegments + 1];
+    unsigned cap;
+
+    SOL_DEV SegQueue(const unsigned* counts, int queue, unsigned capacity) : cap(capacity) {
+        pre[0] = 0;
+#pragma unroll
+        for (int k = 0; k < kNumSegments; ++k) pre[k + 1] = pre[k] + counts[qcount(queue, k)];
+    }
+    SOL_DEV unsigned size() const { return pre[kNumSegments]; }
+    SOL_DEV unsigned entry(unsigned i) const {
+        unsigned seg = 0, start = 0;
+#pragma unroll
+        for (int k = 1; k < kNumSegments; ++k) {
+            bool ge = i >= pre[k];
+            seg = ge ? static_cast<unsigned>(k) : seg;
+            start = ge ? pre[k] : start;
+        }
+        return seg * cap + (i - start);
+    }
+};
 
 template <bool MULTI>
 SOL_DEV const SceneView& scene_of(const SceneView& single, const WaveTables& t, const WaveBuf& b, uint32_t slot,
@@ -204,7 +266,7 @@ SOL_DEV void init_path(const WaveTables& t, const WaveBuf& b, unsigned long long
 
 // The radiance of an ended path goes to its sample entry.
 SOL_DEV void retire(const WaveBuf& b, float4 p1, float4 p2) {
-    b.samples[item_of(p2) - b.items[2]] = make_float4(p1.x, p1.y, p1.z, 0.0f);
+    b.samples[item_of(p2) - b.items[kItemChunkBegin]] = make_float4(p1.x, p1.y, p1.z, 0.0f);
 }
 
 template <int BLOCK>
@@ -243,8 +305,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
                                                  int cur, TraversalScratch sc) {
     __shared__ float4 lds_nodes[kTopNodesMax * 4];
     __shared__ int lds_stack[LDS * BLOCK];
-    const unsigned n_ext = b.counts[kCntExtend0 + cur];
-    const unsigned n_shadow = b.counts[kCntShadow];
+    const SegQueue eqv(b.counts, kQExtend0 + cur, b.cap_e), sqv(b.counts, kQShadow, b.cap_s);
+    const unsigned n_ext = eqv.size();
+    const unsigned n_shadow = sqv.size();
     const unsigned total = n_ext + n_shadow;
     if (total == 0) return;
     if (!MULTI) load_top_nodes<BLOCK>(single, lds_nodes);
@@ -283,7 +346,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
                 if (rank < avail) e = c_next + rank;
                 else if (base2 + (rank - avail) < end2) e = base2 + (rank - avail);
                 if (e < n_ext) {
-                    float4 e0 = q[2 * e], e1 = q[2 * e + 1];
+                    const unsigned a = eqv.entry(e);
+                    float4 e0 = q[2 * a], e1 = q[2 * a + 1];
                     uint32_t slot = as_u(e1.w);
                     if (slot != kHole) {
                         idx = static_cast<int>(e);
@@ -295,7 +359,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
                         ++n_closest;
                     }
                 } else if (e != 0xFFFFFFFFu) {
-                    const float4* se = b.sq + 3 * (e - n_ext);
+                    const float4* se = b.sq + 3 * sqv.entry(e - n_ext);
                     float4 e0 = se[0], e1 = se[1];
                     idx = static_cast<int>(e);
                     any = true;
@@ -338,12 +402,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
                                     : make_float4(as_f(0xFFFFFFFFu), 0.0f, 0.0f, 0.0f);
                 } else if (status == kStepDone) {
                     // unoccluded: add the light sample's contribution
-                    const float4* se = b.sq + 3 * (e - n_ext);
+                    const float4* se = b.sq + 3 * sqv.entry(e - n_ext);
                     float4 e0 = se[0], e1 = se[1], ct = se[2];
                     uint32_t target = as_u(e0.w);
                     if (target == kSampleTarget) {
                         unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) | as_u(ct.w);
-                        float4* d = b.samples + (item - b.items[2]);
+                        float4* d = b.samples + (item - b.items[kItemChunkBegin]);
                         float4 v = *d;
                         *d = make_float4(v.x + ct.x, v.y + ct.y, v.z + ct.z, 0.0f);
                     } else {
@@ -365,11 +429,14 @@ template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_sort(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                  int cur) {
     __shared__ RoundLds<kNumShadeQueues> lds[2];
-    const unsigned count = b.counts[kCntExtend0 + cur];
+    const SegQueue eqv(b.counts, kQExtend0 + cur, b.cap_e);
+    const unsigned count = eqv.size();
     const int nxt = 1 - cur;
-    unsigned* qcnt[kNumShadeQueues] = {&b.counts[kCntShade0], &b.counts[kCntShade0 + 1], &b.counts[kCntShade0 + 2]};
+    const int seg = my_segment();
+    unsigned* qcnt[kNumShadeQueues] = {&b.counts[qcount(kQShade0, seg)], &b.counts[qcount(kQShade0 + 1, seg)],
+                                       &b.counts[qcount(kQShade0 + 2, seg)]};
+    unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
     const float4* q = b.eq[cur];
-    const unsigned long long items_end = b.items[1];
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
         unsigned i = round * kBlock + threadIdx.x;
@@ -378,13 +445,15 @@ __global__ __launch_bounds__(kBlock) void k_sort(SceneView single, WaveTables ta
         bool dead = false;
         uint32_t slot = 0;
         float4 rec0, rec1, rec2, rec3, e1;
+        unsigned a = 0;
         if (valid) {
-            e1 = q[2 * i + 1];
+            a = eqv.entry(i);
+            e1 = q[2 * a + 1];
             slot = as_u(e1.w);
             valid = slot != kHole;
         }
         if (valid) {
-            float4 e0 = q[2 * i], h = b.hits[i];
+            float4 e0 = q[2 * a], h = b.hits[i];
             int tri = static_cast<int>(as_u(h.x));
             if (tri < 0) {
                 dead = true;
@@ -413,10 +482,9 @@ __global__ __launch_bounds__(kBlock) void k_sort(SceneView single, WaveTables ta
             }
             if (dead) retire(b, b.path[slot * 4 + 1], b.path[slot * 4 + 2]);
         }
-        RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, &b.counts[kCntExtend0 + nxt], &b.items[0],
-                                                     items_end, dest, false, dead);
+        RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, dest, false, dead);
         if (plan.pos >= 0) {
-            float4* e = b.mq[dest] + 4 * plan.pos;
+            float4* e = b.mq[dest] + 4 * (seg * b.cap_m + static_cast<unsigned>(plan.pos));
             e[0] = rec0;
             e[1] = rec1;
             e[2] = rec2;
@@ -426,11 +494,11 @@ __global__ __launch_bounds__(kBlock) void k_sort(SceneView single, WaveTables ta
             ++started;
             V3 org, dir;
             init_path(tabs, b, plan.item, slot, org, dir);
-            float4* e = b.eq[nxt] + 2 * plan.ext_pos;
+            float4* e = b.eq[nxt] + 2 * (seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
             e[0] = f4(org, 0.0f);
             e[1] = f4(dir, as_f(slot));
         } else if (dead) {
-            write_hole(b, nxt, plan.ext_pos);
+            write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
         }
     }
     wave_add_stat(&b.stats[2], started);
@@ -444,10 +512,12 @@ template <int QUEUE, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                   int nxt) {
     __shared__ RoundLds<1> lds[2];
-    const unsigned count = b.counts[kCntShade0 + QUEUE];
-    unsigned* scnt[1] = {&b.counts[kCntShadow]};
+    const SegQueue mqv(b.counts, kQShade0 + QUEUE, b.cap_m);
+    const unsigned count = mqv.size();
+    const int seg = my_segment();
+    unsigned* scnt[1] = {&b.counts[qcount(kQShadow, seg)]};
+    unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
     const float4* mq = b.mq[QUEUE];
-    const unsigned long long items_end = b.items[1];
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
         unsigned i = round * kBlock + threadIdx.x;
@@ -457,7 +527,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
         V3 point{0, 0, 0}, sdir{0, 0, 0}, contrib{0, 0, 0}, ndir{0, 0, 0};
         unsigned long long item = 0;
         if (valid) {
-            float4 r0 = mq[4 * i], r1 = mq[4 * i + 1], r2 = mq[4 * i + 2], r3 = mq[4 * i + 3];
+            const unsigned a = mqv.entry(i);
+            float4 r0 = mq[4 * a], r1 = mq[4 * a + 1], r2 = mq[4 * a + 2], r3 = mq[4 * a + 3];
             slot = as_u(r1.w);
             SceneView tmp;
             const SceneView& s = scene_of<MULTI>(single, tabs, b, slot, tmp);
@@ -504,10 +575,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
             if (!cont) retire(b, p1, p2);
         }
         bool retired = valid && !cont;
-        RoundPlan plan = plan_round<1>(lds[parity], scnt, &b.counts[kCntExtend0 + nxt], &b.items[0], items_end,
-                                       shoot ? 0 : -1, cont, retired);
+        RoundPlan plan = plan_round<1>(lds[parity], scnt, ecnt, b.items, seg, shoot ? 0 : -1, cont, retired);
         if (plan.pos >= 0) {
-            float4* e = b.sq + 3 * plan.pos;
+            float4* e = b.sq + 3 * (seg * b.cap_s + static_cast<unsigned>(plan.pos));
             e[0] = f4(point, as_f(cont ? slot : kSampleTarget));
             e[1] = f4(sdir, as_f(static_cast<uint32_t>(item >> 32)));
             e[2] = f4(contrib, as_f(static_cast<uint32_t>(item)));
@@ -521,11 +591,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
                     init_path(tabs, b, plan.item, slot, org, dir);
                     tmin = 0.0f;
                 }
-                float4* e = b.eq[nxt] + 2 * plan.ext_pos;
+                float4* e = b.eq[nxt] + 2 * (seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
                 e[0] = f4(org, tmin);
                 e[1] = f4(dir, as_f(slot));
             } else {
-                write_hole(b, nxt, plan.ext_pos);
+                write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
             }
         }
     }
@@ -537,22 +607,22 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
 // Starts a path in every slot (beginning of a chunk of work).
 __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, int nxt) {
     __shared__ RoundLds<1> lds[2];
-    const unsigned long long items_end = b.items[1];
+    const int seg = my_segment();
+    unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < b.slots; round += gridDim.x, parity ^= 1) {
         unsigned slot = round * kBlock + threadIdx.x;
         bool want = slot < b.slots;
-        RoundPlan plan = plan_round<1>(lds[parity], nullptr, &b.counts[kCntExtend0 + nxt], &b.items[0], items_end,
-                                       -1, false, want);
+        RoundPlan plan = plan_round<1>(lds[parity], nullptr, ecnt, b.items, seg, -1, false, want);
         if (plan.start) {
             ++started;
             V3 org, dir;
             init_path(tabs, b, plan.item, slot, org, dir);
-            float4* e = b.eq[nxt] + 2 * plan.ext_pos;
+            float4* e = b.eq[nxt] + 2 * (seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
             e[0] = f4(org, 0.0f);
             e[1] = f4(dir, as_f(slot));
         } else if (want) {
-            write_hole(b, nxt, plan.ext_pos);
+            write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
         }
     }
     wave_add_stat(&b.stats[2], started);
@@ -562,16 +632,19 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
 // shade(k) append to (kernel, not a memset: it is captured in graphs).
 __global__ void k_reset(WaveBuf b, int cur) {
     int i = threadIdx.x;
-    if (i < kNumWaveCounters && i != kCntExtend0 + cur) b.counts[i] = 0;
+    if (i < kNumWaveCounters && i / kNumSegments != kQExtend0 + cur) b.counts[i * kCounterStride] = 0;
 }
 
 __global__ void k_set_chunk(WaveBuf b, unsigned long long begin, unsigned long long end) {
-    if (threadIdx.x == 0) {
-        b.items[0] = begin;
-        b.items[1] = end;
-        b.items[2] = begin;
+    // segment s hands out the items [begin + n s / S, begin + n (s + 1) / S)
+    const unsigned long long n = end - begin;
+    if (threadIdx.x < kNumSegments) {
+        unsigned long long s = threadIdx.x;
+        b.items[item_next(static_cast<int>(s))] = begin + n * s / kNumSegments;
+        b.items[item_end(static_cast<int>(s))] = begin + n * (s + 1) / kNumSegments;
     }
-    if (threadIdx.x < kNumWaveCounters) b.counts[threadIdx.x] = 0;
+    if (threadIdx.x == 0) b.items[kItemChunkBegin] = begin;
+    if (threadIdx.x < kNumWaveCounters) b.counts[threadIdx.x * kCounterStride] = 0;
 }
 
 // Per pixel of the chunk's tiles: sum the samples in sample order, add to the image once.
@@ -630,7 +703,9 @@ hipError_t resident_grid(K kernel, int compute_units, int cap, int& grid) {
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel),
                                                                kBlock, 0);
     if (e != hipSuccess) return e;
-    grid = std::min(cap, compute_units * std::max(1, per_cu));
+    // a multiple of kNumSegments: round r goes to workgroup r % grid, whose segment is then r % kNumSegments
+    grid = std::min(cap, compute_units * std::max(1, per_cu)) / kNumSegments * kNumSegments;
+    grid = std::max(grid, kNumSegments);
     return hipSuccess;
 }
 } // namespace

@@ -49,6 +49,7 @@ struct WavefrontState {
     hipGraphExec_t graph_exec = nullptr;
     std::string graph_key;
     std::vector<hipEvent_t> events;
+    std::vector<unsigned> host_counts = std::vector<unsigned>(kCounterWords);
 
     ~WavefrontState() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -58,25 +59,34 @@ struct WavefrontState {
 
     int ensure_slots(uint32_t n) {
         if (n == slots && arena.ptr) return SOLHIP_OK;
-        // per slot: path record 64 B, hit 16 B, two extension queues 2 x 32 B, three shade queues
-        // 3 x 64 B, shadow queue 48 B
-        const size_t per_slot_f4 = 4 + 1 + 2 * 2 + 3 * 4 + 3;
-        size_t bytes = align_up(sizeof(float4) * n) * per_slot_f4 + 4096;
+        // Segment capacities (see WaveBuf): extension queues slots / 4 + 8 R, shade queues
+        // slots / 8 + 2 R, shadow queue slots / 8 + 6 R, R = kWaveBlock, rounded up to 64 entries.
+        const size_t R = kWaveBlock;
+        auto round64 = [](size_t v) { return (v + 63) & ~size_t(63); };
+        const size_t cap_e = round64(n / 4 + 8 * R), cap_m = round64(n / 8 + 2 * R), cap_s = round64(n / 8 + 6 * R);
+        // path record 64 B and hit 16 B per slot; queue entries 32 B (extension), 64 B (shade),
+        // 48 B (shadow) per segment entry
+        size_t f4_total = size_t(n) * (4 + 1) + kNumSegments * (2 * cap_e * 2 + kNumShadeQueues * cap_m * 4 + cap_s * 3);
+        size_t bytes = sizeof(float4) * f4_total + 16 * 4096;
         HIP_OK(arena.alloc(bytes));
-        HIP_OK(counters.alloc(256 * 3));
+        HIP_OK(counters.alloc(sizeof(unsigned) * kCounterWords + align_up(sizeof(unsigned long long) * kItemWords) + 256));
         char* p = arena.as<char>();
-        auto f4s = [&](size_t k) { float4* r = reinterpret_cast<float4*>(p); p += align_up(sizeof(float4) * n * k); return r; };
+        auto f4s = [&](size_t count) { float4* r = reinterpret_cast<float4*>(p); p += align_up(sizeof(float4) * count); return r; };
         WaveBuf& b = buf;
         b.slots = n;
-        b.path = f4s(4);
-        b.hits = f4s(1);
-        for (int q = 0; q < 2; ++q) b.eq[q] = f4s(2);
-        for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = f4s(4);
-        b.sq = f4s(3);
+        b.cap_e = static_cast<uint32_t>(cap_e);
+        b.cap_m = static_cast<uint32_t>(cap_m);
+        b.cap_s = static_cast<uint32_t>(cap_s);
+        b.path = f4s(size_t(n) * 4);
+        b.hits = f4s(n);
+        for (int q = 0; q < 2; ++q) b.eq[q] = f4s(kNumSegments * cap_e * 2);
+        for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = f4s(kNumSegments * cap_m * 4);
+        b.sq = f4s(kNumSegments * cap_s * 3);
         char* c = counters.as<char>();
         b.counts = reinterpret_cast<unsigned*>(c);
-        b.items = reinterpret_cast<unsigned long long*>(c + 256);
-        b.stats = reinterpret_cast<unsigned long long*>(c + 512);
+        b.items = reinterpret_cast<unsigned long long*>(c + sizeof(unsigned) * kCounterWords);
+        b.stats = reinterpret_cast<unsigned long long*>(c + sizeof(unsigned) * kCounterWords +
+                                                        align_up(sizeof(unsigned long long) * kItemWords));
         slots = n;
         ++generation;
         return SOLHIP_OK;
@@ -323,8 +333,8 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
             launches += static_cast<uint64_t>(iters) * 6;
             // After an even number of iterations the next extension queue is queue 0; shadow rays
             // of the last shade still need a trace.
-            unsigned counts[kNumWaveCounters];
-            HIP_OK(hipMemcpyAsync(counts, st.buf.counts, sizeof(counts), hipMemcpyDeviceToHost, s));
+            std::vector<unsigned>& counts = st.host_counts;
+            HIP_OK(hipMemcpyAsync(counts.data(), st.buf.counts, sizeof(unsigned) * kCounterWords, hipMemcpyDeviceToHost, s));
             HIP_OK(hipStreamSynchronize(s));
             if (profile) {
                 for (int it = 0; it < iters; ++it) {
@@ -337,7 +347,10 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
                     ctx.stats.trace_launches += 1;
                 }
             }
-            if (counts[kCntExtend0] == 0 && counts[kCntShadow] == 0) break;
+            unsigned pending = 0;
+            for (int k = 0; k < kNumSegments; ++k)
+                pending += counts[qcount(kQExtend0, k)] + counts[qcount(kQShadow, k)];
+            if (pending == 0) break;
         }
         HIP_OK(wave_launch_reduce(L, c.first, c.count, max_tile_pixels, c.begin));
         launches += 1;

@@ -208,3 +208,16 @@ def test_equal_rmse_against_converged_image(ctx, cornell, kind):
     print(f"RMSE vs converged: gpu {rmse_gpu:.6f}, oracle {rmse_cpu:.6f}")
     assert abs(rmse_gpu - rmse_cpu) <= 0.02 * rmse_cpu
     assert abs(gpu.mean() - cpu.mean()) <= 1e-3 * cpu.mean()
+
+
+@pytest.mark.parametrize("w,h,spp", [(5, 3, 1), (5, 3, 7), (17, 9, 3), (32, 32, 3), (70, 40, 2)])
+def test_wavefront_small_renders_take_every_work_item(ctx, cornell, w, h, spp):
+    # Renders with fewer work items than a round per queue segment: every item must still be
+    # handed out (segments whose own range is never reached are drained by the others).
+    scene, _ = cornell
+    cfg = _cfg(max_path_len=8)
+    a, b = sol_amd.new_image(w, h), sol_amd.new_image(w, h)
+    g = ctx.upload(scene)
+    g.render(a, 11, spp, cfg, renderer=F.RENDERER_WAVEFRONT)
+    g.render(b, 11, spp, cfg, renderer=F.RENDERER_MEGAKERNEL)
+    np.testing.assert_array_equal(a, b)
