@@ -170,6 +170,7 @@ struct SceneView {
 struct RayPre {
     V3 org, dir;
     V3 inv;          // 1 / dir, with |dir| < 1e-20 replaced by +-1e-20 (no 0 * inf)
+    V3 off_lo, off_hi;  // slab distance of plane p on axis a: fma(p, inv.a, off_lo.a or off_hi.a)
     int kx, ky, kz;
     float sx, sy, sz;
 };
@@ -181,6 +182,19 @@ SOL_HD RayPre make_ray_pre(V3 org, V3 dir) {
     r.org = org;
     r.dir = dir;
     r.inv = V3{1.0f / safe_dir(dir.x), 1.0f / safe_dir(dir.y), 1.0f / safe_dir(dir.z)};
+    // fma(p, inv, -org * inv) = (p - org) * inv up to one rounding and the rounding of org * inv,
+    // an absolute error of at most 2^-24 |org * inv|: the near plane's distance is lowered and the
+    // far plane's raised by twice that (per axis; which plane is near depends on the sign of inv).
+    auto offsets = [](float o, float inv, float& lo, float& hi) {
+        float oi = o * inv;
+        float e = 0x1p-23f * fabsf(oi);
+        float near_off = -oi - e, far_off = -oi + e;
+        lo = inv >= 0.0f ? near_off : far_off;
+        hi = inv >= 0.0f ? far_off : near_off;
+    };
+    offsets(org.x, r.inv.x, r.off_lo.x, r.off_hi.x);
+    offsets(org.y, r.inv.y, r.off_lo.y, r.off_hi.y);
+    offsets(org.z, r.inv.z, r.off_lo.z, r.off_hi.z);
     float ax = fabsf(dir.x), ay = fabsf(dir.y), az = fabsf(dir.z);
     int kz = 0;
     float m = ax;
@@ -228,11 +242,16 @@ SOL_HD bool intersect_triangle(const RayPre& r, V3 v0, V3 v1, V3 v2, float& t, f
 
 // Slab test of one box against [0, tmax] (see kIntervalHigh); returns the entry distance through
 // `tnear`. tmin is accepted for symmetry with the triangle test and not used.
+// Each slab distance is one fused multiply-add, plane * inv + offset: the offsets widen every
+// slab by the absolute error of the rounded org * inv (see make_ray_pre) and kRobustFar covers
+// the relative rounding (Ize 2013). The test only decides which nodes are visited (the closest
+// hit is the minimum over every triangle the exact test accepts), so it needs to be
+// conservative, not identical to any other box test.
 SOL_HD bool intersect_box(const RayPre& r, float lox, float hix, float loy, float hiy, float loz, float hiz,
                           float tmin, float tmax, float& tnear) {
-    float tx0 = (lox - r.org.x) * r.inv.x, tx1 = (hix - r.org.x) * r.inv.x;
-    float ty0 = (loy - r.org.y) * r.inv.y, ty1 = (hiy - r.org.y) * r.inv.y;
-    float tz0 = (loz - r.org.z) * r.inv.z, tz1 = (hiz - r.org.z) * r.inv.z;
+    float tx0 = fmaf(lox, r.inv.x, r.off_lo.x), tx1 = fmaf(hix, r.inv.x, r.off_hi.x);
+    float ty0 = fmaf(loy, r.inv.y, r.off_lo.y), ty1 = fmaf(hiy, r.inv.y, r.off_hi.y);
+    float tz0 = fmaf(loz, r.inv.z, r.off_lo.z), tz1 = fmaf(hiz, r.inv.z, r.off_hi.z);
     float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
     float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * kRobustFar;
     f = fminf(f, tmax * kIntervalHigh);

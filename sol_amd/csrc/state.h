@@ -41,7 +41,7 @@ inline solhip_tuning default_tuning() {
     solhip_tuning t{};
     t.block_size = 256;
     t.lds_stack = 16;
-    t.path_slots = 1u << 23;
+    t.path_slots = 1u << 24;
     t.tile_w = 32;
     t.tile_h = 32;
     t.graph_iters = 16;

@@ -57,8 +57,9 @@ struct WavefrontState {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
     }
 
+    // Grows the per-slot arena to at least n slots (never shrinks it).
     int ensure_slots(uint32_t n) {
-        if (n == slots && arena.ptr) return SOLHIP_OK;
+        if (arena.ptr && n <= slots) return SOLHIP_OK;
         // Segment capacities (see WaveBuf): extension queues slots / 4 + 8 R, shade queues
         // slots / 8 + 2 R, shadow queue slots / 8 + 6 R, R = kWaveBlock, rounded up to 64 entries.
         const size_t R = kWaveBlock;
@@ -168,8 +169,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     if (!ctx.wave) ctx.wave = new WavefrontState();
     WavefrontState& st = *ctx.wave;
     const solhip_tuning& tu = ctx.tuning;
-    int rc = st.ensure_slots(tu.path_slots);
-    if (rc) return rc;
+    int rc;
 
     // Jobs in passes of samples: a pass keeps a job under 2^32 items and one tile's samples
     // under the sample-buffer capacity (far above any ordinary sample count); each pass is
@@ -249,6 +249,10 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
         chunks.push_back(c);
     }
     if ((rc = st.ensure_samples(std::max<unsigned long long>(biggest, 1)))) return rc;
+    // no more slots than the largest chunk has items: a small render does not walk idle slots
+    const uint32_t want_slots =
+        static_cast<uint32_t>(std::min<unsigned long long>(tu.path_slots, std::max<unsigned long long>(256, (biggest + 255) & ~255ull)));
+    if ((rc = st.ensure_slots(want_slots))) return rc;
 
     const bool multi = views.size() > 1;
     WaveLaunch L{};
@@ -272,8 +276,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     L.tabs.tile_item0 = st.t_tile_item0.as<unsigned long long>();
     L.tabs.num_tiles = num_tiles;
     L.buf = st.buf;
-    // no more slots than the largest chunk has items: a small render does not walk idle slots
-    L.buf.slots = static_cast<uint32_t>(std::min<unsigned long long>(st.slots, (biggest + 255) & ~255ull));
+    L.buf.slots = want_slots;
     L.cfg = cfg;
     L.block = static_cast<int>(tu.block_size);
     L.lds_stack = static_cast<int>(tu.lds_stack);
