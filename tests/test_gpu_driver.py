@@ -1,0 +1,49 @@
+"""The C++ path end to end: sol_render (sol::RenderJob driving sol::GpuPathTracer over the C++
+scene model, the flow of sol's driver) against the same samples rendered through the C ABI from
+Python, and the PFM it writes."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import sol_amd
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "sol_amd", "sol_render")
+
+
+def read_pfm(path):
+    with open(path, "rb") as f:
+        assert f.readline().strip() == b"PF"
+        w, h = (int(v) for v in f.readline().split())
+        assert float(f.readline()) < 0          # little endian
+        data = np.frombuffer(f.read(), dtype="<f4")
+    rows = data.reshape(h, w, 3)[::-1]          # stored bottom to top
+    return np.ascontiguousarray(rows.transpose(2, 0, 1))
+
+
+@pytest.mark.parametrize("renderer", ["wavefront", "megakernel"])
+def test_render_job_driver_matches_c_abi_render(tmp_path, renderer):
+    if not os.path.exists(DRIVER):
+        pytest.fail(f"{DRIVER} is not built (make -C sol_amd)")
+    w, h, spp, spf = 40, 30, 7, 3
+    out = tmp_path / "out.pfm"
+    r = subprocess.run([DRIVER, "--scene", "cornell", "--width", str(w), "--height", str(h), "--spp", str(spp),
+                        "--spf", str(spf), "--max-path-len", "8", "--renderer", renderer, "-o", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "3 frames, 7 samples per pixel" in r.stdout
+    got = read_pfm(out)
+    # RenderJob renders frames of samples [0, 3), [3, 6), [6, 7) into one image, then scales it
+    scene = sol_amd.Scene.cornell_box(aspect=w / h)
+    pt = sol_amd.GpuPathTracer(scene, sol_amd.PathTracerConfig(max_path_len=8), kind=renderer)
+    want = sol_amd.new_image(w, h)
+    for first in range(0, spp, spf):
+        pt.render(want, first, min(spf, spp - first))
+    want *= np.float32(1.0) / np.float32(spp)   # Image::scale(1.0f / spp)
+    np.testing.assert_array_equal(got, want)
+    assert got.mean() > 0.01
