@@ -14,7 +14,7 @@
 namespace solhip {
 
 // Workgroup size of the sort, shade and start kernels.
-constexpr int kWaveBlock = 1024;
+constexpr int kWaveBlock = 512;
 
 enum : int {
     kQueueDiffuse = 0,
