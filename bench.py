@@ -178,7 +178,7 @@ def bench_batched(sol_amd, F, torch, steps):
            "msamples_s": round(samples / batch_s / 1e6, 2), "ms": round(batch_s * 1e3, 2),
            "megakernel_one_by_one_msamples_s": round(samples / mega_s / 1e6, 2),
            "tuned": {k: getattr(tuned, k) for k in ("block_size", "lds_stack", "path_slots", "tile_w", "tile_h",
-                                                   "graph_iters")},
+                                                   "graph_iters", "bvh_width")},
            "build_s": round(build_s, 1), "autotune_s": round(tune_s, 1)}
     log(f"batched: {out}")
     for g in gs:

@@ -133,6 +133,21 @@ typedef struct sol_gpu_node {
     int32_t pad[2];
 } sol_gpu_node;
 
+/* 4-wide GPU node (128 bytes): the boxes of up to four children, each coordinate as an array over
+   the children, and the child references (child[i] >= 0: inner node index, < 0: leaf encoded as in
+   sol_gpu_node, SOL_GPU_EMPTY: unused slot, whose box is never tested). */
+#define SOL_GPU_EMPTY ((int32_t)0x7FFFFFFF)
+typedef struct sol_gpu_node4 {
+    float lo_x[4];
+    float hi_x[4];
+    float lo_y[4];
+    float hi_y[4];
+    float lo_z[4];
+    float hi_z[4];
+    int32_t child[4];
+    int32_t pad[4];
+} sol_gpu_node4;
+
 /* GPU triangle in leaf order: the three vertices (watertight test input) and the original
    triangle index. 48 bytes. */
 typedef struct sol_gpu_tri {

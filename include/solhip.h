@@ -71,6 +71,11 @@ typedef struct solhip_scene_desc {
     sol_camera camera;
     uint32_t bvh_depth;             /* longest root-to-leaf path (nodes)          */
     uint32_t top_nodes;             /* nodes [0, top_nodes) are the top levels    */
+    /* The same tree collapsed to 4-wide nodes (the wavefront traversal kernel's layout). */
+    uint32_t num_nodes4;
+    const sol_gpu_node4* nodes4;    /* node 0 is the root                         */
+    uint32_t bvh4_depth;            /* longest root-to-leaf path (4-wide nodes)   */
+    uint32_t top_nodes4;            /* nodes4 [0, top_nodes4) are the top levels  */
 } solhip_scene_desc;
 
 /* Launch parameters the autotuner chooses. Zero fields mean "library default". */
@@ -81,6 +86,10 @@ typedef struct solhip_tuning {
     uint32_t tile_w;        /* batch: tile shape of the global work queue                    */
     uint32_t tile_h;
     uint32_t graph_iters;   /* bounce-loop iterations per captured graph                     */
+    uint32_t bvh_width;     /* 2 or 4: node layout of the wavefront and intersection traversal
+                               (4 needs the scene's 4-wide layout; scenes without one use the
+                               binary). The megakernel and the adjoint always use the binary
+                               layout, their faster one.                                    */
 } solhip_tuning;
 
 /* Counters accumulated by the context since the last solhip_reset_stats. */

@@ -57,6 +57,14 @@ class GpuNode(C.Structure):
                 ("pad", C.c_int32 * 2)]
 
 
+class GpuNode4(C.Structure):
+    _fields_ = [("lo_x", C.c_float * 4), ("hi_x", C.c_float * 4), ("lo_y", C.c_float * 4), ("hi_y", C.c_float * 4),
+                ("lo_z", C.c_float * 4), ("hi_z", C.c_float * 4), ("child", C.c_int32 * 4), ("pad", C.c_int32 * 4)]
+
+
+GPU_EMPTY = 0x7FFFFFFF
+
+
 class GpuTri(C.Structure):
     _fields_ = [("v0", C.c_float * 3), ("prim", C.c_int32), ("v1", C.c_float * 3), ("pad0", C.c_int32),
                 ("v2", C.c_float * 3), ("pad1", C.c_int32)]
@@ -102,12 +110,14 @@ class SceneDesc(C.Structure):
     _fields_ = [("num_nodes", C.c_uint32), ("nodes", C.POINTER(GpuNode)), ("num_tris", C.c_uint32),
                 ("tris", C.POINTER(GpuTri)), ("shade", C.POINTER(GpuShade)), ("num_materials", C.c_uint32),
                 ("materials", C.POINTER(Material)), ("num_lights", C.c_uint32), ("lights", C.POINTER(Light)),
-                ("camera", Camera), ("bvh_depth", C.c_uint32), ("top_nodes", C.c_uint32)]
+                ("camera", Camera), ("bvh_depth", C.c_uint32), ("top_nodes", C.c_uint32),
+                ("num_nodes4", C.c_uint32), ("nodes4", C.POINTER(GpuNode4)), ("bvh4_depth", C.c_uint32),
+                ("top_nodes4", C.c_uint32)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("lds_stack", C.c_uint32), ("path_slots", C.c_uint32),
-                ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("graph_iters", C.c_uint32)]
+                ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("graph_iters", C.c_uint32), ("bvh_width", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -129,6 +139,7 @@ class BatchJob(C.Structure):
 
 assert C.sizeof(Material) == 64 and C.sizeof(Light) == 80 and C.sizeof(GpuNode) == 64
 assert C.sizeof(GpuTri) == 48 and C.sizeof(GpuShade) == 64 and C.sizeof(Ray) == 32 and C.sizeof(Hit) == 16
+assert C.sizeof(GpuNode4) == 128
 
 
 def _load(name):

@@ -58,6 +58,7 @@ LaunchShape Context::shape(int block, int lds) const {
     s.block = block;
     s.lds_stack = lds;
     s.grid = persistent_grid(info.compute_units, block);
+    s.wide = tuning.bvh_width == 4;
     return s;
 }
 
@@ -158,6 +159,8 @@ int solhip_set_tuning(solhip_ctx* ctx, const solhip_tuning* t) {
     if (!n.tile_w) n.tile_w = d.tile_w;
     if (!n.tile_h) n.tile_h = d.tile_h;
     if (!n.graph_iters) n.graph_iters = d.graph_iters;
+    if (!n.bvh_width) n.bvh_width = d.bvh_width;
+    if (n.bvh_width != 2 && n.bvh_width != 4) return fail(SOLHIP_E_INVALID, "bvh_width must be 2 or 4");
     if (!valid_shape_values(n.block_size, n.lds_stack)) return fail(SOLHIP_E_INVALID, "unsupported block size or LDS stack");
     if (n.path_slots < 4096 || n.path_slots > (1u << 26)) return fail(SOLHIP_E_INVALID, "path_slots out of range");
     if (n.graph_iters > 256) return fail(SOLHIP_E_INVALID, "graph_iters out of range");
@@ -208,6 +211,27 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
         if (m >= static_cast<int>(d->num_materials) || m < -1 || l >= static_cast<int>(d->num_lights) || l < -1)
             return fail(SOLHIP_E_INVALID, "material or light index out of range");
     }
+    if (d->num_nodes4 && !d->nodes4) return fail(SOLHIP_E_INVALID, "null 4-wide node array");
+    if (static_cast<uint64_t>(d->num_nodes4) * sizeof(sol_gpu_node4) > 0xFFFFFFFFull)
+        return fail(SOLHIP_E_INVALID, "scene larger than 4 GiB of nodes");
+    for (uint32_t i = 0; i < d->num_nodes4; ++i) {
+        int used = 0;
+        for (int c = 0; c < 4; ++c) {
+            int32_t ref = d->nodes4[i].child[c];
+            if (ref == SOL_GPU_EMPTY) continue;
+            ++used;
+            if (ref >= 0) {
+                if (static_cast<uint32_t>(ref) >= d->num_nodes4) return fail(SOLHIP_E_INVALID, "4-wide node index out of range");
+            } else {
+                uint32_t code = static_cast<uint32_t>(~ref);
+                if ((code >> 3) + (code & 7u) + 1u > d->num_tris) return fail(SOLHIP_E_INVALID, "leaf range out of range");
+            }
+        }
+        if (used == 0) return fail(SOLHIP_E_INVALID, "4-wide node without children");
+    }
+    // a 4-wide level pushes at most three nodes on the traversal stack
+    if (d->num_nodes4 && 3 * (d->bvh4_depth > 0 ? d->bvh4_depth - 1 : 0) > static_cast<uint32_t>(8 + kSpillEntries))
+        return fail(SOLHIP_E_TOO_DEEP, "4-wide BVH depth " + std::to_string(d->bvh4_depth) + " exceeds the traversal stack");
     for (uint32_t i = 0; i < d->num_nodes; ++i) {
         for (int c = 0; c < 2; ++c) {
             int32_t ref = d->nodes[i].child[c];
@@ -227,9 +251,14 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     s->num_materials = d->num_materials;
     s->num_lights = d->num_lights;
     s->bvh_depth = d->bvh_depth;
+    s->bvh4_depth = d->num_nodes4 ? d->bvh4_depth : 0;
     s->materials_host.assign(d->materials, d->materials + d->num_materials);
     s->lights_host.assign(d->lights, d->lights + d->num_lights);
     HIP_TRY(hipMalloc(&s->nodes, sizeof(sol_gpu_node) * d->num_nodes));
+    if (d->num_nodes4) {
+        HIP_TRY(hipMalloc(&s->nodes4, sizeof(sol_gpu_node4) * d->num_nodes4));
+        HIP_TRY(hipMemcpy(s->nodes4, d->nodes4, sizeof(sol_gpu_node4) * d->num_nodes4, hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMalloc(&s->tris, sizeof(sol_gpu_tri) * d->num_tris));
     HIP_TRY(hipMalloc(&s->shade, sizeof(sol_gpu_shade) * d->num_tris));
     HIP_TRY(hipMalloc(&s->materials, sizeof(sol_material) * std::max<uint32_t>(1, d->num_materials)));
@@ -242,6 +271,9 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     HIP_TRY(hipMemcpy(s->lights, d->lights, sizeof(sol_light) * d->num_lights, hipMemcpyHostToDevice));
     SceneView& v = s->view;
     v.nodes = reinterpret_cast<const float4*>(s->nodes);
+    v.nodes4 = reinterpret_cast<const float4*>(s->nodes4);
+    v.num_nodes4 = d->num_nodes4;
+    v.top_nodes4 = std::min<uint32_t>(d->top_nodes4, d->num_nodes4);
     v.tris = reinterpret_cast<const float4*>(s->tris);
     v.shade = reinterpret_cast<const float4*>(s->shade);
     v.materials = s->materials;
@@ -261,6 +293,7 @@ void solhip_scene_destroy(solhip_scene* s) {
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     (void)hipFree(s->nodes);
+    if (s->nodes4) (void)hipFree(s->nodes4);
     (void)hipFree(s->tris);
     (void)hipFree(s->shade);
     (void)hipFree(s->materials);

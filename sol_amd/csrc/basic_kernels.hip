@@ -9,13 +9,13 @@ namespace solhip {
 
 namespace {
 
-template <bool ANY, int BLOCK, int LDS>
+template <bool ANY, int BLOCK, int LDS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol_ray* __restrict__ rays, uint64_t n,
                                                           sol_hit* __restrict__ hits, int32_t* __restrict__ occluded,
                                                           unsigned long long* counter, TraversalScratch sc) {
     __shared__ float4 lds_nodes[kTopNodesMax * 4];
     __shared__ int lds_stack[LDS * BLOCK];
-    load_top_nodes<BLOCK>(s, lds_nodes);
+    stage_top_nodes<BLOCK, WIDE>(s, lds_nodes);
     Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
     while (true) {
         unsigned long long base = wave_grab(counter, 64);
@@ -27,7 +27,7 @@ __global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol
         V3 org{a.x, a.y, a.z}, dir{b.x, b.y, b.z};
         RayPre r = make_ray_pre(org, dir);
         HitRecord hit;
-        bool found = traverse<ANY, BLOCK, LDS>(s, lds_nodes, st, r, a.w, b.w, hit);
+        bool found = trace_ray<ANY, BLOCK, LDS, WIDE>(s, lds_nodes, st, r, a.w, b.w, hit);
         if (ANY) {
             occluded[i] = found ? 1 : 0;
         } else {
@@ -43,6 +43,9 @@ __global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol
 
 // Pixels are handed out in 8x8 tiles (one tile per wave) so that primary rays of a wave are
 // coherent.
+// The megakernel traverses the binary layout: with whole paths in one kernel its registers bound
+// its occupancy, and the 4-wide traversal's extra registers made it slower (49.7 vs 58.6 ms per
+// 1080p 16-spp frame of the sphere field on MI355X).
 template <int BLOCK, int LDS>
 __global__ __launch_bounds__(BLOCK) void megakernel(SceneView s, sol_pt_config cfg, uint32_t w, uint32_t h,
                                                     uint64_t sample_index, uint32_t sample_count,
@@ -91,14 +94,16 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 template <int BLOCK, int LDS>
 hipError_t intersect_impl(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
                           int32_t* occluded, unsigned long long* counter, const TraversalScratch& sc, int grid,
-                          hipStream_t stream) {
+                          bool wide, hipStream_t stream) {
     (void)hipGetLastError();
-    if (any_hit)
-        hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, rays, n, hits,
-                           occluded, counter, sc);
-    else
-        hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, rays, n,
-                           hits, occluded, counter, sc);
+    dim3 g(grid), b(BLOCK);
+    if (any_hit) {
+        if (wide) hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, true>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
+        else hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, false>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
+    } else {
+        if (wide) hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, true>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
+        else hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, false>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
+    }
     return hipGetLastError();
 }
 
@@ -131,7 +136,8 @@ hipError_t megakernel_impl(const SceneView& s, const sol_pt_config& cfg, uint32_
 hipError_t launch_intersect(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
                             int32_t* occluded, unsigned long long* work_counter, const TraversalScratch& scratch,
                             const LaunchShape& shape, hipStream_t stream) {
-    SOLHIP_DISPATCH(intersect_impl, s, rays, n, any_hit, hits, occluded, work_counter, scratch, shape.grid, stream)
+    SOLHIP_DISPATCH(intersect_impl, s, rays, n, any_hit, hits, occluded, work_counter, scratch, shape.grid,
+                    shape.wide && s.nodes4 != nullptr, stream)
 }
 
 hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint32_t width, uint32_t height,

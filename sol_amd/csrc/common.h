@@ -152,6 +152,9 @@ SOL_DEV Basis ortho_basis(V3 n) {
 
 struct SceneView {
     const float4* nodes;   // 4 x float4 per node (sol_gpu_node)
+    const float4* nodes4;  // 8 x float4 per node (sol_gpu_node4), null if not uploaded
+    uint32_t num_nodes4;
+    uint32_t top_nodes4;
     const float4* tris;    // 3 x float4 per triangle (sol_gpu_tri), leaf order
     const float4* shade;   // 4 x float4 per triangle (sol_gpu_shade), leaf order
     const sol_material* materials;

@@ -20,6 +20,7 @@ struct LaunchShape {
     int block;       // threads per workgroup: 64, 128 or 256
     int lds_stack;   // LDS stack entries per lane: 8, 16 or 32
     int grid;        // workgroups (persistent: work is fetched from an atomic counter)
+    bool wide;       // traverse the 4-wide layout where the scene has one
 };
 
 // Counters a kernel adds to: [0] extension rays, [1] shadow rays, [2] samples.

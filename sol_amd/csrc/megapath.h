@@ -27,7 +27,22 @@ SOL_DEV Stack make_stack(int* lds_stack, const TraversalScratch& sc) {
 
 // One sample of PathTracer::trace_path (src/algorithms/path_tracer.cpp:31-120 of sol): returns its
 // radiance. n_ext / n_shadow count the traced rays.
-template <int BLOCK, int LDS>
+// Closest / any hit of a wave-uniform scene over the binary (WIDE false) or 4-wide layout; the top
+// levels of that layout are staged in lds_nodes.
+template <bool ANY, int BLOCK, int LDS, bool WIDE>
+SOL_DEV bool trace_ray(const SceneView& s, const float4* lds_nodes, const Stack& st, const RayPre& r, float tmin,
+                       float tmax, HitRecord& hit) {
+    if (WIDE) return traverse4_with<ANY, BLOCK, LDS>(FetchUniform4(s, lds_nodes), st, r, tmin, tmax, hit);
+    return traverse<ANY, BLOCK, LDS>(s, lds_nodes, st, r, tmin, tmax, hit);
+}
+
+template <int BLOCK, bool WIDE>
+SOL_DEV void stage_top_nodes(const SceneView& s, float4* lds_nodes) {
+    if (WIDE) load_top_nodes4<BLOCK>(s, lds_nodes);
+    else load_top_nodes<BLOCK>(s, lds_nodes);
+}
+
+template <int BLOCK, int LDS, bool WIDE = false>
 SOL_DEV V3 trace_path(const SceneView& s, const sol_pt_config& cfg, const float4* lds_nodes, const Stack& st,
                       Pcg& rng, V3 org, V3 dir, unsigned& n_ext, unsigned& n_shadow) {
     const float pick = 1.0f / static_cast<float>(s.num_lights);
@@ -39,7 +54,7 @@ SOL_DEV V3 trace_path(const SceneView& s, const sol_pt_config& cfg, const float4
         RayPre r = make_ray_pre(org, dir);
         HitRecord hit;
         ++n_ext;
-        if (!traverse<false, BLOCK, LDS>(s, lds_nodes, st, r, tmin, FLT_MAX, hit)) break;
+        if (!trace_ray<false, BLOCK, LDS, WIDE>(s, lds_nodes, st, r, tmin, FLT_MAX, hit)) break;
         Surface sf = decode_hit(s, org, dir, hit.slot, hit.t, hit.u, hit.v);
         if (sf.light >= 0 && sf.front) {
             V3 Le;
@@ -56,7 +71,7 @@ SOL_DEV V3 trace_path(const SceneView& s, const sol_pt_config& cfg, const float4
                 RayPre sr = make_ray_pre(sf.point, ns.shadow_dir);
                 HitRecord unused;
                 ++n_shadow;
-                if (!traverse<true, BLOCK, LDS>(s, lds_nodes, st, sr, cfg.ray_offset, 1.0f - cfg.ray_offset, unused))
+                if (!trace_ray<true, BLOCK, LDS, WIDE>(s, lds_nodes, st, sr, cfg.ray_offset, 1.0f - cfg.ray_offset, unused))
                     color = color + ns.contrib;
             }
         }

@@ -45,6 +45,7 @@ inline solhip_tuning default_tuning() {
     t.tile_w = 32;
     t.tile_h = 32;
     t.graph_iters = 16;
+    t.bvh_width = 4;
     return t;
 }
 
@@ -78,11 +79,12 @@ struct solhip_ctx : solhip::Context {};
 struct solhip_scene {
     solhip::Context* ctx = nullptr;
     sol_gpu_node* nodes = nullptr;
+    sol_gpu_node4* nodes4 = nullptr;   // may be null (the wavefront traversal then uses nodes)
     sol_gpu_tri* tris = nullptr;
     sol_gpu_shade* shade = nullptr;
     sol_material* materials = nullptr;
     sol_light* lights = nullptr;
-    uint32_t num_nodes = 0, num_tris = 0, num_materials = 0, num_lights = 0, bvh_depth = 0;
+    uint32_t num_nodes = 0, num_tris = 0, num_materials = 0, num_lights = 0, bvh_depth = 0, bvh4_depth = 0;
     std::vector<sol_material> materials_host;
     std::vector<sol_light> lights_host;
     solhip::SceneView view{};

@@ -280,6 +280,180 @@ SOL_DEV bool traverse_with(const Fetch& f, const Stack& st, const RayPre& r, flo
     return ANY ? false : hit.prim >= 0;
 }
 
+// ---- 4-wide BVH (sol_gpu_node4: 8 float4 per node) -------------------------------------------------
+
+// Maximum number of top 4-wide nodes copied to LDS (three levels, 21 nodes, 2.6 KB).
+constexpr int kTopNodes4Max = 21;
+
+template <int BLOCK>
+SOL_DEV void load_top_nodes4(const SceneView& s, float4* lds_nodes) {
+    int n = static_cast<int>(s.top_nodes4 < static_cast<uint32_t>(kTopNodes4Max) ? s.top_nodes4 : kTopNodes4Max) * 8;
+    for (int i = threadIdx.x; i < n; i += BLOCK) lds_nodes[i] = s.nodes4[i];
+    __syncthreads();
+}
+
+struct Node4 {
+    float4 lo_x, hi_x, lo_y, hi_y, lo_z, hi_z;
+    int4 child;
+};
+
+SOL_DEV int4 as_int4(float4 v) {
+    return make_int4(__float_as_int(v.x), __float_as_int(v.y), __float_as_int(v.z), __float_as_int(v.w));
+}
+
+// Wave-uniform scene: top levels from LDS, the rest through buffer loads.
+struct FetchUniform4 {
+    __amdgpu_buffer_rsrc_t nodes, tris;
+    const float4* lds;
+    int top;
+
+    SOL_DEV FetchUniform4(const SceneView& s, const float4* lds_nodes)
+        : nodes(FetchUniform::rsrc(s.nodes4, s.num_nodes4 * 128u)), tris(FetchUniform::rsrc(s.tris, s.num_tris * 48u)),
+          lds(lds_nodes),
+          top(static_cast<int>(s.top_nodes4 < static_cast<uint32_t>(kTopNodes4Max) ? s.top_nodes4 : kTopNodes4Max)) {}
+
+    SOL_DEV void node(int cur, Node4& n) const {
+        if (cur < top) {
+            const float4* p = lds + cur * 8;
+            n.lo_x = p[0]; n.hi_x = p[1]; n.lo_y = p[2]; n.hi_y = p[3]; n.lo_z = p[4]; n.hi_z = p[5];
+            n.child = as_int4(p[6]);
+        } else {
+            uint32_t o = static_cast<uint32_t>(cur) * 128u;
+            n.lo_x = FetchUniform::ld4(nodes, o);
+            n.hi_x = FetchUniform::ld4(nodes, o + 16);
+            n.lo_y = FetchUniform::ld4(nodes, o + 32);
+            n.hi_y = FetchUniform::ld4(nodes, o + 48);
+            n.lo_z = FetchUniform::ld4(nodes, o + 64);
+            n.hi_z = FetchUniform::ld4(nodes, o + 80);
+            n.child = as_int4(FetchUniform::ld4(nodes, o + 96));
+        }
+    }
+    SOL_DEV void tri(uint32_t i, float4& p0, float4& p1, float4& p2) const {
+        uint32_t o = i * 48u;
+        p0 = FetchUniform::ld4(tris, o);
+        p1 = FetchUniform::ld3(tris, o + 16);
+        p2 = FetchUniform::ld3(tris, o + 32);
+    }
+};
+
+// Per-lane scene: global loads.
+struct FetchGlobal4 {
+    typedef FetchGlobal::v4f v4f;
+    typedef FetchGlobal::gptr gptr;
+    gptr nodes, tris;
+
+    SOL_DEV explicit FetchGlobal4(const SceneView& s) : nodes((gptr)(s.nodes4)), tris((gptr)(s.tris)) {}
+    SOL_DEV void node(int cur, Node4& n) const {
+        gptr p = nodes + cur * 8;
+        n.lo_x = FetchGlobal::f4(p[0]); n.hi_x = FetchGlobal::f4(p[1]);
+        n.lo_y = FetchGlobal::f4(p[2]); n.hi_y = FetchGlobal::f4(p[3]);
+        n.lo_z = FetchGlobal::f4(p[4]); n.hi_z = FetchGlobal::f4(p[5]);
+        n.child = as_int4(FetchGlobal::f4(p[6]));
+    }
+    SOL_DEV void tri(uint32_t i, float4& p0, float4& p1, float4& p2) const {
+        gptr p = tris + i * 3;
+        p0 = FetchGlobal::f4(p[0]);
+        p1 = FetchGlobal::f4(p[1]);
+        p2 = FetchGlobal::f4(p[2]);
+    }
+};
+
+template <int BLOCK, int LDS_STACK>
+SOL_DEV void stack_push(const Stack& st, TraversalState& ts, int v) {
+    if (ts.sp < LDS_STACK) {
+        st.lds[ts.sp * BLOCK] = v;
+    } else if (ts.sp < st.capacity) {
+        st.spill[(ts.sp - LDS_STACK) * st.spill_stride] = v;
+    } else {
+        *st.overflow = 1;
+        --ts.sp;
+    }
+    ++ts.sp;
+}
+
+// Orders two (distance, child) pairs by distance.
+SOL_DEV void order_pair(float& ta, int& ca, float& tb, int& cb) {
+    bool s = tb < ta;
+    float t = s ? tb : ta;
+    int c = s ? cb : ca;
+    tb = s ? ta : tb;
+    cb = s ? ca : cb;
+    ta = t;
+    ca = c;
+}
+
+// traverse_step over 4-wide nodes: the four child boxes of a node are tested, the hit children
+// are ordered by entry distance (a five-comparison sorting network), the nearest is visited next
+// and the others are pushed farthest first. Leaves as in traverse_step.
+template <int BLOCK, int LDS_STACK, typename Fetch>
+SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, float tmin, bool any, TraversalState& ts) {
+    constexpr float kMiss = __builtin_huge_valf();
+    while (ts.node >= 0) {
+        Node4 n;
+        f.node(ts.node, n);
+        float t0, t1, t2, t3;
+        bool h0 = intersect_box(r, n.lo_x.x, n.hi_x.x, n.lo_y.x, n.hi_y.x, n.lo_z.x, n.hi_z.x, tmin, ts.tmax, t0);
+        bool h1 = intersect_box(r, n.lo_x.y, n.hi_x.y, n.lo_y.y, n.hi_y.y, n.lo_z.y, n.hi_z.y, tmin, ts.tmax, t1);
+        bool h2 = intersect_box(r, n.lo_x.z, n.hi_x.z, n.lo_y.z, n.hi_y.z, n.lo_z.z, n.hi_z.z, tmin, ts.tmax, t2);
+        bool h3 = intersect_box(r, n.lo_x.w, n.hi_x.w, n.lo_y.w, n.hi_y.w, n.lo_z.w, n.hi_z.w, tmin, ts.tmax, t3);
+        int c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
+        h0 = h0 && c0 != SOL_GPU_EMPTY;
+        h1 = h1 && c1 != SOL_GPU_EMPTY;
+        h2 = h2 && c2 != SOL_GPU_EMPTY;
+        h3 = h3 && c3 != SOL_GPU_EMPTY;
+        const int hits = int(h0) + int(h1) + int(h2) + int(h3);
+        if (hits == 0) {
+            if (!stack_pop<BLOCK, LDS_STACK>(st, ts)) return kStepDone;
+            continue;
+        }
+        float k0 = h0 ? t0 : kMiss, k1 = h1 ? t1 : kMiss, k2 = h2 ? t2 : kMiss, k3 = h3 ? t3 : kMiss;
+        order_pair(k0, c0, k1, c1);
+        order_pair(k2, c2, k3, c3);
+        order_pair(k0, c0, k2, c2);
+        order_pair(k1, c1, k3, c3);
+        order_pair(k1, c1, k2, c2);
+        if (hits > 3) stack_push<BLOCK, LDS_STACK>(st, ts, c3);
+        if (hits > 2) stack_push<BLOCK, LDS_STACK>(st, ts, c2);
+        if (hits > 1) stack_push<BLOCK, LDS_STACK>(st, ts, c1);
+        ts.node = c0;
+    }
+    uint32_t first, count;
+    decode_leaf(ts.node, first, count);
+    for (uint32_t i = 0; i < count; ++i) {
+        float4 p0, p1, p2;
+        f.tri(first + i, p0, p1, p2);
+        float t, u, v;
+        if (intersect_triangle(r, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, t, u, v)) {
+            int prim = __float_as_int(p0.w);
+            if (any) {
+                if (t > tmin && t < ts.tmax) return kStepOccluded;
+            } else if (t > tmin && (t < ts.tmax || (t == ts.tmax && ts.hit.prim >= 0 && prim < ts.hit.prim))) {
+                ts.tmax = t;
+                ts.hit.slot = static_cast<int>(first + i);
+                ts.hit.prim = prim;
+                ts.hit.t = t;
+                ts.hit.u = u;
+                ts.hit.v = v;
+            }
+        }
+    }
+    return stack_pop<BLOCK, LDS_STACK>(st, ts) ? kStepContinue : kStepDone;
+}
+
+// A whole traversal over the 4-wide layout, looping traverse4_step.
+template <bool ANY, int BLOCK, int LDS_STACK, typename Fetch>
+SOL_DEV bool traverse4_with(const Fetch& f, const Stack& st, const RayPre& r, float tmin, float tmax, HitRecord& hit) {
+    TraversalState ts;
+    ts.start(tmax);
+    int status;
+    do {
+        status = traverse4_step<BLOCK, LDS_STACK>(f, st, r, tmin, ANY, ts);
+    } while (status == kStepContinue);
+    hit = ts.hit;
+    if (ANY) return status == kStepOccluded;
+    return ts.hit.prim >= 0;
+}
+
 // Traversal of a wave-uniform scene whose top levels are staged in lds_nodes.
 template <bool ANY, int BLOCK, int LDS_STACK>
 SOL_DEV bool traverse(const SceneView& s, const float4* lds_nodes, const Stack& st, const RayPre& r, float tmin,

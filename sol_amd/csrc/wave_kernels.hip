@@ -300,9 +300,11 @@ constexpr int kRefillLanes = 16;
 // (fetching a new chunk when the current one runs out). Entries [0, n_ext) are the closest-hit
 // queries of extension queue `cur` (hit written in place), [n_ext, n_ext + n_shadow) the any-hit
 // queries of the shadow queue (unoccluded contribution added to the path or its sample).
-template <int BLOCK, int LDS, bool MULTI>
-__global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
+// WIDE: traverse the 4-wide layout (nodes4) instead of the binary one.
+template <int BLOCK, int LDS, bool MULTI, bool WIDE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5 : 6))) void k_trace(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                  int cur, TraversalScratch sc) {
+    static_assert(kTopNodes4Max * 8 <= kTopNodesMax * 4, "LDS node cache too small for the 4-wide top levels");
     __shared__ float4 lds_nodes[kTopNodesMax * 4];
     __shared__ int lds_stack[LDS * BLOCK];
     const SegQueue eqv(b.counts, kQExtend0 + cur, b.cap_e), sqv(b.counts, kQShadow, b.cap_s);
@@ -310,12 +312,17 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
     const unsigned n_shadow = sqv.size();
     const unsigned total = n_ext + n_shadow;
     if (total == 0) return;
-    if (!MULTI) load_top_nodes<BLOCK>(single, lds_nodes);
+    if (!MULTI) {
+        if (WIDE) load_top_nodes4<BLOCK>(single, lds_nodes);
+        else load_top_nodes<BLOCK>(single, lds_nodes);
+    }
     const Stack st = make_stack_w<BLOCK>(lds_stack, sc, LDS);
     const unsigned waves = gridDim.x * (BLOCK / 64);
     const float4* q = b.eq[cur];
     const FetchUniform fu(single, lds_nodes);
+    const FetchUniform4 fu4(single, lds_nodes);
     FetchGlobal fg(single);
+    FetchGlobal4 fg4(single);
     int idx = -1;        // entry this lane traces, -1 = idle
     bool any = false;
     float tmin = 0.0f;
@@ -355,7 +362,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
                         tmin = e0.w;
                         r = make_ray_pre(xyz(e0), xyz(e1));
                         ts.start(FLT_MAX);
-                        if (MULTI) fg = FetchGlobal(tabs.scenes[as_u(b.path[slot * 4 + 3].x)]);
+                        if (MULTI) {
+                            const SceneView& sv = tabs.scenes[as_u(b.path[slot * 4 + 3].x)];
+                            fg = FetchGlobal(sv);
+                            fg4 = FetchGlobal4(sv);
+                        }
                         ++n_closest;
                     }
                 } else if (e != 0xFFFFFFFFu) {
@@ -368,12 +379,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
                     ts.start(1.0f - cfg.ray_offset);
                     if (MULTI) {
                         uint32_t target = as_u(e0.w);
+                        uint32_t sid;
                         if (target == kSampleTarget) {
                             unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) | as_u(se[2].w);
-                            fg = FetchGlobal(tabs.scenes[tabs.job_scene[find_job(tabs, item)]]);
+                            sid = tabs.job_scene[find_job(tabs, item)];
                         } else {
-                            fg = FetchGlobal(tabs.scenes[as_u(b.path[target * 4 + 3].x)]);
+                            sid = as_u(b.path[target * 4 + 3].x);
                         }
+                        fg = FetchGlobal(tabs.scenes[sid]);
+                        fg4 = FetchGlobal4(tabs.scenes[sid]);
                     }
                     ++n_any;
                 }
@@ -392,8 +406,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView single, WaveTables ta
             continue;
         }
         if (idx >= 0) {
-            int status = MULTI ? traverse_step<BLOCK, LDS>(fg, st, r, tmin, any, ts)
-                               : traverse_step<BLOCK, LDS>(fu, st, r, tmin, any, ts);
+            int status;
+            if (WIDE) status = MULTI ? traverse4_step<BLOCK, LDS>(fg4, st, r, tmin, any, ts)
+                                     : traverse4_step<BLOCK, LDS>(fu4, st, r, tmin, any, ts);
+            else status = MULTI ? traverse_step<BLOCK, LDS>(fg, st, r, tmin, any, ts)
+                                : traverse_step<BLOCK, LDS>(fu, st, r, tmin, any, ts);
             if (status != kStepContinue) {
                 const unsigned e = static_cast<unsigned>(idx);
                 if (!any) {
@@ -676,8 +693,12 @@ __global__ __launch_bounds__(256) void k_reduce(WaveTables tabs, WaveBuf b, uint
 
 template <int BLOCK, int LDS, bool MULTI>
 void launch_trace_t(const WaveLaunch& L, int cur) {
-    hipLaunchKernelGGL((k_trace<BLOCK, LDS, MULTI>), dim3(L.grid_traverse), dim3(BLOCK), 0, L.stream, L.single, L.tabs,
-                       L.buf, L.cfg, cur, L.scratch);
+    if (L.wide)
+        hipLaunchKernelGGL((k_trace<BLOCK, LDS, MULTI, true>), dim3(L.grid_traverse), dim3(BLOCK), 0, L.stream, L.single,
+                           L.tabs, L.buf, L.cfg, cur, L.scratch);
+    else
+        hipLaunchKernelGGL((k_trace<BLOCK, LDS, MULTI, false>), dim3(L.grid_traverse), dim3(BLOCK), 0, L.stream,
+                           L.single, L.tabs, L.buf, L.cfg, cur, L.scratch);
 }
 
 } // namespace

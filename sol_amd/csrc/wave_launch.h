@@ -20,6 +20,7 @@ struct WaveLaunch {
     int grid_kshade[3];       // kernels hand out rounds by workgroup index, so every workgroup
     int grid_start;           // of the grid must be resident at once
     bool multi;
+    bool wide;                // every scene has the 4-wide layout: k_trace traverses it
     hipStream_t stream;
 };
 

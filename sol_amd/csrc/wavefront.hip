@@ -154,9 +154,9 @@ std::string graph_key(const WaveLaunch& L, const WavefrontState& st, int iters) 
     k.append(reinterpret_cast<const char*>(&L.single), sizeof(L.single));
     k.append(reinterpret_cast<const char*>(&L.tabs), sizeof(L.tabs));
     k.append(reinterpret_cast<const char*>(&L.buf), sizeof(L.buf));
-    int meta[11] = {L.block,          L.lds_stack,      L.grid_traverse,  L.grid_shade,
+    int meta[12] = {L.block,          L.lds_stack,      L.grid_traverse,  L.grid_shade,
                     L.grid_sort,      L.grid_kshade[0], L.grid_kshade[1], L.grid_kshade[2],
-                    L.grid_start,     L.multi ? 1 : 0,  iters};
+                    L.grid_start,     L.multi ? 1 : 0,  L.wide ? 1 : 0,   iters};
     k.append(reinterpret_cast<const char*>(meta), sizeof(meta));
     k.append(reinterpret_cast<const char*>(&st.generation), sizeof(st.generation));
     k.append(reinterpret_cast<const char*>(&L.scratch), sizeof(L.scratch));
@@ -165,7 +165,7 @@ std::string graph_key(const WaveLaunch& L, const WavefrontState& st, int iters) 
 
 // Runs the jobs to completion. Jobs index `views` through Job::scene.
 int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::vector<Job>& jobs,
-                  const sol_pt_config& cfg, uint32_t max_depth) {
+                  const sol_pt_config& cfg, uint32_t max_depth, uint32_t max_depth4) {
     if (!ctx.wave) ctx.wave = new WavefrontState();
     WavefrontState& st = *ctx.wave;
     const solhip_tuning& tu = ctx.tuning;
@@ -283,12 +283,17 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     L.grid_traverse = persistent_grid(ctx.info.compute_units, L.block);
     L.grid_shade = persistent_grid(ctx.info.compute_units, kWaveBlock);
     L.multi = multi;
+    L.wide = tu.bvh_width == 4 &&
+             std::all_of(views.begin(), views.end(), [](const SceneView& v) { return v.nodes4 != nullptr; });
     HIP_OK(wave_set_grids(L, ctx.info.compute_units));
     L.stream = s;
     if ((rc = ctx.ensure_scratch(L.grid_traverse * L.block))) return rc;
     L.scratch = ctx.scratch;
     if (max_depth > static_cast<uint32_t>(L.lds_stack + ctx.scratch.spill_entries + 1))
         return set_error(SOLHIP_E_TOO_DEEP, "BVH deeper than the traversal stack");
+    // a 4-wide level pushes up to three nodes
+    if (L.wide && 3 * (max_depth4 > 0 ? max_depth4 - 1 : 0) > static_cast<uint32_t>(L.lds_stack + ctx.scratch.spill_entries))
+        return set_error(SOLHIP_E_TOO_DEEP, "4-wide BVH deeper than the traversal stack");
 
     const int iters = static_cast<int>(std::max<uint32_t>(2, tu.graph_iters & ~1u));
     // Profiling records HIP events between the kernels of every iteration. Events recorded by a
@@ -380,7 +385,7 @@ int render_wavefront(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint
     if (cfg.max_path_len == 0) return SOLHIP_OK;
     std::vector<SceneView> views{s.view};
     std::vector<Job> jobs{Job{0, w, h, si, sc, d_accum}};
-    return run_wavefront(*s.ctx, views, jobs, cfg, s.bvh_depth);
+    return run_wavefront(*s.ctx, views, jobs, cfg, s.bvh_depth, s.bvh4_depth);
 }
 
 } // namespace solhip
@@ -396,7 +401,7 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
     std::vector<SceneView> views;
     std::unordered_map<const solhip_scene*, uint32_t> index;
     std::vector<Job> list;
-    uint32_t depth = 0;
+    uint32_t depth = 0, depth4 = 0;
     for (uint32_t i = 0; i < num_jobs; ++i) {
         const solhip_batch_job& j = jobs[i];
         if (!j.scene || !j.accum || j.width == 0 || j.height == 0) return set_error(SOLHIP_E_INVALID, "invalid batch job");
@@ -409,6 +414,7 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
             index[j.scene] = si;
             views.push_back(j.scene->view);
             depth = std::max(depth, j.scene->bvh_depth);
+            depth4 = std::max(depth4, j.scene->bvh4_depth);
         } else {
             si = it->second;
         }
@@ -417,14 +423,14 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
     if (list.empty() || cfg->max_path_len == 0) return SOLHIP_OK;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return set_error(SOLHIP_E_HIP, hipGetErrorString(e));
-    int rc = run_wavefront(*ctx, views, list, *cfg, depth);
+    int rc = run_wavefront(*ctx, views, list, *cfg, depth, depth4);
     if (rc) return rc;
     return ctx->check_overflow();
 }
 
 // Launch-parameter search for the wavefront renderer on one scene and image size: coordinate
 // descent from the current parameters over paths in flight, traversal workgroup size, LDS stack
-// depth, bounce iterations per graph and tile shape. Every candidate renders budget_spp samples
+// depth, bounce iterations per graph, BVH node width and tile shape. Every candidate renders budget_spp samples
 // per pixel twice into a scratch image (the first render builds the slot arena and captures the
 // graph) and the second is timed with HIP events. A candidate replaces the current best if it is
 // at least 1 % faster. The statistics and the profiling switch of the context are restored.
@@ -468,7 +474,7 @@ int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
     solhip_tuning best = start;
     float best_ms = 0.0f;
     int rc = trial(best, best_ms);
-    for (int dim = 0; rc == SOLHIP_OK && dim < 5; ++dim) {
+    for (int dim = 0; rc == SOLHIP_OK && dim < 6; ++dim) {
         std::vector<solhip_tuning> cands;
         solhip_tuning t = best;
         switch (dim) {
@@ -476,6 +482,7 @@ int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
             case 1: for (uint32_t v : {128u, 256u}) { t.block_size = v; cands.push_back(t); } break;
             case 2: for (uint32_t v : {8u, 16u}) { t.lds_stack = v; cands.push_back(t); } break;
             case 3: for (uint32_t v : {8u, 16u, 32u}) { t.graph_iters = v; cands.push_back(t); } break;
+            case 4: for (uint32_t v : {2u, 4u}) { t.bvh_width = v; cands.push_back(t); } break;
             default:
                 for (auto wh : {std::pair<uint32_t, uint32_t>{16, 16}, {32, 32}, {64, 16}, {128, 8}}) {
                     t.tile_w = wh.first;

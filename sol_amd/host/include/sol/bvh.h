@@ -54,6 +54,16 @@ struct GpuLayout {
 };
 GpuLayout to_gpu_layout(const Bvh& bvh, uint32_t top_levels);
 
+// The binary tree collapsed to 4-wide nodes: starting from a node's two children, the child
+// with the largest surface area that is an inner node is replaced by its two children until there
+// are four or only leaves remain. Top levels breadth-first, the rest depth-first.
+struct GpuLayout4 {
+    std::vector<sol_gpu_node4> nodes;
+    uint32_t top_nodes = 0;
+    uint32_t depth = 0;
+};
+GpuLayout4 to_gpu_layout4(const Bvh& bvh, uint32_t top_levels);
+
 } // namespace sol::bvh
 
 #endif

@@ -47,6 +47,8 @@ struct FlatScene {
     std::vector<uint32_t> prim_indices;
     // GPU layout.
     std::vector<sol_gpu_node> gpu_nodes;
+    std::vector<sol_gpu_node4> gpu_nodes4;
+    uint32_t bvh4_depth = 0, top_nodes4 = 0;
     std::vector<sol_gpu_tri> gpu_tris;
     std::vector<sol_gpu_shade> gpu_shade;
     uint32_t bvh_depth = 0, top_nodes = 0;

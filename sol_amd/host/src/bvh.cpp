@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <deque>
+#include <unordered_map>
 #include <numeric>
 #include <queue>
 #include <stdexcept>
@@ -416,6 +417,104 @@ GpuLayout to_gpu_layout(const Bvh& bvh, uint32_t top_levels) {
             g.child[c] = encode_child(bvh, ch, gpu_index);
         }
     }
+    return out;
+}
+
+GpuLayout4 to_gpu_layout4(const Bvh& bvh, uint32_t top_levels) {
+    GpuLayout4 out;
+    std::vector<int32_t> none;
+    auto set_child = [&](sol_gpu_node4& g, int slot, uint32_t id, int32_t ref) {
+        const BBox& b = bvh.nodes[id].box;
+        g.lo_x[slot] = b.lo.x; g.hi_x[slot] = b.hi.x;
+        g.lo_y[slot] = b.lo.y; g.hi_y[slot] = b.hi.y;
+        g.lo_z[slot] = b.lo.z; g.hi_z[slot] = b.hi.z;
+        g.child[slot] = ref;
+    };
+    auto empty_node = [] {
+        sol_gpu_node4 g{};
+        for (int k = 0; k < 4; ++k) g.child[k] = SOL_GPU_EMPTY;
+        return g;
+    };
+    const Node& root = bvh.nodes[0];
+    if (root.is_leaf()) {
+        sol_gpu_node4 g = empty_node();
+        set_child(g, 0, 0, encode_child(bvh, 0, none));
+        out.nodes.push_back(g);
+        out.top_nodes = 1;
+        out.depth = 2;
+        return out;
+    }
+    // The children (binary node ids) of the 4-wide node made from binary inner node id.
+    auto collapse = [&](uint32_t id) {
+        std::vector<uint32_t> kids{bvh.nodes[id].index, bvh.nodes[id].index + 1};
+        while (kids.size() < 4) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (size_t k = 0; k < kids.size(); ++k) {
+                const Node& nd = bvh.nodes[kids[k]];
+                if (nd.is_leaf()) continue;
+                float a = nd.box.half_area();
+                if (a > best_area) {
+                    best_area = a;
+                    best = static_cast<int>(k);
+                }
+            }
+            if (best < 0) break;
+            uint32_t open = kids[best];
+            kids[best] = bvh.nodes[open].index;
+            kids.insert(kids.begin() + best + 1, bvh.nodes[open].index + 1);
+        }
+        return kids;
+    };
+    // Order the 4-wide nodes (identified by their binary node id): top levels breadth-first, then
+    // each remaining subtree depth-first.
+    std::vector<uint32_t> order, rest;
+    std::unordered_map<uint32_t, std::vector<uint32_t>> kids;
+    std::deque<std::pair<uint32_t, uint32_t>> queue{{0u, 1u}};
+    uint32_t depth = 1;
+    while (!queue.empty()) {
+        auto [id, level] = queue.front();
+        queue.pop_front();
+        if (level <= top_levels) {
+            order.push_back(id);
+            auto k = collapse(id);
+            for (uint32_t c : k)
+                if (!bvh.nodes[c].is_leaf()) queue.push_back({c, level + 1});
+            kids.emplace(id, std::move(k));
+        } else {
+            rest.push_back(id);
+        }
+    }
+    out.top_nodes = static_cast<uint32_t>(order.size());
+    std::vector<uint32_t> stack;
+    for (uint32_t r : rest) {
+        stack.push_back(r);
+        while (!stack.empty()) {
+            uint32_t id = stack.back();
+            stack.pop_back();
+            order.push_back(id);
+            auto k = collapse(id);
+            for (size_t j = k.size(); j-- > 0;)
+                if (!bvh.nodes[k[j]].is_leaf()) stack.push_back(k[j]);
+            kids.emplace(id, std::move(k));
+        }
+    }
+    std::vector<int32_t> gpu_index(bvh.nodes.size(), -1);
+    for (size_t i = 0; i < order.size(); ++i) gpu_index[order[i]] = static_cast<int32_t>(i);
+    out.nodes.resize(order.size());
+    std::vector<uint32_t> level_of(order.size(), 0);
+    level_of[0] = 1;
+    for (size_t i = 0; i < order.size(); ++i) {
+        sol_gpu_node4 g = empty_node();
+        const auto& k = kids.at(order[i]);
+        for (size_t j = 0; j < k.size(); ++j) {
+            set_child(g, static_cast<int>(j), k[j], encode_child(bvh, k[j], gpu_index));
+            if (!bvh.nodes[k[j]].is_leaf()) level_of[gpu_index[k[j]]] = level_of[i] + 1;
+            else depth = std::max(depth, level_of[i] + 1);
+        }
+        out.nodes[i] = g;
+    }
+    out.depth = depth;
     return out;
 }
 

@@ -203,6 +203,10 @@ solhip_scene_desc FlatScene::gpu_desc() const {
     d.camera = camera;
     d.bvh_depth = bvh_depth;
     d.top_nodes = top_nodes;
+    d.num_nodes4 = static_cast<uint32_t>(gpu_nodes4.size());
+    d.nodes4 = gpu_nodes4.data();
+    d.bvh4_depth = bvh4_depth;
+    d.top_nodes4 = top_nodes4;
     return d;
 }
 
@@ -272,6 +276,11 @@ FlatScene flatten_scene(const Scene& scene, uint32_t top_levels) {
     f.gpu_nodes = std::move(layout.nodes);
     f.top_nodes = layout.top_nodes;
     f.bvh_depth = layout.depth;
+    // 4-wide: three levels breadth-first (at most 21 nodes, the LDS-staged part)
+    bvh::GpuLayout4 layout4 = bvh::to_gpu_layout4(b, 3);
+    f.gpu_nodes4 = std::move(layout4.nodes);
+    f.top_nodes4 = layout4.top_nodes;
+    f.bvh4_depth = layout4.depth;
     f.gpu_tris.resize(nt);
     f.gpu_shade.resize(nt);
     #pragma omp parallel for schedule(static)

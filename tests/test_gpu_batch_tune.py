@@ -48,15 +48,15 @@ def test_autotune_returns_valid_parameters_and_keeps_the_image(ctx):
     t = g.autotune(160, 90, budget_spp=2, config=cfg)
     assert t.block_size in (128, 256) and t.lds_stack in (8, 16)
     assert 4096 <= t.path_slots <= (1 << 24) and t.graph_iters in (8, 16, 32)
-    assert (t.tile_w, t.tile_h) in ((16, 16), (32, 32), (64, 16), (128, 8))
+    assert (t.tile_w, t.tile_h) in ((16, 16), (32, 32), (64, 16), (128, 8)) and t.bvh_width in (2, 4)
     cur = ctx.tuning()
-    assert (cur.block_size, cur.lds_stack, cur.path_slots, cur.graph_iters, cur.tile_w, cur.tile_h) == \
-        (t.block_size, t.lds_stack, t.path_slots, t.graph_iters, t.tile_w, t.tile_h)
+    fields = ("block_size", "lds_stack", "path_slots", "graph_iters", "tile_w", "tile_h", "bvh_width")
+    assert [getattr(cur, f) for f in fields] == [getattr(t, f) for f in fields]
     assert ctx.stats().samples == 0          # trial renders are not counted
     after = sol_amd.new_image(160, 90)
     g.render(after, 0, 4, cfg)
     np.testing.assert_array_equal(after, before)
-    ctx.set_tuning(**{k: 0 for k in ("block_size", "lds_stack", "path_slots", "tile_w", "tile_h", "graph_iters")})
+    ctx.set_tuning(**{k: 0 for k in fields})
 
 
 def test_tuning_rejects_unsupported_values(ctx):
@@ -64,6 +64,8 @@ def test_tuning_rejects_unsupported_values(ctx):
         ctx.set_tuning(block_size=96)
     with pytest.raises(F.SolError):
         ctx.set_tuning(path_slots=10)
+    with pytest.raises(F.SolError):
+        ctx.set_tuning(bvh_width=3)
     g = ctx.upload(sol_amd.Scene.cornell_box())
     with pytest.raises(F.SolError):
         g.autotune(0, 16)

@@ -221,3 +221,27 @@ def test_wavefront_small_renders_take_every_work_item(ctx, cornell, w, h, spp):
     g.render(a, 11, spp, cfg, renderer=F.RENDERER_WAVEFRONT)
     g.render(b, 11, spp, cfg, renderer=F.RENDERER_MEGAKERNEL)
     np.testing.assert_array_equal(a, b)
+
+
+def test_binary_and_four_wide_traversal_agree(ctx):
+    # The closest hit is the minimum of (t, original triangle index) over every triangle the exact
+    # test accepts, so the node layout cannot change any result.
+    scene = sol_amd.Scene.sphere_field(num_spheres=40, subdivision=2, seed=5, extent=4.0, radius_max=1.0)
+    g = ctx.upload(scene)
+    soup = ctx.upload(sol_amd.Scene.triangle_soup(50_000, seed=8))
+    rays = sol_amd.gen_rays(0, 1 << 16)
+    shadow = sol_amd.gen_rays(1, 1 << 16)
+    cfg = _cfg(max_path_len=12)
+    out = {}
+    try:
+        for width in (2, 4):
+            ctx.set_tuning(bvh_width=width)
+            img = sol_amd.new_image(96, 54)
+            g.render(img, 0, 4, cfg, renderer=F.RENDERER_WAVEFRONT)
+            out[width] = (img, soup.intersect(rays), soup.intersect(shadow, any_hit=True))
+    finally:
+        ctx.set_tuning(bvh_width=0)
+    np.testing.assert_array_equal(out[2][0], out[4][0])
+    for a, b in zip(out[2][1], out[4][1]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(out[2][2], out[4][2])

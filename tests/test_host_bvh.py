@@ -78,6 +78,54 @@ def test_gpu_layout_covers_every_leaf():
     assert d.top_nodes >= 1 and d.bvh_depth >= 2
 
 
+@pytest.mark.parametrize("make", [lambda: sol_amd.Scene.triangle_soup(5000, seed=3),
+                                  lambda: sol_amd.Scene.sphere_field(num_spheres=30, subdivision=2, seed=4),
+                                  lambda: sol_amd.Scene.cornell_box()])
+def test_gpu_layout4_covers_every_leaf_and_bounds_its_subtrees(make):
+    s = make()
+    d = s.gpu_desc()
+    n4 = d.num_nodes4
+    raw = np.ctypeslib.as_array(C.cast(d.nodes4, C.POINTER(C.c_float)), shape=(n4 * 32,)).reshape(-1, 32)
+    boxes = raw[:, :24].reshape(n4, 6, 4)             # lo_x, hi_x, lo_y, hi_y, lo_z, hi_z x 4 slots
+    child = raw[:, 24:28].view(np.int32)
+    tris = np.ctypeslib.as_array(C.cast(d.tris, C.POINTER(C.c_float)), shape=(d.num_tris * 12,)).reshape(-1, 12)
+    verts = tris[:, [0, 1, 2, 4, 5, 6, 8, 9, 10]].reshape(-1, 3, 3)
+    covered = np.zeros(d.num_tris, np.int64)
+    referenced = np.zeros(n4, np.int64)
+    # box of every subtree, bottom-up (children come after parents in the layout)
+    sub_lo = np.full((n4, 3), np.inf, np.float32)
+    sub_hi = np.full((n4, 3), -np.inf, np.float32)
+    filled = []
+    for i in range(n4 - 1, -1, -1):
+        used = 0
+        for k in range(4):
+            c = int(child[i, k])
+            if c == F.GPU_EMPTY:
+                continue
+            used += 1
+            lo = boxes[i, [0, 2, 4], k]
+            hi = boxes[i, [1, 3, 5], k]
+            if c >= 0:
+                assert c > i
+                referenced[c] += 1
+                assert np.all(lo <= sub_lo[c]) and np.all(hi >= sub_hi[c])   # child box bounds its subtree
+            else:
+                code = ~c & 0xFFFFFFFF
+                first, cnt = code >> 3, (code & 7) + 1
+                covered[first:first + cnt] += 1
+                v = verts[first:first + cnt].reshape(-1, 3)
+                assert np.all(lo <= v.min(0)) and np.all(hi >= v.max(0))
+            sub_lo[i] = np.minimum(sub_lo[i], lo)
+            sub_hi[i] = np.maximum(sub_hi[i], hi)
+        assert used >= (1 if n4 == 1 else 2)
+        filled.append(used)
+    assert np.all(covered == 1)
+    assert referenced[0] == 0 and np.all(referenced[1:] == 1)
+    assert 1 <= d.top_nodes4 <= 21 and 2 <= d.bvh4_depth <= d.bvh_depth
+    if d.num_tris > 1000:
+        assert np.mean(filled) > 3.0                    # collapsing fills most slots
+
+
 def test_cornell_box_scene():
     s = sol_amd.Scene.cornell_box()
     assert s.num_tris == 36
