@@ -32,7 +32,8 @@ enum {
     SOL_E_INVALID     = -1, /* invalid argument                          */
     SOL_E_NO_LIGHTS   = -2, /* scene without light (undefined in sol, path_tracer.cpp:27,36) */
     SOL_E_ALLOC       = -3, /* host allocation failed                    */
-    SOL_E_INTERNAL    = -4
+    SOL_E_INTERNAL    = -4,
+    SOL_E_LOAD        = -5  /* scene / OBJ file cannot be opened or parsed */
 };
 
 typedef struct sol_scene sol_scene;
@@ -97,6 +98,16 @@ typedef struct sol_sphere_field_params {
     float radius_max;
 } sol_sphere_field_params;
 
+/* Values a scene file may leave out: sol::Scene::Defaults (reference include/sol/scene.h:23-29).
+   sol's driver sets aspect_ratio to width / height (test/driver.cpp:197-200). */
+typedef struct sol_scene_defaults {
+    float fov;            /* horizontal, degrees: 60 */
+    float aspect_ratio;   /* 1 */
+    float eye[3];         /* 0 0 0 */
+    float dir[3];         /* 0 0 1 */
+    float up[3];          /* 0 1 0 */
+} sol_scene_defaults;
+
 const char* sol_host_last_error(void);
 
 void sol_default_bvh_config(sol_bvh_config* out);
@@ -116,7 +127,26 @@ int sol_scene_sphere_field(const sol_sphere_field_params* p, const sol_bvh_confi
 int sol_scene_triangle_soup(uint32_t num_tris, uint64_t seed, const sol_bvh_config* cfg,
                             sol_scene** out);
 
-/* Scene from caller arrays (copied). */
+void sol_default_scene_defaults(sol_scene_defaults* out);
+
+/* Scene file: sol::Scene::load (reference src/scene_loader.cpp:30-108). TOML with an optional
+   [camera] table (type "perspective"; eye, dir, up, fov, aspect), [[objects]] entries of type
+   "import" naming an OBJ file relative to the scene file, and the name of the root object.
+   OBJ / MTL import follows src/formats/obj.cpp in its non-strict mode; a material with both Kd
+   and Ks becomes SOL_BSDF_INTERP, each triangle of a face with non-zero Ke a triangle light.
+   SOL_E_LOAD with the message in sol_host_last_error() if a file cannot be opened or parsed;
+   defaults may be null (the values above). */
+int sol_scene_load(const char* file_name, const sol_scene_defaults* defaults, const sol_bvh_config* cfg,
+                   sol_scene** out);
+
+/* One OBJ file as the whole scene (sol::obj::load, src/formats/obj.cpp:512-531) seen through
+   the given camera. */
+int sol_scene_load_obj(const char* file_name, const sol_camera_params* camera, const sol_bvh_config* cfg,
+                       sol_scene** out);
+
+/* Scene from caller arrays (copied). A SOL_BSDF_INTERP material adds its diffuse and Phong
+   components to the material table after the caller's materials (indices 0 .. n-1 keep their
+   meaning). */
 int sol_scene_from_arrays(const sol_mesh_view* mesh, const sol_bvh_config* cfg,
                           sol_scene** out);
 

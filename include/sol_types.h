@@ -4,8 +4,10 @@
  * C ABI, be memcpy'd to HBM unchanged and be mirrored by ctypes.
  *
  * The records restate, as flat data, the scene objects that sol keeps behind virtual classes:
- *   sol_material  <- DiffuseBsdf / PhongBsdf / GlassBsdf / MirrorBsdf with constant textures
- *                    (reference: include/sol/bsdfs.h:97-164, include/sol/textures.h:105-144)
+ *   sol_material  <- DiffuseBsdf / PhongBsdf / GlassBsdf / MirrorBsdf with constant textures,
+ *                    and InterpBsdf over a DiffuseBsdf and a PhongBsdf (the mix the OBJ loader
+ *                    makes, src/formats/obj.cpp:388-409)
+ *                    (reference: include/sol/bsdfs.h:97-182, include/sol/textures.h:105-144)
  *   sol_light     <- UniformTriangleLight / PointLight (include/sol/lights.h:91-139,
  *                    include/sol/shapes.h:49-60)
  *   sol_camera_params <- PerspectiveCamera constructor arguments (include/sol/cameras.h:39-46)
@@ -26,7 +28,9 @@ enum {
     SOL_BSDF_DIFFUSE = 0,  /* Lambertian, Type::Diffuse  (bsdfs.cpp:31-52)            */
     SOL_BSDF_PHONG   = 1,  /* modified Phong, Type::Glossy (bsdfs.cpp:64-104)         */
     SOL_BSDF_GLASS   = 2,  /* dielectric, Type::Specular (bsdfs.cpp:131-165)          */
-    SOL_BSDF_MIRROR  = 3   /* perfect mirror, Type::Specular (bsdfs.cpp:108-119)      */
+    SOL_BSDF_MIRROR  = 3,  /* perfect mirror, Type::Specular (bsdfs.cpp:108-119)      */
+    SOL_BSDF_INTERP  = 4   /* InterpBsdf(DiffuseBsdf(kd), PhongBsdf(ks, ns), mix): Type::Diffuse
+                              (bsdfs.cpp:179-230)                                        */
 };
 
 enum {
@@ -45,7 +49,9 @@ typedef struct sol_material {
     float ns_sample;    /* Phong exponent used by sample()/pdf(); equal to ns for plain
                            rendering, held fixed for frozen-sampling gradient checks    */
     int32_t group;      /* parameter group for gradients (-1 = not differentiated)      */
-    float pad[2];
+    float mix;          /* SOL_BSDF_INTERP: weight of the Phong lobe, the constant texture k
+                           of InterpBsdf (obj.cpp: spec_k / (diff_k + spec_k))           */
+    float pad;
 } sol_material;
 
 /* One light. 80 bytes. For a point light only v0 (position) and intensity are used. */

@@ -16,6 +16,9 @@
 //   Diffuse / Phong / Glass / Mirror   bsdfs.cpp:11-165, bsdfs.h:89-94           closed forms: pdf integrals,
 //                                                                               Fresnel at normal incidence,
 //                                                                               TIR, reflection law
+//   Interp (diffuse + Phong mix)       bsdfs.cpp:179-230, color.h:66-71           mix 0 / 1 = the single lobe,
+//                                      (proto::lerp not vendored:                 pdf integral, lobe-pick
+//                                      a (1 - t) + b t assumed)                   frequency
 //   triangle / point lights            lights.cpp:11-111, shapes.cpp:6-29        uniform-density, pdf = 1/A
 //   trace_path / render                path_tracer.cpp:9-120                     furnace identity, MIS
 //                                                                               consistency (NEE-only =
@@ -63,6 +66,8 @@ inline Vec3 vec(const float* p) { return {p[0], p[1], p[2]}; }
 inline Vec3 reflect(Vec3 d, Vec3 n) { return d - n * (2.0f * dot(d, n)); }
 inline Vec3 lerp(Vec3 a, Vec3 b, Vec3 c, float u, float v) { return a * ((1.0f - u) - v) + b * u + c * v; }
 inline float positive_dot(Vec3 a, Vec3 b) { return std::max(dot(a, b), 0.0f); }
+inline float lerp(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+inline Vec3 lerp(Vec3 a, Vec3 b, float t) { return a * (1.0f - t) + b * t; }
 inline float negative_dot(Vec3 a, Vec3 b) { return std::min(dot(a, b), 0.0f); }
 
 using Color = Vec3;
@@ -424,10 +429,23 @@ bool intersect_any(const oracle_scene& s, const Ray& ray, Counters& c) {
 
 enum class BsdfType { Diffuse, Glossy, Specular };
 
+// InterpBsdf::infer_type (bsdfs.cpp:226-230): the mix of a diffuse and a Phong lobe is Diffuse.
 BsdfType bsdf_type(const sol_material& m) {
-    if (m.type == SOL_BSDF_DIFFUSE) return BsdfType::Diffuse;
+    if (m.type == SOL_BSDF_DIFFUSE || m.type == SOL_BSDF_INTERP) return BsdfType::Diffuse;
     if (m.type == SOL_BSDF_PHONG) return BsdfType::Glossy;
     return BsdfType::Specular;
+}
+
+// The two BSDFs an InterpBsdf mixes (a = diffuse, b = Phong, as the OBJ loader builds it).
+sol_material diffuse_part(const sol_material& m) {
+    sol_material d = m;
+    d.type = SOL_BSDF_DIFFUSE;
+    return d;
+}
+sol_material phong_part(const sol_material& m) {
+    sol_material p = m;
+    p.type = SOL_BSDF_PHONG;
+    return p;
 }
 
 struct BsdfSample {
@@ -451,6 +469,8 @@ Color bsdf_eval(const sol_material& m, Vec3 in_dir, const SurfaceInfo& si, Vec3 
     switch (m.type) {
         case SOL_BSDF_DIFFUSE: return vec(m.kd) * kInvPi;
         case SOL_BSDF_PHONG: return phong_value(vec(m.ks), m.ns, reflect_cosine(in_dir, si.normal(), out_dir));
+        case SOL_BSDF_INTERP:
+            return lerp(bsdf_eval(diffuse_part(m), in_dir, si, out_dir), bsdf_eval(phong_part(m), in_dir, si, out_dir), m.mix);
         default: return {0, 0, 0};
     }
 }
@@ -459,6 +479,8 @@ float bsdf_pdf(const sol_material& m, Vec3 in_dir, const SurfaceInfo& si, Vec3 o
     switch (m.type) {
         case SOL_BSDF_DIFFUSE: return cosine_hemisphere_pdf(positive_dot(in_dir, si.normal()));
         case SOL_BSDF_PHONG: return cosine_power_hemisphere_pdf(m.ns_sample, reflect_cosine(in_dir, si.normal(), out_dir));
+        case SOL_BSDF_INTERP:
+            return lerp(bsdf_pdf(diffuse_part(m), in_dir, si, out_dir), bsdf_pdf(phong_part(m), in_dir, si, out_dir), m.mix);
         default: return 0.0f;
     }
 }
@@ -492,6 +514,18 @@ bool bsdf_sample(const sol_material& m, Rng& rng, const SurfaceInfo& si, Vec3 ou
         s.cos = positive_dot(s.in_dir, si.normal());
         s.color = phong_value(vec(m.ks), m.ns, reflect_cosine(s.in_dir, si.normal(), out_dir));
         return validate(si, s, false);
+    }
+    if (m.type == SOL_BSDF_INTERP) {
+        float k = m.mix;
+        sol_material target = phong_part(m), other = diffuse_part(m);
+        if (rng() > k) {
+            std::swap(target, other);
+            k = 1.0f - k;
+        }
+        if (!bsdf_sample(target, rng, si, out_dir, s)) return false;
+        s.pdf = lerp(bsdf_pdf(other, s.in_dir, si, out_dir), s.pdf, k);
+        s.color = lerp(bsdf_eval(other, s.in_dir, si, out_dir), s.color, k);
+        return true;
     }
     if (m.type == SOL_BSDF_GLASS) {
         float eta = si.is_front_side ? m.eta : 1.0f / m.eta;
@@ -873,11 +907,13 @@ int oracle_backward(const oracle_scene* s, const sol_pt_config* cfg, uint32_t w,
             auto add_material = [&](int mi, Color scaled_T, Color le_s, float rc) {
                 // scaled_T * df/dtheta * le_s, per parameter of material mi
                 const sol_material& m = s->materials[mi];
-                if (m.type == SOL_BSDF_DIFFUSE) {
-                    add3(size_t(mi) * 4, scaled_T * le_s * kInvPi);
-                } else if (m.type == SOL_BSDF_PHONG && rc > 0.0f) {
+                // the mix weighs the diffuse lobe by 1 - mix and the Phong lobe by mix
+                const bool mixed = m.type == SOL_BSDF_INTERP;
+                const float wd = mixed ? 1.0f - m.mix : 1.0f, wp = mixed ? m.mix : 1.0f;
+                if (m.type == SOL_BSDF_DIFFUSE || mixed) add3(size_t(mi) * 4, scaled_T * le_s * (kInvPi * wd));
+                if ((m.type == SOL_BSDF_PHONG || mixed) && rc > 0.0f) {
                     float dlog = fpow(rc, m.ns) * m.ns * ((m.ns + 2.0f) * std::log(rc) + 1.0f) * (0.5f * kInvPi);
-                    Color v = scaled_T * vec(m.ks) * le_s * dlog;
+                    Color v = scaled_T * vec(m.ks) * le_s * (dlog * wp);
                     g[size_t(mi) * 4 + 3] += double(delta.x) * v.x + double(delta.y) * v.y + double(delta.z) * v.z;
                 }
             };

@@ -8,16 +8,17 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
-BSDF_NONE, BSDF_DIFFUSE, BSDF_PHONG, BSDF_GLASS, BSDF_MIRROR = -1, 0, 1, 2, 3
+BSDF_NONE, BSDF_DIFFUSE, BSDF_PHONG, BSDF_GLASS, BSDF_MIRROR, BSDF_INTERP = -1, 0, 1, 2, 3, 4
 LIGHT_TRIANGLE, LIGHT_POINT = 0, 1
 PT_DISABLE_MIS, PT_DISABLE_NEE, PT_DISABLE_RR = 1, 2, 4
 RENDERER_MEGAKERNEL, RENDERER_WAVEFRONT = 0, 1
+SOL_E_INVALID, SOL_E_NO_LIGHTS, SOL_E_ALLOC, SOL_E_INTERNAL, SOL_E_LOAD = -1, -2, -3, -4, -5
 
 
 class Material(C.Structure):
     _fields_ = [("type", C.c_int32), ("kd", C.c_float * 3), ("ks", C.c_float * 3), ("kt", C.c_float * 3),
                 ("ns", C.c_float), ("eta", C.c_float), ("ns_sample", C.c_float), ("group", C.c_int32),
-                ("pad", C.c_float * 2)]
+                ("mix", C.c_float), ("pad", C.c_float)]
 
 
 class Light(C.Structure):
@@ -106,6 +107,11 @@ class SphereFieldParams(C.Structure):
                 ("radius_min", C.c_float), ("radius_max", C.c_float)]
 
 
+class SceneDefaults(C.Structure):
+    _fields_ = [("fov", C.c_float), ("aspect_ratio", C.c_float), ("eye", C.c_float * 3), ("dir", C.c_float * 3),
+                ("up", C.c_float * 3)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("num_nodes", C.c_uint32), ("nodes", C.POINTER(GpuNode)), ("num_tris", C.c_uint32),
                 ("tris", C.POINTER(GpuTri)), ("shade", C.POINTER(GpuShade)), ("num_materials", C.c_uint32),
@@ -186,6 +192,9 @@ _SIGS = {
         "sol_scene_sphere_field": (C.c_int, [C.POINTER(SphereFieldParams), C.POINTER(BvhConfig), C.POINTER(_P)]),
         "sol_scene_triangle_soup": (C.c_int, [C.c_uint32, C.c_uint64, C.POINTER(BvhConfig), C.POINTER(_P)]),
         "sol_scene_from_arrays": (C.c_int, [C.POINTER(MeshView), C.POINTER(BvhConfig), C.POINTER(_P)]),
+        "sol_default_scene_defaults": (None, [C.POINTER(SceneDefaults)]),
+        "sol_scene_load": (C.c_int, [C.c_char_p, C.POINTER(SceneDefaults), C.POINTER(BvhConfig), C.POINTER(_P)]),
+        "sol_scene_load_obj": (C.c_int, [C.c_char_p, C.POINTER(CameraParams), C.POINTER(BvhConfig), C.POINTER(_P)]),
         "sol_scene_destroy": (None, [_P]),
         "sol_scene_mesh": (C.c_int, [_P, C.POINTER(MeshView)]),
         "sol_scene_bvh": (C.c_int, [_P, C.POINTER(BvhView)]),

@@ -11,7 +11,8 @@
 //   continuation through the sampled f_j     dtheta_j += delta * R_j * (df/dtheta) / f
 // where R_j is the radiance the path gathers after vertex j: C minus every term added up to and
 // including vertex j's NEE. For the diffuse BSDF f = kd / pi, (df/dkd_c) / f_c = 1 / kd_c; for the
-// Phong BSDF f = ks rc^ns (ns + 2) / (2 pi), (df/dlog ns) / f = ns (ln rc + 1 / (ns + 2)).
+// Phong BSDF f = ks rc^ns (ns + 2) / (2 pi), (df/dlog ns) / f = ns (ln rc + 1 / (ns + 2)); for their
+// mix f = (1 - k) kd / pi + k Phong, each lobe's share of f times its own ratio.
 // Channels whose BSDF value is exactly zero (kd_c = 0, rc = 0) contribute nothing.
 //
 // Gradients are summed per workgroup in LDS (one float per parameter, LDS float atomics) and
@@ -50,13 +51,25 @@ struct GradSink {
     }
 
     // v = delta * (the radiance that carries the BSDF value f of material `mi` as a factor).
-    SOL_DEV void material(const sol_material& m, int mi, float rc, V3 v) const {
+    SOL_DEV void material(const sol_material& m, int mi, float rc, V3 f, V3 v) const {
         if (m.type == SOL_BSDF_DIFFUSE) {
             if (m.kd[0] > 0.0f) add(4 * mi + 0, v.x / m.kd[0]);
             if (m.kd[1] > 0.0f) add(4 * mi + 1, v.y / m.kd[1]);
             if (m.kd[2] > 0.0f) add(4 * mi + 2, v.z / m.kd[2]);
         } else if (m.type == SOL_BSDF_PHONG) {
             if (rc > 0.0f) add(4 * mi + 3, (v.x + v.y + v.z) * (m.ns * (logf(rc) + 1.0f / (m.ns + 2.0f))));
+        } else if (m.type == SOL_BSDF_INTERP) {
+            // f = (1 - mix) kd / pi + mix Phong: the lobes' shares of f, channel by channel
+            const float wd = (1.0f - m.mix) * kInvPi;
+            if (f.x > 0.0f) add(4 * mi + 0, v.x * (wd / f.x));
+            if (f.y > 0.0f) add(4 * mi + 1, v.y * (wd / f.y));
+            if (f.z > 0.0f) add(4 * mi + 2, v.z * (wd / f.z));
+            if (rc > 0.0f) {
+                V3 p = phong_eval(load3(m.ks), m.ns, rc) * m.mix;
+                float sp = (f.x > 0.0f ? v.x * (p.x / f.x) : 0.0f) + (f.y > 0.0f ? v.y * (p.y / f.y) : 0.0f) +
+                           (f.z > 0.0f ? v.z * (p.z / f.z) : 0.0f);
+                add(4 * mi + 3, sp * (m.ns * (logf(rc) + 1.0f / (m.ns + 2.0f))));
+            }
         }
     }
 };
@@ -95,7 +108,7 @@ SOL_DEV void replay_path(const SceneView& s, const sol_pt_config& cfg, const flo
                 HitRecord unused;
                 ++n_shadow;
                 if (!traverse<true, BLOCK, LDS>(s, lds_nodes, st, sr, cfg.ray_offset, 1.0f - cfg.ray_offset, unused)) {
-                    g.material(m, sf.mat, ns.rc, delta * ns.contrib);
+                    g.material(m, sf.mat, ns.rc, ns.f, delta * ns.contrib);
                     g.light(ns.light, delta * (T * ns.f * ns.scale));
                     R = R - ns.contrib;
                 }
@@ -107,7 +120,7 @@ SOL_DEV void replay_path(const SceneView& s, const sol_pt_config& cfg, const flo
         if (!bsdf_sample(m, sf, ortho_basis(sf.n), out, rng, bs)) break;
         // R is what the path gathers beyond this vertex; after the last bounce it is only the
         // rounding residue of the subtractions, so nothing is attributed there.
-        if (depth + 1 < cfg.max_path_len) g.material(m, sf.mat, bs.rc, delta * R);
+        if (depth + 1 < cfg.max_path_len) g.material(m, sf.mat, bs.rc, bs.color, delta * R);
         T = T * (bs.color * (bs.cos / (bs.pdf * q)));
         org = sf.point;
         dir = bs.in;

@@ -192,6 +192,16 @@ int solhip_reset_stats(solhip_ctx* ctx) {
     return SOLHIP_OK;
 }
 
+// Known BSDF kinds; a mix weight outside [0, 1] would make the lobe pick and the lerp meaningless.
+static const char* check_materials(const sol_material* m, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        if (m[i].type < SOL_BSDF_DIFFUSE || m[i].type > SOL_BSDF_INTERP) return "unknown material type";
+        if (m[i].type == SOL_BSDF_INTERP && !(m[i].mix >= 0.0f && m[i].mix <= 1.0f))
+            return "mixed material weight outside [0, 1]";
+    }
+    return nullptr;
+}
+
 int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scene** out) {
     if (!ctx || !d || !out) return fail(SOLHIP_E_INVALID, "null argument");
     *out = nullptr;
@@ -199,6 +209,7 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
         return fail(SOLHIP_E_INVALID, "scene without geometry");
     if (!d->num_lights || !d->lights) return fail(SOLHIP_E_NO_LIGHTS, "scene without light");
     if (d->num_materials && !d->materials) return fail(SOLHIP_E_INVALID, "null material table");
+    if (const char* e = check_materials(d->materials, d->num_materials)) return fail(SOLHIP_E_INVALID, e);
     if (d->num_tris >= (1u << 28)) return fail(SOLHIP_E_INVALID, "too many triangles");
     // traversal addresses nodes and triangles with 32-bit byte offsets
     if (static_cast<uint64_t>(d->num_tris) * sizeof(sol_gpu_tri) > 0xFFFFFFFFull ||
@@ -306,6 +317,7 @@ void solhip_scene_destroy(solhip_scene* s) {
 
 int solhip_scene_set_materials(solhip_scene* s, const sol_material* m, uint32_t n) {
     if (!s || !m || n != s->num_materials) return fail(SOLHIP_E_INVALID, "material count mismatch");
+    if (const char* e = check_materials(m, n)) return fail(SOLHIP_E_INVALID, e);
     HIP_TRY(hipSetDevice(s->ctx->device));
     HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     HIP_TRY(hipMemcpy(s->materials, m, sizeof(sol_material) * n, hipMemcpyHostToDevice));

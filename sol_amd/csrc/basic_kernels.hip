@@ -45,9 +45,11 @@ __global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol
 // coherent.
 // The megakernel traverses the binary layout: with whole paths in one kernel its registers bound
 // its occupancy, and the 4-wide traversal's extra registers made it slower (49.7 vs 58.6 ms per
-// 1080p 16-spp frame of the sphere field on MI355X).
+// 1080p 16-spp frame of the sphere field on MI355X). Three waves per SIMD need at most 168 VGPRs;
+// the compiler is held to that (the mixed diffuse-Phong BSDF took the unconstrained build to 170
+// VGPRs and two waves, 23 % slower).
 template <int BLOCK, int LDS>
-__global__ __launch_bounds__(BLOCK) void megakernel(SceneView s, sol_pt_config cfg, uint32_t w, uint32_t h,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void megakernel(SceneView s, sol_pt_config cfg, uint32_t w, uint32_t h,
                                                     uint64_t sample_index, uint32_t sample_count,
                                                     float* __restrict__ accum, unsigned long long* counter,
                                                     unsigned long long* stats, TraversalScratch sc) {

@@ -59,6 +59,9 @@ SOL_HD V3 load3(const float* p) { return V3{p[0], p[1], p[2]}; }
 SOL_HD V3 reflect(V3 d, V3 n) { return d - n * (2.0f * dot(d, n)); }
 // lerp(a, b, c, u, v) = a (1 - u - v) + b u + c v
 SOL_HD V3 lerp3(V3 a, V3 b, V3 c, float u, float v) { return a * ((1.0f - u) - v) + b * u + c * v; }
+// lerp(a, b, t) = a (1 - t) + b t, per channel for colours (color.h:66-71)
+SOL_HD float lerp1(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+SOL_HD V3 lerp2(V3 a, V3 b, float t) { return a * (1.0f - t) + b * t; }
 SOL_HD float luminance(V3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
 SOL_HD float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 SOL_HD float balance_heuristic(float x, float y) { return 1.0f / (1.0f + y / x); }

@@ -85,15 +85,28 @@ SOL_DEV V3 phong_eval(V3 ks, float ns, float rc) {
     return ks * pow_det(rc, ns) * (ns + 2.0f) * (0.5f * kInvPi);
 }
 
+// The two lobes, as DiffuseBsdf and PhongBsdf evaluate them (bsdfs.cpp:31-37, 64-70, 87-104).
+SOL_DEV V3 diffuse_eval(const sol_material& m) { return load3(m.kd) * kInvPi; }
+SOL_DEV float diffuse_pdf(V3 in, const Surface& sf) { return cosine_hemisphere_pdf(fmaxf(dot(in, sf.n), 0.0f)); }
+SOL_DEV V3 phong_lobe_eval(const sol_material& m, V3 in, const Surface& sf, V3 out) {
+    return phong_eval(load3(m.ks), m.ns, reflect_cosine(in, sf.n, out));
+}
+SOL_DEV float phong_pdf(const sol_material& m, V3 in, const Surface& sf, V3 out) {
+    return cosine_power_hemisphere_pdf(m.ns_sample, reflect_cosine(in, sf.n, out));
+}
+
+// InterpBsdf(diffuse, Phong, mix): eval and pdf are lerp(diffuse, Phong, mix) (bsdfs.cpp:185-212).
 SOL_DEV V3 bsdf_eval(const sol_material& m, V3 in, const Surface& sf, V3 out) {
-    if (m.type == SOL_BSDF_DIFFUSE) return load3(m.kd) * kInvPi;
-    if (m.type == SOL_BSDF_PHONG) return phong_eval(load3(m.ks), m.ns, reflect_cosine(in, sf.n, out));
+    if (m.type == SOL_BSDF_DIFFUSE) return diffuse_eval(m);
+    if (m.type == SOL_BSDF_PHONG) return phong_lobe_eval(m, in, sf, out);
+    if (m.type == SOL_BSDF_INTERP) return lerp2(diffuse_eval(m), phong_lobe_eval(m, in, sf, out), m.mix);
     return V3{0.0f, 0.0f, 0.0f};
 }
 
 SOL_DEV float bsdf_pdf(const sol_material& m, V3 in, const Surface& sf, V3 out) {
-    if (m.type == SOL_BSDF_DIFFUSE) return cosine_hemisphere_pdf(fmaxf(dot(in, sf.n), 0.0f));
-    if (m.type == SOL_BSDF_PHONG) return cosine_power_hemisphere_pdf(m.ns_sample, reflect_cosine(in, sf.n, out));
+    if (m.type == SOL_BSDF_DIFFUSE) return diffuse_pdf(in, sf);
+    if (m.type == SOL_BSDF_PHONG) return phong_pdf(m, in, sf, out);
+    if (m.type == SOL_BSDF_INTERP) return lerp1(diffuse_pdf(in, sf), phong_pdf(m, in, sf, out), m.mix);
     return 0.0f;
 }
 
@@ -102,7 +115,7 @@ struct BsdfSample {
     float pdf;
     float cos;
     V3 color;
-    float rc;    // Phong: reflect cosine of the sampled direction (for gradients)
+    float rc;    // Phong and mixed lobes: reflect cosine of the sampled direction (for gradients)
 };
 
 // validate_sample (bsdfs.h:89-94)
@@ -117,34 +130,57 @@ SOL_DEV float fresnel_factor(float eta, float cos_i, float cos_t) {
     return (rs * rs + rp * rp) * 0.5f;
 }
 
-// BSDF sampling (bsdfs.cpp:39-48, 72-85, 113-119, 137-164). Returns false when the sample is
-// rejected; the path then ends.
+SOL_DEV bool sample_diffuse_lobe(const sol_material& m, const Surface& sf, const Basis& basis, Pcg& rng,
+                                 BsdfSample& bs) {
+    float u = rng.next_float();
+    float v = rng.next_float();
+    float pdf;
+    V3 l = sample_cosine_hemisphere(u, v, pdf);
+    bs.in = basis.to_world(l);
+    bs.pdf = pdf;
+    bs.cos = l.z;
+    bs.color = diffuse_eval(m);
+    return valid_sample(sf, bs, false);
+}
+
+SOL_DEV bool sample_phong_lobe(const sol_material& m, const Surface& sf, V3 out, Pcg& rng, BsdfSample& bs) {
+    Basis lobe = ortho_basis(reflect(-out, sf.n));
+    float u = rng.next_float();
+    float v = rng.next_float();
+    float pdf;
+    V3 l = sample_cosine_power_hemisphere(m.ns_sample, u, v, pdf);
+    bs.in = lobe.to_world(l);
+    bs.pdf = pdf;
+    bs.cos = fmaxf(dot(bs.in, sf.n), 0.0f);
+    bs.rc = reflect_cosine(bs.in, sf.n, out);
+    bs.color = phong_eval(load3(m.ks), m.ns, bs.rc);
+    return valid_sample(sf, bs, false);
+}
+
+// BSDF sampling (bsdfs.cpp:39-48, 72-85, 113-119, 137-164, 192-205). Returns false when the
+// sample is rejected; the path then ends.
 SOL_DEV bool bsdf_sample(const sol_material& m, const Surface& sf, const Basis& basis, V3 out, Pcg& rng,
                          BsdfSample& bs) {
     bs.rc = 0.0f;
-    if (m.type == SOL_BSDF_DIFFUSE) {
-        float u = rng.next_float();
-        float v = rng.next_float();
-        float pdf;
-        V3 l = sample_cosine_hemisphere(u, v, pdf);
-        bs.in = basis.to_world(l);
-        bs.pdf = pdf;
-        bs.cos = l.z;
-        bs.color = load3(m.kd) * kInvPi;
-        return valid_sample(sf, bs, false);
-    }
-    if (m.type == SOL_BSDF_PHONG) {
-        Basis lobe = ortho_basis(reflect(-out, sf.n));
-        float u = rng.next_float();
-        float v = rng.next_float();
-        float pdf;
-        V3 l = sample_cosine_power_hemisphere(m.ns_sample, u, v, pdf);
-        bs.in = lobe.to_world(l);
-        bs.pdf = pdf;
-        bs.cos = fmaxf(dot(bs.in, sf.n), 0.0f);
-        bs.rc = reflect_cosine(bs.in, sf.n, out);
-        bs.color = phong_eval(load3(m.ks), m.ns, bs.rc);
-        return valid_sample(sf, bs, false);
+    if (m.type == SOL_BSDF_DIFFUSE) return sample_diffuse_lobe(m, sf, basis, rng, bs);
+    if (m.type == SOL_BSDF_PHONG) return sample_phong_lobe(m, sf, out, rng, bs);
+    if (m.type == SOL_BSDF_INTERP) {
+        // One draw picks the lobe: the Phong lobe (weight k) unless it exceeds k; the sampled
+        // lobe's value and density are then mixed with the other lobe's.
+        float k = m.mix;
+        bool phong = !(rng.next_float() > k);
+        if (!phong) k = 1.0f - k;
+        bool ok = phong ? sample_phong_lobe(m, sf, out, rng, bs) : sample_diffuse_lobe(m, sf, basis, rng, bs);
+        if (!ok) return false;
+        if (phong) {
+            bs.pdf = lerp1(diffuse_pdf(bs.in, sf), bs.pdf, k);
+            bs.color = lerp2(diffuse_eval(m), bs.color, k);
+        } else {
+            bs.rc = reflect_cosine(bs.in, sf.n, out);
+            bs.pdf = lerp1(cosine_power_hemisphere_pdf(m.ns_sample, bs.rc), bs.pdf, k);
+            bs.color = lerp2(phong_eval(load3(m.ks), m.ns, bs.rc), bs.color, k);
+        }
+        return true;
     }
     if (m.type == SOL_BSDF_GLASS) {
         float eta = sf.front ? m.eta : 1.0f / m.eta;
@@ -232,7 +268,7 @@ SOL_DEV bool nee_sample(const SceneView& s, uint32_t flags, const Surface& sf, c
     if (flags & SOL_PT_DISABLE_MIS) w = 1.0f;
     V3 f = bsdf_eval(m, in, sf, out);
     ns.f = f;
-    ns.rc = m.type == SOL_BSDF_PHONG ? reflect_cosine(in, sf.n, out) : 0.0f;
+    ns.rc = (m.type == SOL_BSDF_PHONG || m.type == SOL_BSDF_INTERP) ? reflect_cosine(in, sf.n, out) : 0.0f;
     ns.scale = G * cos_surf * w / pdf_L;
     ns.contrib = Le * T * f * ns.scale;
     return true;

@@ -206,8 +206,12 @@ SOL_DEV const SceneView& scene_of(const SceneView& single, const WaveTables& t, 
     return tmp;
 }
 
+// Diffuse / glossy (Phong, and the diffuse-Phong mix) / specular: the queues follow the BSDF's
+// shading work, the mixed BSDF shares the Phong queue (it runs both lobes).
 SOL_DEV int queue_of(int type) {
-    return type == SOL_BSDF_DIFFUSE ? kQueueDiffuse : (type == SOL_BSDF_PHONG ? kQueuePhong : kQueueSpecular);
+    if (type == SOL_BSDF_DIFFUSE) return kQueueDiffuse;
+    if (type == SOL_BSDF_PHONG || type == SOL_BSDF_INTERP) return kQueuePhong;
+    return kQueueSpecular;
 }
 
 // ---- work items ------------------------------------------------------------------------------------
@@ -561,7 +565,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
             V3 out = -xyz(r3);
             sol_material m = s.materials[sf.mat];
             if (QUEUE == kQueueDiffuse) m.type = SOL_BSDF_DIFFUSE;
-            if (QUEUE == kQueuePhong) m.type = SOL_BSDF_PHONG;
+            if (QUEUE == kQueuePhong && m.type != SOL_BSDF_INTERP) m.type = SOL_BSDF_PHONG;
+            if (QUEUE == kQueueSpecular && m.type != SOL_BSDF_GLASS) m.type = SOL_BSDF_MIRROR;
             float4 p0 = b.path[slot * 4 + 0], p1 = b.path[slot * 4 + 1], p2 = b.path[slot * 4 + 2];
             item = item_of(p2);
             V3 T = xyz(p0);

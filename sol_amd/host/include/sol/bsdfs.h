@@ -11,7 +11,7 @@ namespace sol {
 
 class Bsdf {
 public:
-    enum class Tag { DiffuseBsdf, PhongBsdf, MirrorBsdf, GlassBsdf };
+    enum class Tag { DiffuseBsdf, PhongBsdf, MirrorBsdf, GlassBsdf, InterpBsdf };
     // Classification used to skip next-event estimation on specular surfaces
     // (bsdfs.h:39-43, path_tracer.cpp:66 of the reference).
     enum class Type { Diffuse = 0, Glossy = 1, Specular = 2 };
@@ -62,6 +62,28 @@ private:
     const ColorTexture& ks_;
     const ColorTexture& kt_;
     const Texture& eta_;
+};
+
+/// Mix of two BSDFs with weight k for b (reference bsdfs.cpp:179-230). Its type is Diffuse if
+/// either side is, else Glossy if either side is, else Specular. On the GPU path a must be a
+/// DiffuseBsdf and b a PhongBsdf - the mix the OBJ loader builds for materials with both a
+/// diffuse and a specular colour - and flatten() rejects other pairs.
+class InterpBsdf final : public Bsdf {
+public:
+    InterpBsdf(const Bsdf* a, const Bsdf* b, const Texture& k)
+        : Bsdf(Tag::InterpBsdf, infer_type(a->type, b->type)), a_(a), b_(b), k_(k) {}
+    sol_material flatten() const override;
+    const Bsdf* a() const { return a_; }
+    const Bsdf* b() const { return b_; }
+private:
+    static Type infer_type(Type a, Type b) {
+        if (a == Type::Diffuse || b == Type::Diffuse) return Type::Diffuse;
+        if (a == Type::Glossy || b == Type::Glossy) return Type::Glossy;
+        return Type::Specular;
+    }
+    const Bsdf* a_;
+    const Bsdf* b_;
+    const Texture& k_;
 };
 
 } // namespace sol

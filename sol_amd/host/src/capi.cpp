@@ -7,6 +7,7 @@
 #include <string>
 
 #include "sol_host.h"
+#include "sol/scene_loader.h"
 #include "sol/scenes.h"
 #include "sol/samplers.h"
 #include "sol/triangle_mesh.h"
@@ -20,6 +21,11 @@ struct sol_scene {
 namespace {
 
 thread_local std::string g_error;
+
+// A file that cannot be opened or parsed (reported as SOL_E_LOAD).
+struct LoadFailure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 int fail(int code, const std::string& msg) {
     g_error = msg;
@@ -68,6 +74,8 @@ int make_scene(const sol_bvh_config* cfg, sol_scene** out, F&& make) {
         return SOL_OK;
     } catch (const std::bad_alloc&) {
         return fail(SOL_E_ALLOC, "out of host memory");
+    } catch (const LoadFailure& e) {
+        return fail(SOL_E_LOAD, e.what());
     } catch (const std::invalid_argument& e) {
         std::string m = e.what();
         return fail(m.find("no light") != std::string::npos ? SOL_E_NO_LIGHTS : SOL_E_INVALID, m);
@@ -129,6 +137,61 @@ int sol_scene_triangle_soup(uint32_t num_tris, uint64_t seed, const sol_bvh_conf
     return make_scene(cfg, out, [&](const sol::bvh::BuildConfig& b) { return sol::scenes::triangle_soup(num_tris, seed, b); });
 }
 
+void sol_default_scene_defaults(sol_scene_defaults* out) {
+    if (!out) return;
+    sol::SceneDefaults d;
+    out->fov = d.fov;
+    out->aspect_ratio = d.aspect_ratio;
+    for (int i = 0; i < 3; ++i) {
+        out->eye[i] = d.eye_pos[i];
+        out->dir[i] = d.dir_vector[i];
+        out->up[i] = d.up_vector[i];
+    }
+}
+
+int sol_scene_load(const char* file_name, const sol_scene_defaults* defaults, const sol_bvh_config* cfg,
+                   sol_scene** out) {
+    if (!file_name) return fail(SOL_E_INVALID, "null file name");
+    sol_scene_defaults dd;
+    sol_default_scene_defaults(&dd);
+    if (!defaults) defaults = &dd;
+    return make_scene(cfg, out, [&](const sol::bvh::BuildConfig& b) {
+        sol::SceneDefaults d;
+        d.fov = defaults->fov;
+        d.aspect_ratio = defaults->aspect_ratio;
+        d.eye_pos = sol::Vec3f(defaults->eye[0], defaults->eye[1], defaults->eye[2]);
+        d.dir_vector = sol::Vec3f(defaults->dir[0], defaults->dir[1], defaults->dir[2]);
+        d.up_vector = sol::Vec3f(defaults->up[0], defaults->up[1], defaults->up[2]);
+        sol::Scene s;
+        try {
+            sol::SceneLoader loader(s, d, b);
+            loader.load(file_name);
+        } catch (const std::runtime_error& e) {
+            throw LoadFailure(e.what());
+        }
+        return s;
+    });
+}
+
+int sol_scene_load_obj(const char* file_name, const sol_camera_params* camera, const sol_bvh_config* cfg,
+                       sol_scene** out) {
+    if (!file_name || !camera) return fail(SOL_E_INVALID, "null argument");
+    return make_scene(cfg, out, [&](const sol::bvh::BuildConfig& b) {
+        sol::Scene s;
+        try {
+            sol::SceneLoader loader(s, sol::SceneDefaults{}, b);
+            s.root = sol::obj::load(loader, file_name);
+        } catch (const std::runtime_error& e) {
+            throw LoadFailure(e.what());
+        }
+        const sol_camera_params& c = *camera;
+        s.camera = std::make_unique<sol::PerspectiveCamera>(sol::Vec3f(c.eye[0], c.eye[1], c.eye[2]),
+                                                            sol::Vec3f(c.dir[0], c.dir[1], c.dir[2]),
+                                                            sol::Vec3f(c.up[0], c.up[1], c.up[2]), c.fov, c.aspect);
+        return s;
+    });
+}
+
 int sol_scene_from_arrays(const sol_mesh_view* mv, const sol_bvh_config* cfg, sol_scene** out) {
     if (!mv || !mv->positions || !mv->normals || !mv->indices || !mv->tri_material || !mv->tri_light ||
         (mv->num_materials && !mv->materials) || (mv->num_lights && !mv->lights))
@@ -145,6 +208,7 @@ int sol_scene_from_arrays(const sol_mesh_view* mv, const sol_bvh_config* cfg, so
             s.textures.push_back(std::make_unique<sol::ConstantTexture>(v));
             return s.textures.back().get();
         };
+        std::vector<std::unique_ptr<sol::Bsdf>> parts;
         for (uint32_t i = 0; i < mv->num_materials; ++i) {
             const sol_material& m = mv->materials[i];
             std::unique_ptr<sol::Bsdf> bsdf;
@@ -153,11 +217,20 @@ int sol_scene_from_arrays(const sol_mesh_view* mv, const sol_bvh_config* cfg, so
                 case SOL_BSDF_PHONG: bsdf = std::make_unique<sol::PhongBsdf>(*color(m.ks), *scalar(m.ns)); break;
                 case SOL_BSDF_GLASS: bsdf = std::make_unique<sol::GlassBsdf>(*color(m.ks), *color(m.kt), *scalar(m.eta)); break;
                 case SOL_BSDF_MIRROR: bsdf = std::make_unique<sol::MirrorBsdf>(*color(m.ks)); break;
+                case SOL_BSDF_INTERP: {
+                    // the two mixed BSDFs join the table after the caller's materials
+                    parts.push_back(std::make_unique<sol::DiffuseBsdf>(*color(m.kd)));
+                    parts.push_back(std::make_unique<sol::PhongBsdf>(*color(m.ks), *scalar(m.ns)));
+                    bsdf = std::make_unique<sol::InterpBsdf>(parts[parts.size() - 2].get(), parts.back().get(),
+                                                             *scalar(m.mix));
+                    break;
+                }
                 default: throw std::invalid_argument("unknown material type");
             }
             bsdf->group = m.group;
             s.bsdfs.push_back(std::move(bsdf));
         }
+        for (auto& p : parts) s.bsdfs.push_back(std::move(p));
         for (uint32_t i = 0; i < mv->num_lights; ++i) {
             const sol_light& l = mv->lights[i];
             std::unique_ptr<sol::Light> light;

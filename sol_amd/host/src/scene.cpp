@@ -55,6 +55,21 @@ sol_material GlassBsdf::flatten() const {
     return m;
 }
 
+sol_material InterpBsdf::flatten() const {
+    if (!a_ || !b_ || a_->tag != Tag::DiffuseBsdf || b_->tag != Tag::PhongBsdf)
+        throw std::invalid_argument("InterpBsdf: the GPU path mixes a DiffuseBsdf (a) and a PhongBsdf (b) only");
+    sol_material d = a_->flatten(), p = b_->flatten();
+    sol_material m = blank_material(SOL_BSDF_INTERP);
+    for (int c = 0; c < 3; ++c) {
+        m.kd[c] = d.kd[c];
+        m.ks[c] = p.ks[c];
+    }
+    m.ns = p.ns;
+    m.ns_sample = p.ns_sample;
+    m.mix = k_.sample(Vec2f(0, 0));
+    return m;
+}
+
 // -- Lights -------------------------------------------------------------------------------------
 
 sol_light PointLight::flatten() const {

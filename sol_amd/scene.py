@@ -1,6 +1,7 @@
 """Host-side scenes: thin Python handles over libsol_host.so (C++20 scene model, BVH builder,
 flattening). A Scene owns the C++ sol::Scene and its flat arrays."""
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -57,6 +58,40 @@ class Scene:
         h = C.c_void_p()
         F.check_host(F.host.sol_scene_triangle_soup(num_tris, seed, C.byref(bvh or default_bvh_config()), C.byref(h)),
                      "sol_scene_triangle_soup")
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, path, width=None, height=None, aspect=None, bvh=None, **defaults):
+        """A TOML scene file (sol::Scene::load). As sol's driver does, the default aspect ratio is
+        width / height; other defaults (fov, eye, dir, up) may be overridden by keyword."""
+        d = F.SceneDefaults()
+        F.host.sol_default_scene_defaults(C.byref(d))
+        if aspect is None and width and height:
+            aspect = float(width) / float(height)
+        if aspect is not None:
+            d.aspect_ratio = aspect
+        for k, v in defaults.items():
+            if k in ("eye", "dir", "up"):
+                getattr(d, k)[:] = [float(x) for x in v]
+            else:
+                setattr(d, k, v)
+        h = C.c_void_p()
+        F.check_host(F.host.sol_scene_load(os.fsencode(path), C.byref(d), C.byref(bvh or default_bvh_config()),
+                                           C.byref(h)), "sol_scene_load")
+        return cls(h.value)
+
+    @classmethod
+    def load_obj(cls, path, eye, dir, up, fov, aspect, bvh=None):
+        """One OBJ file (with its MTL libraries) as the whole scene, seen through the given camera."""
+        cam = F.CameraParams()
+        cam.eye[:] = [float(x) for x in eye]
+        cam.dir[:] = [float(x) for x in dir]
+        cam.up[:] = [float(x) for x in up]
+        cam.fov = fov
+        cam.aspect = aspect
+        h = C.c_void_p()
+        F.check_host(F.host.sol_scene_load_obj(os.fsencode(path), C.byref(cam), C.byref(bvh or default_bvh_config()),
+                                               C.byref(h)), "sol_scene_load_obj")
         return cls(h.value)
 
     @classmethod

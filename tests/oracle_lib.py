@@ -50,16 +50,81 @@ def f32(*v):
     return np.array(v, np.float32)
 
 
+def material_struct(rec):
+    """sol_material of an oracle loader material record (oracle/scene_files.py)."""
+    m = F.Material()
+    m.type = rec["type"]
+    m.group = -1
+    for key in ("kd", "ks", "kt"):
+        if key in rec:
+            getattr(m, key)[:] = [float(x) for x in rec[key]]
+    if "ns" in rec:
+        m.ns = m.ns_sample = float(rec["ns"])
+    if "eta" in rec:
+        m.eta = float(rec["eta"])
+    if "mix" in rec:
+        m.mix = float(rec["mix"])
+    return m
+
+
+def light_struct(rec):
+    """sol_light of an oracle loader light record."""
+    lt = F.Light()
+    lt.type = F.LIGHT_TRIANGLE
+    for name, v in zip(("v0", "v1", "v2"), rec["v"]):
+        getattr(lt, name)[:] = [float(x) for x in v]
+    lt.normal[:] = [float(x) for x in rec["normal"]]
+    lt.inv_area = float(rec["inv_area"])
+    lt.intensity[:] = [float(x) for x in rec["intensity"]]
+    lt.group = -1
+    return lt
+
+
 class OracleScene:
-    """The oracle's copy of a sol_amd.Scene; own BVH unless use_product_bvh."""
+    """The oracle's copy of a sol_amd.Scene (own BVH unless use_product_bvh), or of the tables of
+    the oracle's scene-file loader (from_tables)."""
 
     def __init__(self, scene, use_product_bvh=False):
         mv = scene.mesh_view()
         bv = scene.bvh_view() if use_product_bvh else None
+        self._create(mv, bv)
+
+    def _create(self, mv, bv=None):
         h = C.c_void_p()
         if lib.oracle_scene_create(C.byref(mv), C.byref(bv) if bv is not None else None, C.byref(h)) != 0:
             raise RuntimeError("oracle_scene_create failed")
         self._h = h
+
+    @classmethod
+    def from_tables(cls, t):
+        """t: the dict oracle.scene_files.load_scene returns (camera included)."""
+        self = cls.__new__(cls)
+        pos = np.ascontiguousarray(t["positions"], np.float32)
+        nrm = np.ascontiguousarray(t["normals"], np.float32)
+        idx = np.ascontiguousarray(t["indices"], np.uint32)
+        tm = np.ascontiguousarray(t["tri_material"], np.int32)
+        tl = np.ascontiguousarray(t["tri_light"], np.int32)
+        mats = (F.Material * max(1, len(t["materials"])))(*[material_struct(r) for r in t["materials"]])
+        lts = (F.Light * max(1, len(t["lights"])))(*[light_struct(r) for r in t["lights"]])
+        mv = F.MeshView()
+        mv.num_vertices = pos.shape[0]
+        mv.positions = pos.ctypes.data_as(C.POINTER(C.c_float))
+        mv.normals = nrm.ctypes.data_as(C.POINTER(C.c_float))
+        mv.num_tris = idx.shape[0]
+        mv.indices = idx.ctypes.data_as(C.POINTER(C.c_uint32))
+        mv.tri_material = tm.ctypes.data_as(C.POINTER(C.c_int32))
+        mv.tri_light = tl.ctypes.data_as(C.POINTER(C.c_int32))
+        mv.num_materials = len(t["materials"])
+        mv.materials = mats
+        mv.num_lights = len(t["lights"])
+        mv.lights = lts
+        cam = t["camera"]
+        for key in ("eye", "dir", "up"):
+            getattr(mv.camera, key)[:] = [float(x) for x in cam[key]]
+        mv.camera.fov = float(cam["fov"])
+        mv.camera.aspect = float(cam["aspect"])
+        self._create(mv)
+        return self
 
     def __del__(self):
         if getattr(self, "_h", None) and self._h.value:

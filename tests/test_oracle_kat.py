@@ -237,3 +237,43 @@ def test_phong_sample_is_energy_normalised():
     assert ok and abs(rgb[0] - (ns + 2) / (2 * math.pi) * (d[2] ** ns)) < 1e-3 * rgb[0]
     # The estimator f cos / pdf for normal incidence equals ((n+2)/(n+1)) c for c = cos theta.
     assert abs(rgb[0] * cos / pdf - (ns + 2) / (ns + 1) * d[2]) < 1e-4
+
+
+# ---- InterpBsdf: mix of a diffuse lobe (weight 1 - k) and a Phong lobe (weight k) --------------
+
+def _interp(k, kd=(0.2, 0.4, 0.8), ks=(0.9, 0.7, 0.5), ns=12.0):
+    return _mat(F.BSDF_INTERP, kd=kd, ks=ks, ns=ns, ns_sample=ns, mix=k)
+
+
+@pytest.mark.parametrize("u,v", [(0.3, 0.7), (0.81, 0.05), (0.5, 0.5)])
+def test_interp_with_weight_zero_or_one_is_the_single_lobe(u, v):
+    n, out = (0.0, 0.6, 0.8), np.float32([0.0, 0.0, 1.0])
+    # weight 0: the pick draw (any value > 0) selects the diffuse lobe, whose sample is unchanged
+    ok, d, pdf, cos, rgb = O.bsdf_sample(_interp(0.0), n, out, [0.5, u, v])
+    ok1, d1, pdf1, cos1, rgb1 = O.bsdf_sample(_mat(F.BSDF_DIFFUSE, kd=(0.2, 0.4, 0.8)), n, out, [u, v])
+    assert ok == ok1 and np.array_equal(d, d1) and pdf == pdf1 and cos == cos1 and np.array_equal(rgb, rgb1)
+    # weight 1: every pick draw (< 1) selects the Phong lobe
+    ok, d, pdf, cos, rgb = O.bsdf_sample(_interp(1.0), n, out, [0.999, u, v])
+    ok1, d1, pdf1, cos1, rgb1 = O.bsdf_sample(_mat(F.BSDF_PHONG, ks=(0.9, 0.7, 0.5), ns=12.0, ns_sample=12.0),
+                                              n, out, [u, v])
+    assert ok == ok1 and np.array_equal(d, d1) and pdf == pdf1 and cos == cos1 and np.array_equal(rgb, rgb1)
+
+
+def test_interp_pick_threshold_and_mixture_density():
+    """A pick draw at or below k samples the Phong lobe, above k the diffuse lobe; either way the
+    density is (1 - k) cos / pi + k p_phong and the value (1 - k) kd / pi + k f_phong."""
+    k, ns = 0.3, 12.0
+    n, out = np.float32([0.0, 0.0, 1.0]), np.float32([0.6, 0.0, 0.8])
+    kd, ks = np.float32([0.2, 0.4, 0.8]), np.float32([0.9, 0.7, 0.5])
+    refl = np.float32([-0.6, 0.0, 0.8])   # mirror direction of `out` about n
+    for r, lobe in ((0.3, F.BSDF_PHONG), (0.2999, F.BSDF_PHONG), (0.3001, F.BSDF_DIFFUSE), (0.9, F.BSDF_DIFFUSE)):
+        ok, d, pdf, cos, rgb = O.bsdf_sample(_interp(k, ns=ns), n, out, [r, 0.4, 0.6])
+        single = _mat(lobe, kd=tuple(kd), ks=tuple(ks), ns=ns, ns_sample=ns)
+        ok1, d1, _, cos1, _ = O.bsdf_sample(single, n, out, [0.4, 0.6])
+        assert ok and ok1 and np.array_equal(d, d1) and cos == cos1, (r, lobe)
+        c_n = float(np.dot(d, n))
+        c_r = max(float(np.dot(d, refl)), 0.0)
+        p_phong = (ns + 1) / (2 * math.pi) * c_r ** ns
+        f_phong = ks * (ns + 2) / (2 * math.pi) * c_r ** ns
+        assert abs(pdf - ((1 - k) * c_n / math.pi + k * p_phong)) < 1e-5 * max(pdf, 1)
+        np.testing.assert_allclose(rgb, (1 - k) * kd / math.pi + k * f_phong, rtol=1e-5, atol=1e-7)
