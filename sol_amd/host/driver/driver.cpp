@@ -1,17 +1,20 @@
 // sol_render: command-line renderer over the C++ API, following the flow of sol's driver
 // (reference test/driver.cpp:196-245): build a scene, make the renderer, run a RenderJob, scale
-// the accumulated image by 1 / spp and save it. Scenes are the procedural ones (sol's driver
-// loads TOML/OBJ files, whose parsers are not part of this path); images are written as PFM.
+// the accumulated image by 1 / spp and save it. The scene is a TOML scene file, loaded as sol's
+// driver loads it (Scene::load with the aspect ratio width / height as default), or one of the
+// procedural scenes; images are written as PFM.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <sstream>
 #include <string>
 
 #include "sol/algorithms/gpu_path_tracer.h"
 #include "sol/image.h"
 #include "sol/render_job.h"
+#include "sol/scene_loader.h"
 #include "sol/scenes.h"
 
 namespace {
@@ -19,7 +22,7 @@ namespace {
 void usage() {
     std::fprintf(stderr,
                  "usage: sol_render [options]\n"
-                 "  --scene cornell|spheres     procedural scene (default cornell)\n"
+                 "  --scene FILE.toml|cornell|spheres  scene file or procedural scene (default cornell)\n"
                  "  --width W --height H        image size (default 256 x 256)\n"
                  "  --spp N                     samples per pixel (default 16)\n"
                  "  --spf N                     samples per frame of the render job (default spp)\n"
@@ -78,11 +81,20 @@ int main(int argc, char** argv) {
         const float aspect = static_cast<float>(width) / static_cast<float>(height);
         sol::bvh::BuildConfig build;
         auto t0 = std::chrono::steady_clock::now();
-        if (scene_name != "cornell" && scene_name != "spheres") {
+        const bool from_file = scene_name.size() > 5 && scene_name.ends_with(".toml");
+        if (!from_file && scene_name != "cornell" && scene_name != "spheres") {
             usage();
             return 1;
         }
         sol::Scene scene = [&] {
+            if (from_file) {
+                sol::SceneDefaults defaults;
+                defaults.aspect_ratio = aspect;
+                std::ostringstream err;
+                auto loaded = sol::load_scene(scene_name, defaults, build, &err);
+                if (!loaded) throw std::runtime_error("cannot load scene: " + err.str());
+                return std::move(*loaded);
+            }
             if (scene_name == "cornell") return sol::scenes::cornell_box(aspect, build);
             sol_sphere_field_params p;
             sol_default_sphere_field_params(&p);

@@ -1,6 +1,6 @@
 """The C++ path end to end: sol_render (sol::RenderJob driving sol::GpuPathTracer over the C++
-scene model, the flow of sol's driver) against the same samples rendered through the C ABI from
-Python, and the PFM it writes."""
+scene model, the flow of sol's driver, on a procedural scene or a TOML scene file) against the
+same samples rendered through the C ABI from Python, and the PFM it writes."""
 import os
 import subprocess
 
@@ -25,13 +25,17 @@ def read_pfm(path):
     return np.ascontiguousarray(rows.transpose(2, 0, 1))
 
 
-@pytest.mark.parametrize("renderer", ["wavefront", "megakernel"])
-def test_render_job_driver_matches_c_abi_render(tmp_path, renderer):
+GLOSSY = os.path.join(ROOT, "tests", "golden", "cornell", "cornell_box_glossy.toml")
+
+
+@pytest.mark.parametrize("renderer,scene_arg", [("wavefront", "cornell"), ("megakernel", "cornell"),
+                                                ("wavefront", GLOSSY)])
+def test_render_job_driver_matches_c_abi_render(tmp_path, renderer, scene_arg):
     if not os.path.exists(DRIVER):
         pytest.fail(f"{DRIVER} is not built (make -C sol_amd)")
     w, h, spp, spf = 40, 30, 7, 3
     out = tmp_path / "out.pfm"
-    r = subprocess.run([DRIVER, "--scene", "cornell", "--width", str(w), "--height", str(h), "--spp", str(spp),
+    r = subprocess.run([DRIVER, "--scene", scene_arg, "--width", str(w), "--height", str(h), "--spp", str(spp),
                         "--spf", str(spf), "--max-path-len", "8", "--renderer", renderer, "-o", str(out)],
                        capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
@@ -39,7 +43,7 @@ def test_render_job_driver_matches_c_abi_render(tmp_path, renderer):
     assert "3 frames, 7 samples per pixel" in r.stdout
     got = read_pfm(out)
     # RenderJob renders frames of samples [0, 3), [3, 6), [6, 7) into one image, then scales it
-    scene = sol_amd.Scene.cornell_box(aspect=w / h)
+    scene = sol_amd.Scene.cornell_box(aspect=w / h) if scene_arg == "cornell" else sol_amd.Scene.load(GLOSSY, w, h)
     pt = sol_amd.GpuPathTracer(scene, sol_amd.PathTracerConfig(max_path_len=8), kind=renderer)
     want = sol_amd.new_image(w, h)
     for first in range(0, spp, spf):
