@@ -161,6 +161,17 @@ int sol_scene_build_stats(const sol_scene* scene, sol_build_stats* out);
 int sol_scene_set_materials(sol_scene* scene, const sol_material* m, uint32_t n);
 int sol_scene_set_lights(sol_scene* scene, const sol_light* l, uint32_t n);
 
+/* Image files: sol::Image::save (reference src/image.cpp:26-37). format: 0 = auto (EXR, sol's
+   default), 4 = EXR (uncompressed FLOAT scanlines, channels B, G, R as src/formats/exr.cpp:48-100
+   writes them), 5 = PFM; 1-3 (PNG, JPEG, TIFF) have no codec in this build and fail. pixels is
+   planar fp32, channel by channel, rows top to bottom. */
+enum {
+    SOL_IMAGE_AUTO = 0, SOL_IMAGE_PNG = 1, SOL_IMAGE_JPEG = 2, SOL_IMAGE_TIFF = 3, SOL_IMAGE_EXR = 4,
+    SOL_IMAGE_PFM = 5
+};
+int sol_image_save(const char* path, int format, const float* pixels, uint32_t width, uint32_t height,
+                   uint32_t channels);
+
 /* Computes the camera basis from its parameters (cameras.cpp:5-21). */
 void sol_camera_from_params(const sol_camera_params* p, sol_camera* out);
 

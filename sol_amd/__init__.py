@@ -6,4 +6,5 @@ Product libraries (built in-tree by `make -C sol_amd`):
 """
 from . import _ffi  # noqa: F401  (raises ImportError if the libraries are not built)
 from .gpu import Context, GpuPathTracer, GpuScene, PathTracerConfig, new_image  # noqa: F401
-from .scene import Scene, camera_basis, default_bvh_config, default_sphere_field_params, gen_rays  # noqa: F401
+from .scene import (Scene, camera_basis, default_bvh_config, default_sphere_field_params, gen_rays,  # noqa: F401
+                    save_image)

@@ -13,6 +13,7 @@ LIGHT_TRIANGLE, LIGHT_POINT = 0, 1
 PT_DISABLE_MIS, PT_DISABLE_NEE, PT_DISABLE_RR = 1, 2, 4
 RENDERER_MEGAKERNEL, RENDERER_WAVEFRONT = 0, 1
 SOL_E_INVALID, SOL_E_NO_LIGHTS, SOL_E_ALLOC, SOL_E_INTERNAL, SOL_E_LOAD = -1, -2, -3, -4, -5
+IMAGE_AUTO, IMAGE_PNG, IMAGE_JPEG, IMAGE_TIFF, IMAGE_EXR, IMAGE_PFM = 0, 1, 2, 3, 4, 5
 
 
 class Material(C.Structure):
@@ -193,6 +194,7 @@ _SIGS = {
         "sol_scene_triangle_soup": (C.c_int, [C.c_uint32, C.c_uint64, C.POINTER(BvhConfig), C.POINTER(_P)]),
         "sol_scene_from_arrays": (C.c_int, [C.POINTER(MeshView), C.POINTER(BvhConfig), C.POINTER(_P)]),
         "sol_default_scene_defaults": (None, [C.POINTER(SceneDefaults)]),
+        "sol_image_save": (C.c_int, [C.c_char_p, C.c_int, _P, C.c_uint32, C.c_uint32, C.c_uint32]),
         "sol_scene_load": (C.c_int, [C.c_char_p, C.POINTER(SceneDefaults), C.POINTER(BvhConfig), C.POINTER(_P)]),
         "sol_scene_load_obj": (C.c_int, [C.c_char_p, C.POINTER(CameraParams), C.POINTER(BvhConfig), C.POINTER(_P)]),
         "sol_scene_destroy": (None, [_P]),

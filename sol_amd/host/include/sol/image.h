@@ -1,6 +1,8 @@
-// Planar fp32 image, as sol::Image (reference include/sol/image.h:18-98, src/image.cpp:13-23).
-// Pixel (0, 0) is the top-left corner. Only the in-memory operations are on the path; a PFM
-// writer stands in for the EXR/PNG codecs, whose libraries are absent.
+// Planar fp32 image, as sol::Image (reference include/sol/image.h:18-98, src/image.cpp:13-41).
+// Pixel (0, 0) is the top-left corner. Files: OpenEXR (sol's default format, written as sol's
+// exr::save writes it: uncompressed 32-bit float scanlines, channels B, G, R) and PFM; PNG, JPEG
+// and TIFF need codec libraries this build does not have, so saving in them fails, as sol's does
+// when it is built without them.
 #ifndef SOL_IMAGE_H
 #define SOL_IMAGE_H
 
@@ -43,8 +45,15 @@ struct Image {
     Channel& channel(size_t i) { return channels_[i]; }
     const Channel& channel(size_t i) const { return channels_[i]; }
 
+    enum class Format { Auto, Png, Jpeg, Tiff, Exr, Pfm };
+
     void clear(float value = 0.0f);
     void scale(float value);
+    /// Image::save (image.cpp:26-37): Auto means EXR. False if the file cannot be written or the
+    /// format has no writer here (PNG, JPEG, TIFF).
+    bool save(std::string_view path, Format format = Format::Auto) const;
+    // OpenEXR, single part, scanline, uncompressed, FLOAT channels (exr.cpp:48-100).
+    bool save_exr(std::string_view path) const;
     // Portable float map (RGB, bottom-to-top rows as the format requires).
     bool save_pfm(std::string_view path) const;
 

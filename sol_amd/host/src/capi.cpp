@@ -7,6 +7,7 @@
 #include <string>
 
 #include "sol_host.h"
+#include "sol/image.h"
 #include "sol/scene_loader.h"
 #include "sol/scenes.h"
 #include "sol/samplers.h"
@@ -135,6 +136,23 @@ int sol_scene_sphere_field(const sol_sphere_field_params* p, const sol_bvh_confi
 
 int sol_scene_triangle_soup(uint32_t num_tris, uint64_t seed, const sol_bvh_config* cfg, sol_scene** out) {
     return make_scene(cfg, out, [&](const sol::bvh::BuildConfig& b) { return sol::scenes::triangle_soup(num_tris, seed, b); });
+}
+
+int sol_image_save(const char* path, int format, const float* pixels, uint32_t width, uint32_t height,
+                   uint32_t channels) {
+    if (!path || !pixels || !width || !height || !channels) return fail(SOL_E_INVALID, "null or empty image");
+    if (format < SOL_IMAGE_AUTO || format > SOL_IMAGE_PFM) return fail(SOL_E_INVALID, "unknown image format");
+    if (format == SOL_IMAGE_PFM && channels != 3) return fail(SOL_E_INVALID, "PFM images have three channels");
+    try {
+        sol::Image img(width, height, channels);
+        const size_t plane = size_t(width) * height;
+        for (uint32_t c = 0; c < channels; ++c) std::memcpy(img.channel(c).get(), pixels + c * plane, plane * sizeof(float));
+        if (!img.save(path, static_cast<sol::Image::Format>(format)))
+            return fail(SOL_E_LOAD, std::string("cannot save image '") + path + "' in this format");
+        return SOL_OK;
+    } catch (const std::bad_alloc&) {
+        return fail(SOL_E_ALLOC, "out of host memory");
+    }
 }
 
 void sol_default_scene_defaults(sol_scene_defaults* out) {

@@ -209,3 +209,17 @@ def camera_basis(params):
     cam = F.Camera()
     F.host.sol_camera_from_params(C.byref(params), C.byref(cam))
     return cam
+
+
+_IMAGE_FORMATS = {"auto": F.IMAGE_AUTO, "png": F.IMAGE_PNG, "jpeg": F.IMAGE_JPEG, "tiff": F.IMAGE_TIFF,
+                  "exr": F.IMAGE_EXR, "pfm": F.IMAGE_PFM}
+
+
+def save_image(path, image, format="auto"):
+    """Writes a planar (channels, height, width) float32 image as sol::Image::save does: EXR by
+    default, or PFM; PNG / JPEG / TIFF have no codec in this build and raise."""
+    img = np.ascontiguousarray(image, np.float32)
+    if img.ndim != 3:
+        raise ValueError("image must be (channels, height, width)")
+    F.check_host(F.host.sol_image_save(os.fsencode(path), _IMAGE_FORMATS[format], img.ctypes.data, img.shape[2],
+                                       img.shape[1], img.shape[0]), "sol_image_save")

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import sol_amd
+from tests.test_image_files import read_exr
 
 pytestmark = pytest.mark.gpu
 
@@ -51,3 +52,31 @@ def test_render_job_driver_matches_c_abi_render(tmp_path, renderer, scene_arg):
     want *= np.float32(1.0) / np.float32(spp)   # Image::scale(1.0f / spp)
     np.testing.assert_array_equal(got, want)
     assert got.mean() > 0.01
+
+
+def test_driver_takes_sol_options_and_writes_exr_by_default(tmp_path):
+    """sol's driver command line (positional scene file, -w, --height, -spp, the Russian-roulette
+    and ray-offset options): the image goes to render.exr in channels B, G, R."""
+    if not os.path.exists(DRIVER):
+        pytest.fail(f"{DRIVER} is not built (make -C sol_amd)")
+    w, h, spp = 36, 24, 5
+    r = subprocess.run([DRIVER, GLOSSY, "-w", str(w), "--height", str(h), "-spp", str(spp), "--max-path-len", "6",
+                        "--min-rr-path-len", "2", "--min-survival-prob", "0.1", "--max-survival-prob", "0.9",
+                        "--ray-offset", "2e-5", "-a", "path_tracer"],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "Image was saved to 'render.exr'" in r.stdout
+    assert "Scene summary:" in r.stdout
+    names, ch = read_exr(tmp_path / "render.exr")
+    assert names == ["B", "G", "R"]
+    got = np.stack([ch["R"], ch["G"], ch["B"]])
+    cfg = sol_amd.PathTracerConfig(max_path_len=6, min_rr_path_len=2, min_survival_prob=0.1, max_survival_prob=0.9,
+                                   ray_offset=2e-5)
+    pt = sol_amd.GpuPathTracer(sol_amd.Scene.load(GLOSSY, w, h), cfg)
+    want = sol_amd.new_image(w, h)
+    pt.render(want, 0, spp)
+    want *= np.float32(1.0) / np.float32(spp)
+    np.testing.assert_array_equal(got, want)
+    # unknown options and formats are refused as sol's driver refuses them
+    bad = subprocess.run([DRIVER, GLOSSY, "-f", "bmp"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert bad.returncode == 1 and "Unknown image format 'bmp'" in bad.stderr
