@@ -384,60 +384,74 @@ SOL_DEV void order_pair(float& ta, int& ca, float& tb, int& cb) {
 
 // traverse_step over 4-wide nodes: the four child boxes of a node are tested, the hit children
 // are ordered by entry distance (a five-comparison sorting network), the nearest is visited next
-// and the others are pushed farthest first. Leaves as in traverse_step.
+// and the others are pushed farthest first. Leaves are postponed (Aila and Laine 2009,
+// speculative traversal): a lane that reaches a leaf keeps it and goes on visiting inner nodes
+// while other lanes of the wave are still looking for theirs, so that fewer lanes idle in the
+// node loop; then every lane tests the triangles of its leaf. The visiting order does not change
+// the result: the closest hit is unique by (t, original triangle index).
 template <int BLOCK, int LDS_STACK, typename Fetch>
 SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, float tmin, bool any, TraversalState& ts) {
     constexpr float kMiss = __builtin_huge_valf();
-    while (ts.node >= 0) {
-        Node4 n;
-        f.node(ts.node, n);
-        float t0, t1, t2, t3;
-        bool h0 = intersect_box(r, n.lo_x.x, n.hi_x.x, n.lo_y.x, n.hi_y.x, n.lo_z.x, n.hi_z.x, tmin, ts.tmax, t0);
-        bool h1 = intersect_box(r, n.lo_x.y, n.hi_x.y, n.lo_y.y, n.hi_y.y, n.lo_z.y, n.hi_z.y, tmin, ts.tmax, t1);
-        bool h2 = intersect_box(r, n.lo_x.z, n.hi_x.z, n.lo_y.z, n.hi_y.z, n.lo_z.z, n.hi_z.z, tmin, ts.tmax, t2);
-        bool h3 = intersect_box(r, n.lo_x.w, n.hi_x.w, n.lo_y.w, n.hi_y.w, n.lo_z.w, n.hi_z.w, tmin, ts.tmax, t3);
-        int c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
-        h0 = h0 && c0 != SOL_GPU_EMPTY;
-        h1 = h1 && c1 != SOL_GPU_EMPTY;
-        h2 = h2 && c2 != SOL_GPU_EMPTY;
-        h3 = h3 && c3 != SOL_GPU_EMPTY;
-        const int hits = int(h0) + int(h1) + int(h2) + int(h3);
-        if (hits == 0) {
-            if (!stack_pop<BLOCK, LDS_STACK>(st, ts)) return kStepDone;
-            continue;
+    int leaf = 0;          // postponed leaf (leaf references are negative), 0 = none
+    bool more = true;      // ts.node still has to be visited
+    while (__ballot(more && leaf == 0) != 0) {
+        if (more && ts.node >= 0) {
+            Node4 n;
+            f.node(ts.node, n);
+            float t0, t1, t2, t3;
+            bool h0 = intersect_box(r, n.lo_x.x, n.hi_x.x, n.lo_y.x, n.hi_y.x, n.lo_z.x, n.hi_z.x, tmin, ts.tmax, t0);
+            bool h1 = intersect_box(r, n.lo_x.y, n.hi_x.y, n.lo_y.y, n.hi_y.y, n.lo_z.y, n.hi_z.y, tmin, ts.tmax, t1);
+            bool h2 = intersect_box(r, n.lo_x.z, n.hi_x.z, n.lo_y.z, n.hi_y.z, n.lo_z.z, n.hi_z.z, tmin, ts.tmax, t2);
+            bool h3 = intersect_box(r, n.lo_x.w, n.hi_x.w, n.lo_y.w, n.hi_y.w, n.lo_z.w, n.hi_z.w, tmin, ts.tmax, t3);
+            int c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
+            h0 = h0 && c0 != SOL_GPU_EMPTY;
+            h1 = h1 && c1 != SOL_GPU_EMPTY;
+            h2 = h2 && c2 != SOL_GPU_EMPTY;
+            h3 = h3 && c3 != SOL_GPU_EMPTY;
+            const int hits = int(h0) + int(h1) + int(h2) + int(h3);
+            if (hits == 0) {
+                more = stack_pop<BLOCK, LDS_STACK>(st, ts);
+            } else {
+                float k0 = h0 ? t0 : kMiss, k1 = h1 ? t1 : kMiss, k2 = h2 ? t2 : kMiss, k3 = h3 ? t3 : kMiss;
+                order_pair(k0, c0, k1, c1);
+                order_pair(k2, c2, k3, c3);
+                order_pair(k0, c0, k2, c2);
+                order_pair(k1, c1, k3, c3);
+                order_pair(k1, c1, k2, c2);
+                if (hits > 3) stack_push<BLOCK, LDS_STACK>(st, ts, c3);
+                if (hits > 2) stack_push<BLOCK, LDS_STACK>(st, ts, c2);
+                if (hits > 1) stack_push<BLOCK, LDS_STACK>(st, ts, c1);
+                ts.node = c0;
+            }
         }
-        float k0 = h0 ? t0 : kMiss, k1 = h1 ? t1 : kMiss, k2 = h2 ? t2 : kMiss, k3 = h3 ? t3 : kMiss;
-        order_pair(k0, c0, k1, c1);
-        order_pair(k2, c2, k3, c3);
-        order_pair(k0, c0, k2, c2);
-        order_pair(k1, c1, k3, c3);
-        order_pair(k1, c1, k2, c2);
-        if (hits > 3) stack_push<BLOCK, LDS_STACK>(st, ts, c3);
-        if (hits > 2) stack_push<BLOCK, LDS_STACK>(st, ts, c2);
-        if (hits > 1) stack_push<BLOCK, LDS_STACK>(st, ts, c1);
-        ts.node = c0;
+        if (more && ts.node < 0 && leaf == 0) {
+            leaf = ts.node;
+            more = stack_pop<BLOCK, LDS_STACK>(st, ts);
+        }
     }
-    uint32_t first, count;
-    decode_leaf(ts.node, first, count);
-    for (uint32_t i = 0; i < count; ++i) {
-        float4 p0, p1, p2;
-        f.tri(first + i, p0, p1, p2);
-        float t, u, v;
-        if (intersect_triangle(r, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, t, u, v)) {
-            int prim = __float_as_int(p0.w);
-            if (any) {
-                if (t > tmin && t < ts.tmax) return kStepOccluded;
-            } else if (t > tmin && (t < ts.tmax || (t == ts.tmax && ts.hit.prim >= 0 && prim < ts.hit.prim))) {
-                ts.tmax = t;
-                ts.hit.slot = static_cast<int>(first + i);
-                ts.hit.prim = prim;
-                ts.hit.t = t;
-                ts.hit.u = u;
-                ts.hit.v = v;
+    if (leaf != 0) {
+        uint32_t first, count;
+        decode_leaf(leaf, first, count);
+        for (uint32_t i = 0; i < count; ++i) {
+            float4 p0, p1, p2;
+            f.tri(first + i, p0, p1, p2);
+            float t, u, v;
+            if (intersect_triangle(r, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, t, u, v)) {
+                int prim = __float_as_int(p0.w);
+                if (any) {
+                    if (t > tmin && t < ts.tmax) return kStepOccluded;
+                } else if (t > tmin && (t < ts.tmax || (t == ts.tmax && ts.hit.prim >= 0 && prim < ts.hit.prim))) {
+                    ts.tmax = t;
+                    ts.hit.slot = static_cast<int>(first + i);
+                    ts.hit.prim = prim;
+                    ts.hit.t = t;
+                    ts.hit.u = u;
+                    ts.hit.v = v;
+                }
             }
         }
     }
-    return stack_pop<BLOCK, LDS_STACK>(st, ts) ? kStepContinue : kStepDone;
+    return more ? kStepContinue : kStepDone;
 }
 
 // A whole traversal over the 4-wide layout, looping traverse4_step.
