@@ -174,16 +174,62 @@ def bench_batched(sol_amd, F, torch, steps):
         g.render_device(b.data_ptr(), sp.width, sp.height, 0, sp.spp, cfg, renderer=F.RENDERER_MEGAKERNEL)
     torch.cuda.synchronize()
     mega_s = time.perf_counter() - t
+    # the same batch with every BVH built on the GPU
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    gbs = [ctx.build(sc) for sc in scenes]
+    torch.cuda.synchronize()
+    gpu_build_s = time.perf_counter() - t
+    host_build_s = sum(sc.build_stats().build_seconds + sc.build_stats().optimize_seconds for sc in scenes)
+    gjobs = [(g, sp.width, sp.height, 0, sp.spp, b.data_ptr()) for sp, g, b in zip(specs, gbs, bufs)]
+    ctx.render_batch(gjobs, cfg)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        ctx.render_batch(gjobs, cfg)
+    torch.cuda.synchronize()
+    gbatch_s = (time.perf_counter() - t) / steps
+    for g in gbs:
+        g.close()
     out = {"scenes": len(specs), "triangles": int(sum(sc.num_tris for sc in scenes)), "samples": samples,
            "msamples_s": round(samples / batch_s / 1e6, 2), "ms": round(batch_s * 1e3, 2),
            "megakernel_one_by_one_msamples_s": round(samples / mega_s / 1e6, 2),
            "tuned": {k: getattr(tuned, k) for k in ("block_size", "lds_stack", "path_slots", "tile_w", "tile_h",
                                                    "graph_iters", "bvh_width")},
-           "build_s": round(build_s, 1), "autotune_s": round(tune_s, 1)}
+           "build_s": round(build_s, 1), "autotune_s": round(tune_s, 1),
+           "host_bvh_build_s": round(host_build_s, 2), "gpu_bvh_build_s": round(gpu_build_s, 3),
+           "gpu_bvh_msamples_s": round(samples / gbatch_s / 1e6, 2)}
     log(f"batched: {out}")
     for g in gs:
         g.close()
     ctx.close()
+    return out
+
+
+def bench_gpu_build(sol_amd, F, torch, ctx, scene, cfg, spp):
+    """The headline scene's BVH built on the GPU (solhip_scene_build) against the host build, and
+    the frame rendered on it."""
+    bst = scene.build_stats()
+    warm = sol_amd.Scene.triangle_soup(4096, seed=1)
+    ctx.build(warm).close()                      # first-use costs (code objects) out of the timing
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    gb = ctx.build(scene)
+    build_s = time.perf_counter() - t
+    st = gb.build_stats
+    img = torch.zeros((3, HEIGHT, WIDTH), dtype=torch.float32, device="cuda")
+    gb.render_device(img.data_ptr(), WIDTH, HEIGHT, 0, spp, cfg, renderer=F.RENDERER_WAVEFRONT)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    gb.render_device(img.data_ptr(), WIDTH, HEIGHT, 0, spp, cfg, renderer=F.RENDERER_WAVEFRONT)
+    torch.cuda.synchronize()
+    frame_s = time.perf_counter() - t
+    out = {"build_ms": round(build_s * 1e3, 1), "host_build_s": round(bst.build_seconds + bst.optimize_seconds, 2),
+           "sah": round(st.sah_cost, 3), "host_sah": round(bst.sah_cost_final, 3), "passes": st.iterations,
+           "depth": st.depth, "depth4": st.depth4,
+           "wavefront_msamples_s": round(WIDTH * HEIGHT * spp / frame_s / 1e6, 2)}
+    log(f"gpu build: {out}")
+    gb.close()
     return out
 
 
@@ -342,6 +388,7 @@ def main():
     traffic = read_traffic()
     log(f"device copy: {copy_gbs:.0f} GB/s; k_trace HBM traffic per launch: {traffic}")
 
+    gpu_build = None if args.no_small else bench_gpu_build(sol_amd, F, torch, ctx, scene, cfg, args.spp)
     intersect = None if args.no_small else bench_intersect(sol_amd, torch, not args.no_cpu)
     cornell = None if args.no_small else bench_cornell(sol_amd, F, torch, not args.no_cpu)
     batched = None if args.no_batch else bench_batched(sol_amd, F, torch, args.steps)
@@ -391,6 +438,7 @@ def main():
                                             "sort": round(wst.sort_ms / args.steps, 2),
                                             "shade": round(wst.shade_ms / args.steps, 2)}},
         "cpu_baseline": cpu,
+        "gpu_build": gpu_build,
         "intersect": intersect,
         "cornell": cornell,
         "batched": batched,

@@ -128,6 +128,47 @@ typedef struct solhip_batch_job {
     float* accum;
 } solhip_batch_job;
 
+/* Raw mesh for the GPU BVH builder (host memory; solhip_scene_build copies it). */
+typedef struct solhip_mesh_desc {
+    uint32_t num_vertices;
+    const float* positions;         /* 3 floats per vertex                              */
+    const float* normals;           /* 3 floats per vertex                              */
+    uint32_t num_tris;
+    const uint32_t* indices;        /* 3 vertex indices per triangle                    */
+    const int32_t* tri_material;    /* per triangle, -1 = no BSDF                       */
+    const int32_t* tri_light;       /* per triangle, -1 = not emissive                  */
+    uint32_t num_materials;
+    const sol_material* materials;
+    uint32_t num_lights;
+    const sol_light* lights;
+    sol_camera camera;
+} solhip_mesh_desc;
+
+/* Options of the GPU builder. Zero fields mean the defaults in brackets. */
+typedef struct solhip_build_config {
+    uint32_t max_leaf_size;         /* triangles per leaf, 1..8 [8]                     */
+    float traversal_cost;           /* SAH cost of an inner-node visit [1]              */
+    float intersection_cost;        /* SAH cost of a triangle test [1]                  */
+    uint32_t search_radius;         /* clustering neighbourhood, 1..32 [16]             */
+} solhip_build_config;
+
+typedef struct solhip_build_stats {
+    double build_ms;                /* wall time of the build on the device side        */
+    uint32_t iterations;            /* clustering passes                                */
+    uint32_t num_nodes;             /* binary inner nodes                               */
+    uint32_t num_nodes4;            /* 4-wide nodes                                     */
+    uint32_t num_leaves;
+    uint32_t depth;                 /* longest root-to-leaf path, binary (nodes)         */
+    uint32_t depth4;                /* the same, 4-wide                                 */
+    float sah_cost;                 /* SAH cost of the binary tree / root area           */
+} solhip_build_stats;
+
+/* Sizes of an uploaded or built scene, for solhip_scene_download. */
+typedef struct solhip_scene_info {
+    uint32_t num_nodes, num_nodes4, num_tris, num_materials, num_lights;
+    uint32_t bvh_depth, bvh4_depth, top_nodes, top_nodes4;
+} solhip_scene_info;
+
 const char* solhip_last_error(void);
 
 int solhip_create(int device, solhip_ctx** out);
@@ -144,6 +185,21 @@ int solhip_reset_stats(solhip_ctx* ctx);
 
 int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* desc, solhip_scene** out);
 void solhip_scene_destroy(solhip_scene* scene);
+
+/* Builds the scene's BVH on the GPU: no counterpart in sol, whose TriangleMesh builds its BVH on
+   the host (src/triangle_mesh.cpp:119-146; this library's host builder follows that one). Morton
+   order, then agglomerative clustering of nearest neighbours by merged surface area within a
+   window of the order (PLOC, Meister and Bittner 2018), SAH collapse of subtrees into leaves of up
+   to max_leaf_size triangles, breadth-first layouts (binary and 4-wide). Renders of the result are
+   bit-identical to renders of any other valid BVH of the same mesh. cfg and stats may be null. */
+int solhip_scene_build(solhip_ctx* ctx, const solhip_mesh_desc* mesh, const solhip_build_config* cfg,
+                       solhip_scene** out, solhip_build_stats* stats);
+
+int solhip_scene_info_get(const solhip_scene* scene, solhip_scene_info* out);
+/* Copies the device arrays of a scene to host memory (sizes from solhip_scene_info_get); null
+   pointers are skipped. */
+int solhip_scene_download(const solhip_scene* scene, sol_gpu_node* nodes, sol_gpu_node4* nodes4, sol_gpu_tri* tris,
+                          sol_gpu_shade* shade);
 /* Replace the material or light table (host arrays, same counts as uploaded). */
 int solhip_scene_set_materials(solhip_scene* scene, const sol_material* m, uint32_t n);
 int solhip_scene_set_lights(solhip_scene* scene, const sol_light* l, uint32_t n);

@@ -122,6 +122,30 @@ class SceneDesc(C.Structure):
                 ("top_nodes4", C.c_uint32)]
 
 
+class MeshDesc(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("num_tris", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
+                ("tri_material", C.POINTER(C.c_int32)), ("tri_light", C.POINTER(C.c_int32)),
+                ("num_materials", C.c_uint32), ("materials", C.POINTER(Material)), ("num_lights", C.c_uint32),
+                ("lights", C.POINTER(Light)), ("camera", Camera)]
+
+
+class GpuBuildConfig(C.Structure):
+    _fields_ = [("max_leaf_size", C.c_uint32), ("traversal_cost", C.c_float), ("intersection_cost", C.c_float),
+                ("search_radius", C.c_uint32)]
+
+
+class GpuBuildStats(C.Structure):
+    _fields_ = [("build_ms", C.c_double), ("iterations", C.c_uint32), ("num_nodes", C.c_uint32),
+                ("num_nodes4", C.c_uint32), ("num_leaves", C.c_uint32), ("depth", C.c_uint32), ("depth4", C.c_uint32),
+                ("sah_cost", C.c_float)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [(k, C.c_uint32) for k in ("num_nodes", "num_nodes4", "num_tris", "num_materials", "num_lights",
+                                           "bvh_depth", "bvh4_depth", "top_nodes", "top_nodes4")]
+
+
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("lds_stack", C.c_uint32), ("path_slots", C.c_uint32),
                 ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("graph_iters", C.c_uint32), ("bvh_width", C.c_uint32)]
@@ -173,6 +197,10 @@ _SIGS = {
         "solhip_reset_stats": (C.c_int, [_P]),
         "solhip_scene_upload": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
         "solhip_scene_destroy": (None, [_P]),
+        "solhip_scene_build": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(GpuBuildConfig), C.POINTER(_P),
+                                         C.POINTER(GpuBuildStats)]),
+        "solhip_scene_info_get": (C.c_int, [_P, C.POINTER(SceneInfo)]),
+        "solhip_scene_download": (C.c_int, [_P, _P, _P, _P, _P]),
         "solhip_scene_set_materials": (C.c_int, [_P, C.POINTER(Material), C.c_uint32]),
         "solhip_scene_set_lights": (C.c_int, [_P, C.POINTER(Light), C.c_uint32]),
         "solhip_intersect": (C.c_int, [_P, _P, C.c_uint64, C.c_int, _P, C.c_int]),

@@ -315,6 +315,63 @@ void solhip_scene_destroy(solhip_scene* s) {
     if (ctx->live_scenes == 0 && ctx->destroy_requested) destroy_context(ctx);
 }
 
+int solhip_scene_build(solhip_ctx* ctx, const solhip_mesh_desc* m, const solhip_build_config* cfg, solhip_scene** out,
+                       solhip_build_stats* stats) {
+    if (!ctx || !m || !out) return fail(SOLHIP_E_INVALID, "null argument");
+    *out = nullptr;
+    if (!m->num_tris || !m->num_vertices || !m->positions || !m->normals || !m->indices || !m->tri_material ||
+        !m->tri_light)
+        return fail(SOLHIP_E_INVALID, "mesh without geometry");
+    if (!m->num_lights || !m->lights) return fail(SOLHIP_E_NO_LIGHTS, "scene without light");
+    if (m->num_materials && !m->materials) return fail(SOLHIP_E_INVALID, "null material table");
+    if (const char* e = check_materials(m->materials, m->num_materials)) return fail(SOLHIP_E_INVALID, e);
+    if (m->num_tris >= (1u << 28)) return fail(SOLHIP_E_INVALID, "too many triangles");
+    for (uint64_t i = 0; i < 3ull * m->num_tris; ++i)
+        if (m->indices[i] >= m->num_vertices) return fail(SOLHIP_E_INVALID, "vertex index out of range");
+    for (uint32_t t = 0; t < m->num_tris; ++t) {
+        int mi = m->tri_material[t], li = m->tri_light[t];
+        if (mi >= static_cast<int>(m->num_materials) || mi < -1 || li >= static_cast<int>(m->num_lights) || li < -1)
+            return fail(SOLHIP_E_INVALID, "material or light index out of range");
+    }
+    solhip_build_config c{8, 1.0f, 1.0f, 16};
+    if (cfg) {
+        if (cfg->max_leaf_size) c.max_leaf_size = cfg->max_leaf_size;
+        if (cfg->traversal_cost > 0.0f) c.traversal_cost = cfg->traversal_cost;
+        if (cfg->intersection_cost > 0.0f) c.intersection_cost = cfg->intersection_cost;
+        if (cfg->search_radius) c.search_radius = cfg->search_radius;
+    }
+    if (c.max_leaf_size > 8 || c.search_radius > 32) return fail(SOLHIP_E_INVALID, "build configuration out of range");
+    if (stats) *stats = solhip_build_stats{};
+    return build_scene(*ctx, *m, c, out, stats);
+}
+
+int solhip_scene_info_get(const solhip_scene* s, solhip_scene_info* out) {
+    if (!s || !out) return fail(SOLHIP_E_INVALID, "null argument");
+    out->num_nodes = s->num_nodes;
+    out->num_nodes4 = s->view.num_nodes4;
+    out->num_tris = s->num_tris;
+    out->num_materials = s->num_materials;
+    out->num_lights = s->num_lights;
+    out->bvh_depth = s->bvh_depth;
+    out->bvh4_depth = s->bvh4_depth;
+    out->top_nodes = s->view.top_nodes;
+    out->top_nodes4 = s->view.top_nodes4;
+    return SOLHIP_OK;
+}
+
+int solhip_scene_download(const solhip_scene* s, sol_gpu_node* nodes, sol_gpu_node4* nodes4, sol_gpu_tri* tris,
+                          sol_gpu_shade* shade) {
+    if (!s) return fail(SOLHIP_E_INVALID, "null scene");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    if (nodes) HIP_TRY(hipMemcpy(nodes, s->nodes, sizeof(sol_gpu_node) * s->num_nodes, hipMemcpyDeviceToHost));
+    if (nodes4 && s->nodes4)
+        HIP_TRY(hipMemcpy(nodes4, s->nodes4, sizeof(sol_gpu_node4) * s->view.num_nodes4, hipMemcpyDeviceToHost));
+    if (tris) HIP_TRY(hipMemcpy(tris, s->tris, sizeof(sol_gpu_tri) * s->num_tris, hipMemcpyDeviceToHost));
+    if (shade) HIP_TRY(hipMemcpy(shade, s->shade, sizeof(sol_gpu_shade) * s->num_tris, hipMemcpyDeviceToHost));
+    return SOLHIP_OK;
+}
+
 int solhip_scene_set_materials(solhip_scene* s, const sol_material* m, uint32_t n) {
     if (!s || !m || n != s->num_materials) return fail(SOLHIP_E_INVALID, "material count mismatch");
     if (const char* e = check_materials(m, n)) return fail(SOLHIP_E_INVALID, e);

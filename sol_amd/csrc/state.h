@@ -91,6 +91,9 @@ struct solhip_scene {
 };
 
 namespace solhip {
+// GPU BVH construction (gpu_build.hip); the mesh and configuration are validated by the caller.
+int build_scene(Context& ctx, const solhip_mesh_desc& mesh, const solhip_build_config& cfg, solhip_scene** out,
+                solhip_build_stats* stats);
 int render_megakernel(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,
                       float* d_accum);
 int render_wavefront(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,

@@ -112,6 +112,12 @@ class Context:
     def upload(self, scene):
         return GpuScene(self, scene)
 
+    def build(self, scene, max_leaf_size=0, search_radius=0, traversal_cost=0.0, intersection_cost=0.0):
+        """The scene's mesh with a BVH built on the GPU (solhip_scene_build); the stats of the
+        build are in the result's build_stats."""
+        return GpuScene(self, scene, gpu_build=F.GpuBuildConfig(max_leaf_size, traversal_cost, intersection_cost,
+                                                             search_radius))
+
     def render_batch(self, jobs, config=None):
         """jobs: list of (GpuScene, width, height, sample_index, sample_count, device_ptr)."""
         arr = (F.BatchJob * len(jobs))()
@@ -127,14 +133,44 @@ class Context:
 class GpuScene:
     """A scene uploaded to HBM (solhip_scene)."""
 
-    def __init__(self, ctx, scene):
+    def __init__(self, ctx, scene, gpu_build=None):
         self.ctx = ctx
         self.scene = scene
-        desc = scene.gpu_desc()
+        self.build_stats = None
         h = C.c_void_p()
-        F.check_hip(F.hip.solhip_scene_upload(ctx.handle, C.byref(desc), C.byref(h)), "solhip_scene_upload")
+        if gpu_build is None:
+            desc = scene.gpu_desc()
+            F.check_hip(F.hip.solhip_scene_upload(ctx.handle, C.byref(desc), C.byref(h)), "solhip_scene_upload")
+        else:
+            mv = scene.mesh_view()
+            md = F.MeshDesc()
+            for k in ("num_vertices", "positions", "normals", "num_tris", "indices", "tri_material", "tri_light",
+                      "num_materials", "materials", "num_lights", "lights"):
+                setattr(md, k, getattr(mv, k))
+            F.host.sol_camera_from_params(C.byref(mv.camera), C.byref(md.camera))
+            st = F.GpuBuildStats()
+            F.check_hip(F.hip.solhip_scene_build(ctx.handle, C.byref(md), C.byref(gpu_build), C.byref(h), C.byref(st)),
+                        "solhip_scene_build")
+            self.build_stats = st
         self._h = h
         _live_scenes.add(self)
+
+    def info(self):
+        i = F.SceneInfo()
+        F.check_hip(F.hip.solhip_scene_info_get(self._h, C.byref(i)), "solhip_scene_info_get")
+        return i
+
+    def download(self):
+        """The scene's device arrays as numpy structured arrays: nodes, nodes4, tris, shade."""
+        i = self.info()
+        nodes = (F.GpuNode * max(1, i.num_nodes))()
+        nodes4 = (F.GpuNode4 * max(1, i.num_nodes4))()
+        tris = (F.GpuTri * i.num_tris)()
+        shade = (F.GpuShade * i.num_tris)()
+        F.check_hip(F.hip.solhip_scene_download(self._h, C.addressof(nodes), C.addressof(nodes4) if i.num_nodes4 else None,
+                                                C.addressof(tris), C.addressof(shade)), "solhip_scene_download")
+        return {"nodes": nodes[:i.num_nodes], "nodes4": nodes4[:i.num_nodes4], "tris": tris[:], "shade": shade[:],
+                "info": i}
 
     def close(self):
         if self._h and self._h.value:
