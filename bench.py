@@ -74,6 +74,17 @@ def read_traffic():
     return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -403,11 +414,20 @@ def main():
         t = time.perf_counter()
         osc.render(cimg, 0, args.cpu_spp, cfg, threads=threads)
         dt = time.perf_counter() - t
+        # one thread on one sample of the frame (a secondary figure)
+        one = sol_amd.new_image(WIDTH, HEIGHT)
+        t = time.perf_counter()
+        osc.render(one, 0, 1, cfg, threads=1)
+        dt1 = time.perf_counter() - t
         cpu = {"value": WIDTH * HEIGHT * args.cpu_spp / dt / 1e6, "unit": "Msamples/s", "cores": threads,
                "kind": "port",
                "sample": f"1920x1080 at {args.cpu_spp} spp (samples 0..{args.cpu_spp - 1}) of the same frame, "
-                         f"oracle over the same BVH, {dt:.1f} s"}
-        log(f"cpu oracle: {cpu['value']:.2f} Msamples/s on {threads} threads")
+                         f"oracle over the same BVH, {dt:.1f} s",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+               "one_thread": {"value": round(WIDTH * HEIGHT / dt1 / 1e6, 3),
+                              "sample": f"1920x1080 at 1 spp (sample 0), {dt1:.1f} s"}}
+        log(f"cpu oracle: {cpu['value']:.2f} Msamples/s on {threads} threads; one thread "
+            f"{cpu['one_thread']['value']:.3f} Msamples/s")
 
     out = {
         "metric": "Msamples/s at 1080p 64spp, wavefront-batched vs plain GPU, at equal RMSE",

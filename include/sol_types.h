@@ -167,7 +167,9 @@ typedef struct sol_gpu_tri {
 
 /* Shading record in leaf order: the three vertex normals, the geometric normal
    cross(v1 - v0, v2 - v0) (computed in fp32 without fused multiply-add), material and light of
-   the triangle. 64 bytes. */
+   the triangle, and the BSDF kind of that material (what the hit sorting reads, one dependent
+   load fewer; the GPU library recomputes it on upload and whenever the materials change).
+   64 bytes. */
 typedef struct sol_gpu_shade {
     float n0[3];
     float n1[3];
@@ -175,7 +177,8 @@ typedef struct sol_gpu_shade {
     float face_n[3];
     int32_t material;  /* index into the material table, -1 = no BSDF */
     int32_t light;     /* index into the light table, -1 = not emissive */
-    int32_t pad[2];
+    int32_t bsdf;      /* SOL_BSDF_* of the material, SOL_BSDF_NONE without one */
+    int32_t pad;
 } sol_gpu_shade;
 
 #ifdef __cplusplus

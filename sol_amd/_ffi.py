@@ -74,7 +74,7 @@ class GpuTri(C.Structure):
 
 class GpuShade(C.Structure):
     _fields_ = [("n0", C.c_float * 3), ("n1", C.c_float * 3), ("n2", C.c_float * 3), ("face_n", C.c_float * 3),
-                ("material", C.c_int32), ("light", C.c_int32), ("pad", C.c_int32 * 2)]
+                ("material", C.c_int32), ("light", C.c_int32), ("bsdf", C.c_int32), ("pad", C.c_int32)]
 
 
 class BvhConfig(C.Structure):

@@ -32,6 +32,22 @@ int fail(int code, const std::string& msg) {
         }                                                                                        \
     } while (0)
 
+// BSDF kind of every shading record from the current material table.
+__global__ void k_shade_kinds(sol_gpu_shade* shade, uint32_t n, const sol_material* mats) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int m = shade[i].material;
+    shade[i].bsdf = m >= 0 ? mats[m].type : SOL_BSDF_NONE;
+}
+
+int refresh_kinds(solhip_scene* s) {
+    hipLaunchKernelGGL(k_shade_kinds, dim3((s->num_tris + 255) / 256), dim3(256), 0, s->ctx->stream, s->shade,
+                       s->num_tris, s->materials);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    return SOLHIP_OK;
+}
+
 bool valid_shape_values(uint32_t block, uint32_t lds) {
     return (block == 64 || block == 128 || block == 256) && (lds == 8 || lds == 16 || lds == 32);
 }
@@ -294,6 +310,7 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     v.num_nodes = d->num_nodes;
     v.num_tris = d->num_tris;
     v.camera = d->camera;
+    if (int st = refresh_kinds(s.get())) return st;
     ctx->live_scenes += 1;
     *out = s.release();
     return SOLHIP_OK;
@@ -378,7 +395,10 @@ int solhip_scene_set_materials(solhip_scene* s, const sol_material* m, uint32_t 
     HIP_TRY(hipSetDevice(s->ctx->device));
     HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     HIP_TRY(hipMemcpy(s->materials, m, sizeof(sol_material) * n, hipMemcpyHostToDevice));
+    bool kinds_changed = false;
+    for (uint32_t i = 0; i < n; ++i) kinds_changed = kinds_changed || m[i].type != s->materials_host[i].type;
     s->materials_host.assign(m, m + n);
+    if (kinds_changed) return refresh_kinds(s);
     return SOLHIP_OK;
 }
 

@@ -421,7 +421,8 @@ __global__ void k_single_leaf(int root, uint32_t n, const float4* nlo, const flo
 // Triangles and shading records in leaf order (the geometric normal as the host computes it:
 // cross(v1 - v0, v2 - v0) in fp32 without contraction).
 __global__ void k_gather(const unsigned* order, uint32_t n, const float* P, const float* N, const uint32_t* I,
-                         const int32_t* tmat, const int32_t* tlight, sol_gpu_tri* tris, sol_gpu_shade* shade) {
+                         const int32_t* tmat, const int32_t* tlight, const sol_material* mats, sol_gpu_tri* tris,
+                         sol_gpu_shade* shade) {
     uint32_t s = blockIdx.x * kBB + threadIdx.x;
     if (s >= n) return;
     const uint32_t t = order[s];
@@ -443,7 +444,8 @@ __global__ void k_gather(const unsigned* order, uint32_t n, const float* P, cons
     h.face_n[0] = fn.x; h.face_n[1] = fn.y; h.face_n[2] = fn.z;
     h.material = tmat[t];
     h.light = tlight[t];
-    h.pad[0] = h.pad[1] = 0;
+    h.bsdf = h.material >= 0 ? mats[h.material].type : SOL_BSDF_NONE;
+    h.pad = 0;
     shade[s] = h;
 }
 
@@ -685,12 +687,12 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     BUILD_TRY(shade.alloc(n));
     BUILD_TRY(mats.alloc(m.num_materials));
     BUILD_TRY(lights.alloc(m.num_lights));
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kBB), 0, st, order.p, n, P.p, N.p, I.p, tmat.p, tlight.p,
-                       tris.p, shade.p);
-    BUILD_TRY(hipGetLastError());
     if (m.num_materials)
         BUILD_TRY(hipMemcpyAsync(mats.p, m.materials, sizeof(sol_material) * m.num_materials, hipMemcpyHostToDevice, st));
     BUILD_TRY(hipMemcpyAsync(lights.p, m.lights, sizeof(sol_light) * m.num_lights, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kBB), 0, st, order.p, n, P.p, N.p, I.p, tmat.p, tlight.p,
+                       mats.p, tris.p, shade.p);
+    BUILD_TRY(hipGetLastError());
     float sah = 0.0f;
     if (stats) {
         Buf<double> acc;

@@ -17,6 +17,7 @@ struct Surface {
     bool front;    // the ray hit the side the geometric normal faces
     int mat;       // material index, -1 = no BSDF
     int light;     // light index, -1 = not emissive
+    int bsdf;      // SOL_BSDF_* of the material (from the shading record)
     float su, sv;  // surface coordinates (barycentrics of v1, v2)
 };
 
@@ -39,6 +40,7 @@ SOL_DEV Surface decode_hit(const SceneView& s, V3 org, V3 dir, int slot, float t
     sf.n = normalize(normal);
     sf.mat = __float_as_int(d.x);
     sf.light = __float_as_int(d.y);
+    sf.bsdf = __float_as_int(d.z);
     sf.su = u;
     sf.sv = v;
     return sf;

@@ -450,7 +450,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
 // ---- sort ------------------------------------------------------------------------------------------
 
 template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_sort(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sort(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                  int cur) {
     __shared__ RoundLds<kNumShadeQueues> lds[2];
     const SegQueue eqv(b.counts, kQExtend0 + cur, b.cap_e);
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_sort(SceneView single, WaveTables ta
                     rec1 = f4(sf.n, as_f(slot));
                     rec2 = f4(sf.face_n, 0.0f);
                     rec3 = f4(dir, 0.0f);
-                    dest = queue_of(s.materials[sf.mat].type);
+                    dest = queue_of(sf.bsdf);
                 }
             }
             if (dead) retire(b, b.path[slot * 4 + 1], b.path[slot * 4 + 2]);

@@ -335,6 +335,7 @@ int sol_scene_build_stats(const sol_scene* s, sol_build_stats* out) {
 int sol_scene_set_materials(sol_scene* s, const sol_material* m, uint32_t n) {
     if (!s || !m || n != s->flat.materials.size()) return fail(SOL_E_INVALID, "material count mismatch");
     std::memcpy(s->flat.materials.data(), m, sizeof(sol_material) * n);
+    for (sol_gpu_shade& r : s->flat.gpu_shade) r.bsdf = r.material >= 0 ? m[r.material].type : SOL_BSDF_NONE;
     return SOL_OK;
 }
 

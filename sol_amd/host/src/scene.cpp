@@ -319,6 +319,7 @@ FlatScene flatten_scene(const Scene& scene, uint32_t top_levels) {
         copy3(s.face_n, cross(v1 - v0, v2 - v0));
         s.material = f.tri_material[t];
         s.light = f.tri_light[t];
+        s.bsdf = s.material >= 0 ? f.materials[s.material].type : SOL_BSDF_NONE;
     }
     f.flatten_seconds = seconds_since(t0);
     return f;

@@ -154,6 +154,30 @@ def test_wavefront_equals_megakernel_bitwise(ctx):
     np.testing.assert_array_equal(a, b)
 
 
+def test_material_kind_changes_reach_the_wavefront_queues(ctx):
+    # The hit sorting reads each triangle's BSDF kind from its shading record: after
+    # set_materials turns diffuse materials into mirrors and Phong into the diffuse-Phong mix, the
+    # wavefront must still render what the megakernel (which reads the material table) renders.
+    scene = sol_amd.Scene.sphere_field(num_spheres=30, subdivision=2, seed=8, extent=4.0, radius_max=1.0)
+    g = ctx.upload(scene)
+    mats = scene.materials()
+    for m in mats:
+        if m.type == F.BSDF_DIFFUSE:
+            m.type = F.BSDF_MIRROR
+            m.ks[:] = list(m.kd)
+        elif m.type == F.BSDF_PHONG:
+            m.type = F.BSDF_INTERP
+            m.kd[:] = [0.3, 0.3, 0.3]
+            m.mix = 0.25
+    g.set_materials(mats)
+    cfg = _cfg(max_path_len=10)
+    a, b = sol_amd.new_image(64, 40), sol_amd.new_image(64, 40)
+    g.render(a, 0, 4, cfg, renderer=F.RENDERER_MEGAKERNEL)
+    g.render(b, 0, 4, cfg, renderer=F.RENDERER_WAVEFRONT)
+    assert a.any()
+    np.testing.assert_array_equal(a, b)
+
+
 def test_wavefront_profiling_mode_matches_graph_mode(ctx, cornell):
     scene, _ = cornell
     g = ctx.upload(scene)
