@@ -1,9 +1,10 @@
 // Adjoint path replay: gradients of an image loss with respect to the diffuse albedo and the
 // Phong exponent (log ns) of every material and the emission of every light.
 //
-// For each (pixel, sample) the kernel runs trace_path once to get the sample's radiance C, then
-// replays the same path (same PCG stream, same traversal) and attributes C to the parameters
-// (path replay backpropagation, Vicini, Speierer and Jakob 2021). With sampling decisions, MIS
+// The sample's radiance C comes from a forward pass of the wavefront renderer that keeps every
+// sample's radiance (the same fp32 operations as trace_path, so the same C); the kernel replays
+// each path (same PCG stream, same traversal) and attributes C to the parameters (path replay
+// backpropagation, Vicini, Speierer and Jakob 2021). With sampling decisions, MIS
 // weights and Russian roulette held constant, every term the path adds is a product of BSDF values
 // f_j, one emission Le and constants, so with delta = dLoss/dImage at the pixel:
 //   emission term T Le w at a light hit      dLe  += delta * T * w
@@ -133,6 +134,7 @@ SOL_DEV void replay_path(const SceneView& s, const sol_pt_config& cfg, const flo
 template <int BLOCK, int LDS>
 __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_config cfg, uint32_t w, uint32_t h,
                                                         uint64_t sample_index, uint32_t sample_count,
+                                                        const float4* __restrict__ radiance,
                                                         const float* __restrict__ d_image, uint32_t num_materials,
                                                         float* d_mat, float* d_light, unsigned long long* counter,
                                                         unsigned long long* stats, TraversalScratch sc) {
@@ -164,16 +166,12 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_conf
         V3 delta{d_image[p], d_image[plane + p], d_image[2 * plane + p]};
         if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) continue;
         for (uint32_t i = 0; i < sample_count; ++i) {
-            uint32_t seed = pixel_seed(sample_index + i, x, y);
+            const float4 c = radiance[p * sample_count + i];
             Pcg rng;
-            rng.seed(seed);
+            rng.seed(pixel_seed(sample_index + i, x, y));
             V3 org, dir;
             camera_ray(s.camera, x, y, w, h, rng, org, dir);
-            V3 C = trace_path<BLOCK, LDS>(s, cfg, lds_nodes, st, rng, org, dir, n_ext, n_shadow);
-            Pcg rng2;
-            rng2.seed(seed);
-            camera_ray(s.camera, x, y, w, h, rng2, org, dir);
-            replay_path<BLOCK, LDS>(s, cfg, lds_nodes, st, rng2, org, dir, C, delta, g, n_ext, n_shadow);
+            replay_path<BLOCK, LDS>(s, cfg, lds_nodes, st, rng, org, dir, V3{c.x, c.y, c.z}, delta, g, n_ext, n_shadow);
         }
         n_samples += sample_count;
     }
@@ -194,31 +192,33 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_conf
 
 template <int BLOCK, int LDS>
 hipError_t adjoint_impl(const SceneView& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si,
-                        uint32_t count, const float* d_image, uint32_t num_materials, float* d_mat, float* d_light,
+                        uint32_t count, const float4* radiance, const float* d_image, uint32_t num_materials,
+                        float* d_mat, float* d_light,
                         unsigned long long* counter, unsigned long long* stats, const TraversalScratch& sc, int grid,
                         hipStream_t stream) {
     (void)hipGetLastError();
     hipLaunchKernelGGL((adjoint_kernel<BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, cfg, w, h, si, count,
-                       d_image, num_materials, d_mat, d_light, counter, stats, sc);
+                       radiance, d_image, num_materials, d_mat, d_light, counter, stats, sc);
     return hipGetLastError();
 }
 
 } // namespace
 
 hipError_t launch_adjoint(const SceneView& s, const sol_pt_config& cfg, uint32_t width, uint32_t height,
-                          uint64_t sample_index, uint32_t sample_count, const float* d_image, uint32_t num_materials,
+                          uint64_t sample_index, uint32_t sample_count, const float4* radiance, const float* d_image,
+                          uint32_t num_materials,
                           float* d_mat, float* d_light, unsigned long long* work_counter, StatCounters stats,
                           const TraversalScratch& scratch, const LaunchShape& shape, hipStream_t stream) {
     switch (shape.block * 100 + shape.lds_stack) {
-        case 6408: return adjoint_impl<64, 8>(s, cfg, width, height, sample_index, sample_count, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 6416: return adjoint_impl<64, 16>(s, cfg, width, height, sample_index, sample_count, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 6432: return adjoint_impl<64, 32>(s, cfg, width, height, sample_index, sample_count, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 12808: return adjoint_impl<128, 8>(s, cfg, width, height, sample_index, sample_count, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 12816: return adjoint_impl<128, 16>(s, cfg, width, height, sample_index, sample_count, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 12832: return adjoint_impl<128, 32>(s, cfg, width, height, sample_index, sample_count, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 25608: return adjoint_impl<256, 8>(s, cfg, width, height, sample_index, sample_count, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 25616: return adjoint_impl<256, 16>(s, cfg, width, height, sample_index, sample_count, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 25632: return adjoint_impl<256, 32>(s, cfg, width, height, sample_index, sample_count, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 6408: return adjoint_impl<64, 8>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 6416: return adjoint_impl<64, 16>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 6432: return adjoint_impl<64, 32>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 12808: return adjoint_impl<128, 8>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 12816: return adjoint_impl<128, 16>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 12832: return adjoint_impl<128, 32>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 25608: return adjoint_impl<256, 8>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 25616: return adjoint_impl<256, 16>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 25632: return adjoint_impl<256, 32>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
         default: return hipErrorInvalidValue;
     }
 }

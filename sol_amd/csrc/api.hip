@@ -487,7 +487,7 @@ int solhip_backward(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
     if (width == 0 || height == 0) return fail(SOLHIP_E_INVALID, "empty image");
     if (cfg->min_survival_prob < 0 || cfg->max_survival_prob > 1 || cfg->min_survival_prob > cfg->max_survival_prob)
         return fail(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
-    if (sample_count == 0) return SOLHIP_OK;
+    if (sample_count == 0 || cfg->max_path_len == 0) return SOLHIP_OK;   // no path, no gradient
     Context* ctx = s->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t plane = static_cast<size_t>(width) * height;
@@ -508,20 +508,34 @@ int solhip_backward(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
     LaunchShape shape = ctx->shape(ctx->tuning.block_size, ctx->tuning.lds_stack);
     int st = ctx->ensure_scratch(shape.grid * shape.block);
     if (st) return st;
-    HIP_TRY(launch_zero_u64(ctx->counters, 4, ctx->stream));
-    HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
-    HIP_TRY(launch_adjoint(s->view, *cfg, width, height, sample_index, sample_count, img, s->num_materials, gm, gl,
-                           ctx->counters, ctx->counters + 1, ctx->scratch, shape, ctx->stream));
-    HIP_TRY(hipEventRecord(ctx->ev_end, ctx->stream));
-    unsigned long long c[4];
-    st = ctx->read_counters(c, 4);
-    if (st) return st;
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
-    ctx->stats.render_ms += ms;
-    ctx->stats.extension_rays += c[1];
-    ctx->stats.shadow_rays += c[2];
-    ctx->stats.launches += 1;
+    // Samples in batches whose per-sample radiance fits the forward buffer (kAdjointBytes).
+    constexpr size_t kAdjointBytes = size_t(4) << 30;
+    const uint32_t batch = static_cast<uint32_t>(
+        std::max<size_t>(1, std::min<size_t>(sample_count, kAdjointBytes / (sizeof(float4) * plane))));
+    HIP_TRY(ctx->adj_radiance.reserve(sizeof(float4) * plane * batch));
+    HIP_TRY(ctx->adj_image.reserve(sizeof(float) * 3 * plane));
+    for (uint32_t first = 0; first < sample_count; first += batch) {
+        const uint32_t n = std::min(batch, sample_count - first);
+        // forward: every sample's radiance (the image it accumulates is discarded)
+        st = render_wavefront(*s, *cfg, width, height, sample_index + first, n, ctx->adj_image.as<float>(),
+                              ctx->adj_radiance.as<float4>());
+        if (st) return st;
+        HIP_TRY(launch_zero_u64(ctx->counters, 4, ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
+        HIP_TRY(launch_adjoint(s->view, *cfg, width, height, sample_index + first, n, ctx->adj_radiance.as<float4>(),
+                               img, s->num_materials, gm, gl, ctx->counters, ctx->counters + 1, ctx->scratch, shape,
+                               ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev_end, ctx->stream));
+        unsigned long long c[4];
+        st = ctx->read_counters(c, 4);
+        if (st) return st;
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+        ctx->stats.render_ms += ms;
+        ctx->stats.extension_rays += c[1];
+        ctx->stats.shadow_rays += c[2];
+        ctx->stats.launches += 1;
+    }
     if (!device_ptrs) {
         std::vector<float> g(n_mat + n_light);
         HIP_TRY(hipMemcpy(g.data(), gm, sizeof(float) * (n_mat + n_light), hipMemcpyDeviceToHost));

@@ -36,9 +36,11 @@ hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint3
                              const LaunchShape& shape, hipStream_t stream);
 
 // Adjoint path replay: adds dLoss/dparameter into d_mat (num_materials x 4) and d_light
-// (num_lights x 3) for the loss gradient d_image (planar RGB, width x height).
+// (num_lights x 3) for the loss gradient d_image (planar RGB, width x height); radiance holds each
+// sample's radiance, pixel-major (sample_count per pixel).
 hipError_t launch_adjoint(const SceneView& s, const sol_pt_config& cfg, uint32_t width, uint32_t height,
-                          uint64_t sample_index, uint32_t sample_count, const float* d_image, uint32_t num_materials,
+                          uint64_t sample_index, uint32_t sample_count, const float4* radiance, const float* d_image,
+                          uint32_t num_materials,
                           float* d_mat, float* d_light, unsigned long long* work_counter, StatCounters stats,
                           const TraversalScratch& scratch, const LaunchShape& shape, hipStream_t stream);
 

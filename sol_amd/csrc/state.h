@@ -60,6 +60,7 @@ struct Context {
     TraversalScratch scratch{};
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     WavefrontState* wave = nullptr;
+    DeviceBuffer adj_radiance, adj_image;   // forward pass of solhip_backward (grow-only)
     int live_scenes = 0;             // scenes uploaded and not yet destroyed
     bool destroy_requested = false;  // solhip_destroy was called while scenes were alive
 
@@ -96,6 +97,7 @@ int build_scene(Context& ctx, const solhip_mesh_desc& mesh, const solhip_build_c
                 solhip_build_stats* stats);
 int render_megakernel(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,
                       float* d_accum);
+// d_samples (optional): each sample's radiance, pixel-major, sc per pixel.
 int render_wavefront(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,
-                     float* d_accum);
+                     float* d_accum, float4* d_samples = nullptr);
 } // namespace solhip

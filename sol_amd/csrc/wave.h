@@ -113,6 +113,11 @@ struct WaveTables {
     const unsigned long long* job_sample0;  // absolute sample index of the job's first sample
     const uint32_t* job_scene;
     float* const* job_accum;          // device image of each job
+    // Optional per-sample radiance of each job (null = not kept): sample s of pixel p (row-major)
+    // goes to job_sout[job][p * job_sstride[job] + job_soff[job] + s].
+    float4* const* job_sout;
+    const uint32_t* job_sstride;
+    const uint32_t* job_soff;
     const uint32_t* job_w;
     const uint32_t* job_h;
     const uint32_t* job_spp;

@@ -682,18 +682,21 @@ __global__ __launch_bounds__(256) void k_reduce(WaveTables tabs, WaveBuf b, uint
     if (pix >= per_sample) return;
     unsigned long long k0 = tabs.tile_item0[tile] - chunk_begin;
     unsigned long long spp = (tabs.tile_item0[tile + 1] - tabs.tile_item0[tile]) / per_sample;
+    uint32_t job = tabs.tile_job[tile];
+    uint32_t w = tabs.job_w[job], h = tabs.job_h[job];
+    uint32_t x = tabs.tile_x0[tile] + pix % tw, y = tabs.tile_y0[tile] + pix / tw;
+    size_t plane = static_cast<size_t>(w) * h, p = static_cast<size_t>(y) * w + x;
+    float4* sout = tabs.job_sout[job];
+    if (sout) sout += p * tabs.job_sstride[job] + tabs.job_soff[job];
     float r = 0.0f, g = 0.0f, bl = 0.0f;
     for (unsigned long long s = 0; s < spp; ++s) {
         float4 v = b.samples[k0 + s * per_sample + pix];
+        if (sout) sout[s] = v;
         r = r + v.x;
         g = g + v.y;
         bl = bl + v.z;
     }
-    uint32_t job = tabs.tile_job[tile];
-    uint32_t w = tabs.job_w[job], h = tabs.job_h[job];
-    uint32_t x = tabs.tile_x0[tile] + pix % tw, y = tabs.tile_y0[tile] + pix / tw;
     float* accum = tabs.job_accum[job];
-    size_t plane = static_cast<size_t>(w) * h, p = static_cast<size_t>(y) * w + x;
     accum[p] += r;
     accum[plane + p] += g;
     accum[2 * plane + p] += bl;

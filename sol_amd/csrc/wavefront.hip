@@ -33,6 +33,7 @@ struct Job {
     uint64_t sample_index;
     uint32_t sample_count;
     float* accum;
+    float4* samples = nullptr;   // per-sample radiance, pixel-major (sample_count per pixel), or null
 };
 
 } // namespace
@@ -42,6 +43,7 @@ struct WavefrontState {
     DeviceBuffer arena, counters, samples;
     unsigned long long sample_capacity = 0;
     WaveBuf buf{};
+    DeviceBuffer t_job_sout, t_job_sstride, t_job_soff;
     DeviceBuffer t_scenes, t_job_item0, t_job_sample0, t_job_scene, t_job_accum, t_job_w, t_job_h, t_job_spp,
         t_job_tw, t_job_th, t_tile_job, t_tile_x0, t_tile_y0, t_tile_w, t_tile_h, t_tile_item0;
     uint64_t generation = 0;
@@ -178,6 +180,8 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     std::vector<unsigned long long> job_item0, job_sample0;
     std::vector<uint32_t> job_scene, job_w, job_h, job_spp, job_tw, job_th;
     std::vector<float*> job_accum;
+    std::vector<float4*> job_sout;
+    std::vector<uint32_t> job_sstride, job_soff;
     std::vector<uint32_t> tile_job, tile_x0, tile_y0, tile_w, tile_h;
     std::vector<unsigned long long> tile_item0;
     unsigned long long total = 0;
@@ -200,6 +204,9 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
             job_tw.push_back(tw);
             job_th.push_back(th);
             job_accum.push_back(j.accum);
+            job_sout.push_back(j.samples);
+            job_sstride.push_back(j.sample_count);
+            job_soff.push_back(s0);
             // tiles in item order: bands of th rows, tiles of tw columns
             for (uint32_t y0 = 0; y0 < j.h; y0 += th) {
                 uint32_t bh = std::min(th, j.h - y0);
@@ -227,6 +234,8 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     if ((rc = st.upload(st.t_scenes, views, s)) || (rc = st.upload(st.t_job_item0, job_item0, s)) ||
         (rc = st.upload(st.t_job_sample0, job_sample0, s)) || (rc = st.upload(st.t_job_scene, job_scene, s)) ||
         (rc = st.upload(st.t_job_accum, job_accum, s)) || (rc = st.upload(st.t_job_w, job_w, s)) ||
+        (rc = st.upload(st.t_job_sout, job_sout, s)) || (rc = st.upload(st.t_job_sstride, job_sstride, s)) ||
+        (rc = st.upload(st.t_job_soff, job_soff, s)) ||
         (rc = st.upload(st.t_job_h, job_h, s)) || (rc = st.upload(st.t_job_spp, job_spp, s)) ||
         (rc = st.upload(st.t_job_tw, job_tw, s)) || (rc = st.upload(st.t_job_th, job_th, s)) ||
         (rc = st.upload(st.t_tile_job, tile_job, s)) || (rc = st.upload(st.t_tile_x0, tile_x0, s)) ||
@@ -263,6 +272,9 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     L.tabs.job_sample0 = st.t_job_sample0.as<unsigned long long>();
     L.tabs.job_scene = st.t_job_scene.as<uint32_t>();
     L.tabs.job_accum = st.t_job_accum.as<float*>();
+    L.tabs.job_sout = st.t_job_sout.as<float4*>();
+    L.tabs.job_sstride = st.t_job_sstride.as<uint32_t>();
+    L.tabs.job_soff = st.t_job_soff.as<uint32_t>();
     L.tabs.job_w = st.t_job_w.as<uint32_t>();
     L.tabs.job_h = st.t_job_h.as<uint32_t>();
     L.tabs.job_spp = st.t_job_spp.as<uint32_t>();
@@ -381,10 +393,10 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
 } // namespace
 
 int render_wavefront(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,
-                     float* d_accum) {
+                     float* d_accum, float4* d_samples) {
     if (cfg.max_path_len == 0) return SOLHIP_OK;
     std::vector<SceneView> views{s.view};
-    std::vector<Job> jobs{Job{0, w, h, si, sc, d_accum}};
+    std::vector<Job> jobs{Job{0, w, h, si, sc, d_accum, d_samples}};
     return run_wavefront(*s.ctx, views, jobs, cfg, s.bvh_depth, s.bvh4_depth);
 }
 
