@@ -224,9 +224,11 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
                         const sol_pt_config* cfg);
 
 /* Adjoint path replay. d_image: 3 * width * height floats (planar RGB), dLoss/d(accum).
-   Adds gradients into d_materials (num_materials x 4: d kd.r, d kd.g, d kd.b, d log ns) and
+   Adds gradients into d_materials (num_materials x 4: d kd.r, d kd.g, d kd.b, d log ns; kd of a
+   diffuse BSDF or of the diffuse lobe of SOL_BSDF_INTERP, ns of a Phong BSDF or lobe) and
    d_lights (num_lights x 3: d intensity r, g, b). Sampling decisions, MIS weights and Russian
-   roulette are treated as constants (detached), pdfs use ns_sample.
+   roulette are treated as constants (detached), pdfs use ns_sample. A forward pass of the
+   wavefront renderer supplies each sample's radiance; the paths are then replayed once.
    device_ptrs != 0: all three arrays are device pointers, else host pointers. */
 int solhip_backward(solhip_scene* scene, const sol_pt_config* cfg, uint32_t width,
                     uint32_t height, uint64_t sample_index, uint32_t sample_count,
