@@ -52,7 +52,6 @@ def check_structure(scene, built):
     v = np.array([[list(t.v0), list(t.v1), list(t.v2)] for t in tris], np.float32)
     np.testing.assert_array_equal(v, pos[idx[prim]])
     # shading records equal the host flattening's record of the same triangle
-    host = sol_amd.Context  # noqa: F841  (the host records come from the scene's own layout)
     up = scene.gpu_desc()
     hp = np.array([up.tris[i].prim for i in range(n)])
     hshade = {int(p): bytes(up.shade[i]) for i, p in enumerate(hp)}
@@ -177,3 +176,21 @@ def test_gpu_build_quality_and_errors(ctx):
         ctx.build(scene, max_leaf_size=9)
     with pytest.raises(F.SolError):
         ctx.build(scene, search_radius=33)
+
+
+def test_batch_of_gpu_built_scenes_equals_uploaded_ones(ctx):
+    import torch
+    from sol_amd.workloads import BATCH_DEPTH, batch_specs
+    specs = batch_specs(count=6, seed=21, max_spheres=120, sides=(64, 96), spps=(1, 3))
+    cfg = sol_amd.PathTracerConfig(max_path_len=BATCH_DEPTH)
+    scenes = [sp.scene() for sp in specs]
+    out = []
+    for make in (ctx.upload, ctx.build):
+        gs = [make(sc) for sc in scenes]
+        bufs = [torch.zeros((3, sp.height, sp.width), dtype=torch.float32, device="cuda") for sp in specs]
+        torch.cuda.synchronize()
+        ctx.render_batch([(g, sp.width, sp.height, 0, sp.spp, b.data_ptr()) for sp, g, b in zip(specs, gs, bufs)], cfg)
+        out.append([b.cpu().numpy() for b in bufs])
+    for a, b in zip(*out):
+        assert a.any()
+        np.testing.assert_array_equal(a, b)
