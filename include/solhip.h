@@ -86,10 +86,10 @@ typedef struct solhip_tuning {
     uint32_t tile_w;        /* batch: tile shape of the global work queue                    */
     uint32_t tile_h;
     uint32_t graph_iters;   /* bounce-loop iterations per captured graph                     */
-    uint32_t bvh_width;     /* 2 or 4: node layout of the wavefront and intersection traversal
-                               (4 needs the scene's 4-wide layout; scenes without one use the
-                               binary). The megakernel and the adjoint always use the binary
-                               layout, their faster one.                                    */
+    uint32_t bvh_width;     /* 2 or 4: node layout of the traversal in the wavefront renderer,
+                               the megakernel and the intersection kernel (4 needs the scene's
+                               4-wide layout; scenes without one use the binary). The adjoint
+                               replay uses the binary layout.                               */
 } solhip_tuning;
 
 /* Counters accumulated by the context since the last solhip_reset_stats. */

@@ -43,19 +43,18 @@ __global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol
 
 // Pixels are handed out in 8x8 tiles (one tile per wave) so that primary rays of a wave are
 // coherent.
-// The megakernel traverses the binary layout: with whole paths in one kernel its registers bound
-// its occupancy, and the 4-wide traversal's extra registers made it slower (49.7 vs 58.6 ms per
-// 1080p 16-spp frame of the sphere field on MI355X). Three waves per SIMD need at most 168 VGPRs;
-// the compiler is held to that (the mixed diffuse-Phong BSDF took the unconstrained build to 170
-// VGPRs and two waves, 23 % slower).
-template <int BLOCK, int LDS>
+// WIDE: the 4-wide layout with leaf postponing (the trace kernel's traversal), 808 vs 685
+// Msamples/s for the binary one on the 1080p sphere field. With whole paths in one kernel its
+// registers bound its occupancy: three waves per SIMD need at most 168 VGPRs and the compiler is
+// held to that (left free it takes 170-173 VGPRs and two waves, 20 % slower).
+template <int BLOCK, int LDS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void megakernel(SceneView s, sol_pt_config cfg, uint32_t w, uint32_t h,
                                                     uint64_t sample_index, uint32_t sample_count,
                                                     float* __restrict__ accum, unsigned long long* counter,
                                                     unsigned long long* stats, TraversalScratch sc) {
     __shared__ float4 lds_nodes[kTopNodesMax * 4];
     __shared__ int lds_stack[LDS * BLOCK];
-    load_top_nodes<BLOCK>(s, lds_nodes);
+    stage_top_nodes<BLOCK, WIDE>(s, lds_nodes);
     Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
     const uint32_t tiles_x = (w + 7) / 8, tiles_y = (h + 7) / 8;
     const unsigned long long total = static_cast<unsigned long long>(tiles_x) * tiles_y * 64;
@@ -75,7 +74,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
             rng.seed(pixel_seed(sample_index + i, x, y));
             V3 org, dir;
             camera_ray(s.camera, x, y, w, h, rng, org, dir);
-            color = color + trace_path<BLOCK, LDS>(s, cfg, lds_nodes, st, rng, org, dir, n_ext, n_shadow);
+            color = color + trace_path<BLOCK, LDS, WIDE>(s, cfg, lds_nodes, st, rng, org, dir, n_ext, n_shadow);
         }
         n_samples += sample_count;
         size_t p = static_cast<size_t>(y) * w + x;
@@ -112,10 +111,14 @@ hipError_t intersect_impl(const SceneView& s, const sol_ray* rays, uint64_t n, b
 template <int BLOCK, int LDS>
 hipError_t megakernel_impl(const SceneView& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si,
                            uint32_t sc_count, float* accum, unsigned long long* counter, unsigned long long* stats,
-                           const TraversalScratch& sc, int grid, hipStream_t stream) {
+                           const TraversalScratch& sc, int grid, bool wide, hipStream_t stream) {
     (void)hipGetLastError();
-    hipLaunchKernelGGL((megakernel<BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, cfg, w, h, si, sc_count, accum,
-                       counter, stats, sc);
+    if (wide)
+        hipLaunchKernelGGL((megakernel<BLOCK, LDS, true>), dim3(grid), dim3(BLOCK), 0, stream, s, cfg, w, h, si, sc_count,
+                           accum, counter, stats, sc);
+    else
+        hipLaunchKernelGGL((megakernel<BLOCK, LDS, false>), dim3(grid), dim3(BLOCK), 0, stream, s, cfg, w, h, si,
+                           sc_count, accum, counter, stats, sc);
     return hipGetLastError();
 }
 
@@ -147,7 +150,7 @@ hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint3
                              unsigned long long* work_counter, StatCounters stats, const TraversalScratch& scratch,
                              const LaunchShape& shape, hipStream_t stream) {
     SOLHIP_DISPATCH(megakernel_impl, s, cfg, width, height, sample_index, sample_count, accum, work_counter, stats,
-                    scratch, shape.grid, stream)
+                    scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream)
 }
 
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t stream) {
