@@ -177,6 +177,7 @@ struct RayPre {
     V3 org, dir;
     V3 inv;          // 1 / dir, with |dir| < 1e-20 replaced by +-1e-20 (no 0 * inf)
     V3 off_lo, off_hi;  // slab distance of plane p on axis a: fma(p, inv.a, off_lo.a or off_hi.a)
+    V3 off_near, off_far;   // the same offsets by role: on axis a the near plane is lo if inv.a >= 0, else hi
     int kx, ky, kz;
     float sx, sy, sz;
 };
@@ -191,16 +192,17 @@ SOL_HD RayPre make_ray_pre(V3 org, V3 dir) {
     // fma(p, inv, -org * inv) = (p - org) * inv up to one rounding and the rounding of org * inv,
     // an absolute error of at most 2^-24 |org * inv|: the near plane's distance is lowered and the
     // far plane's raised by twice that (per axis; which plane is near depends on the sign of inv).
-    auto offsets = [](float o, float inv, float& lo, float& hi) {
+    auto offsets = [](float o, float inv, float& lo, float& hi, float& near_off, float& far_off) {
         float oi = o * inv;
         float e = 0x1p-23f * fabsf(oi);
-        float near_off = -oi - e, far_off = -oi + e;
+        near_off = -oi - e;
+        far_off = -oi + e;
         lo = inv >= 0.0f ? near_off : far_off;
         hi = inv >= 0.0f ? far_off : near_off;
     };
-    offsets(org.x, r.inv.x, r.off_lo.x, r.off_hi.x);
-    offsets(org.y, r.inv.y, r.off_lo.y, r.off_hi.y);
-    offsets(org.z, r.inv.z, r.off_lo.z, r.off_hi.z);
+    offsets(org.x, r.inv.x, r.off_lo.x, r.off_hi.x, r.off_near.x, r.off_far.x);
+    offsets(org.y, r.inv.y, r.off_lo.y, r.off_hi.y, r.off_near.y, r.off_far.y);
+    offsets(org.z, r.inv.z, r.off_lo.z, r.off_hi.z, r.off_near.z, r.off_far.z);
     float ax = fabsf(dir.x), ay = fabsf(dir.y), az = fabsf(dir.z);
     int kz = 0;
     float m = ax;
@@ -260,6 +262,27 @@ SOL_HD bool intersect_box(const RayPre& r, float lox, float hix, float loy, floa
     float tz0 = fmaf(loz, r.inv.z, r.off_lo.z), tz1 = fmaf(hiz, r.inv.z, r.off_hi.z);
     float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
     float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * kRobustFar;
+    f = fminf(f, tmax * kIntervalHigh);
+    tnear = n;
+    return n <= f;
+}
+
+// Byte offset of the near-plane row of an axis in a sol_gpu_node4 whose lo row is at `lo_row`: the
+// lo row if inv >= 0, else the hi row 16 bytes on (inv is never zero). The far row is at near ^ 16.
+SOL_HD uint32_t near_row(float inv, uint32_t lo_row) { return lo_row | ((__float_as_uint(inv) >> 27) & 16u); }
+
+// The slab test with the near and far plane of every axis given (the plane rows selected by the
+// sign of inv, near_row). For a box with lo <= hi these are the planes intersect_box picks
+// with min and max - fma is monotonic in the plane and near_off <= far_off - so the decision and
+// the entry distance are the same, for six operations fewer. (They could differ only where a slab
+// distance is NaN, which needs |org| near 1e18 and a direction component below 1e-20.)
+SOL_HD bool intersect_box_nf(const RayPre& r, float nx, float fx, float ny, float fy, float nz, float fz, float tmax,
+                             float& tnear) {
+    float tx0 = fmaf(nx, r.inv.x, r.off_near.x), tx1 = fmaf(fx, r.inv.x, r.off_far.x);
+    float ty0 = fmaf(ny, r.inv.y, r.off_near.y), ty1 = fmaf(fy, r.inv.y, r.off_far.y);
+    float tz0 = fmaf(nz, r.inv.z, r.off_near.z), tz1 = fmaf(fz, r.inv.z, r.off_far.z);
+    float n = fmaxf(fmaxf(tx0, ty0), fmaxf(tz0, 0.0f));
+    float f = fminf(fminf(tx1, ty1), tz1) * kRobustFar;
     f = fminf(f, tmax * kIntervalHigh);
     tnear = n;
     return n <= f;

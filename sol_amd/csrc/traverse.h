@@ -292,8 +292,10 @@ SOL_DEV void load_top_nodes4(const SceneView& s, float4* lds_nodes) {
     __syncthreads();
 }
 
+// A 4-wide node as one ray sees it: per axis the row of near planes and the row of far planes of
+// the four children (which of lo / hi is near depends on the ray's direction), and the children.
 struct Node4 {
-    float4 lo_x, hi_x, lo_y, hi_y, lo_z, hi_z;
+    float4 near_x, far_x, near_y, far_y, near_z, far_z;
     int4 child;
 };
 
@@ -312,19 +314,27 @@ struct FetchUniform4 {
           lds(lds_nodes),
           top(static_cast<int>(s.top_nodes4 < static_cast<uint32_t>(kTopNodes4Max) ? s.top_nodes4 : kTopNodes4Max)) {}
 
-    SOL_DEV void node(int cur, Node4& n) const {
+    // The rows are picked by address: a node is 128 bytes at a multiple of 128, so the near row
+    // of an axis is at o | near and the far row at (o | near) ^ 16.
+    SOL_DEV void node(int cur, const RayPre& r, Node4& n) const {
+        const uint32_t o = static_cast<uint32_t>(cur) * 128u;
+        const uint32_t ax = near_row(r.inv.x, o), ay = near_row(r.inv.y, o | 32u), az = near_row(r.inv.z, o | 64u);
         if (cur < top) {
-            const float4* p = lds + cur * 8;
-            n.lo_x = p[0]; n.hi_x = p[1]; n.lo_y = p[2]; n.hi_y = p[3]; n.lo_z = p[4]; n.hi_z = p[5];
-            n.child = as_int4(p[6]);
+            const char* p = reinterpret_cast<const char*>(lds);
+            n.near_x = *reinterpret_cast<const float4*>(p + ax);
+            n.far_x = *reinterpret_cast<const float4*>(p + (ax ^ 16u));
+            n.near_y = *reinterpret_cast<const float4*>(p + ay);
+            n.far_y = *reinterpret_cast<const float4*>(p + (ay ^ 16u));
+            n.near_z = *reinterpret_cast<const float4*>(p + az);
+            n.far_z = *reinterpret_cast<const float4*>(p + (az ^ 16u));
+            n.child = as_int4(*reinterpret_cast<const float4*>(p + o + 96));
         } else {
-            uint32_t o = static_cast<uint32_t>(cur) * 128u;
-            n.lo_x = FetchUniform::ld4(nodes, o);
-            n.hi_x = FetchUniform::ld4(nodes, o + 16);
-            n.lo_y = FetchUniform::ld4(nodes, o + 32);
-            n.hi_y = FetchUniform::ld4(nodes, o + 48);
-            n.lo_z = FetchUniform::ld4(nodes, o + 64);
-            n.hi_z = FetchUniform::ld4(nodes, o + 80);
+            n.near_x = FetchUniform::ld4(nodes, ax);
+            n.far_x = FetchUniform::ld4(nodes, ax ^ 16u);
+            n.near_y = FetchUniform::ld4(nodes, ay);
+            n.far_y = FetchUniform::ld4(nodes, ay ^ 16u);
+            n.near_z = FetchUniform::ld4(nodes, az);
+            n.far_z = FetchUniform::ld4(nodes, az ^ 16u);
             n.child = as_int4(FetchUniform::ld4(nodes, o + 96));
         }
     }
@@ -343,11 +353,12 @@ struct FetchGlobal4 {
     gptr nodes, tris;
 
     SOL_DEV explicit FetchGlobal4(const SceneView& s) : nodes((gptr)(s.nodes4)), tris((gptr)(s.tris)) {}
-    SOL_DEV void node(int cur, Node4& n) const {
+    SOL_DEV void node(int cur, const RayPre& r, Node4& n) const {
         gptr p = nodes + cur * 8;
-        n.lo_x = FetchGlobal::f4(p[0]); n.hi_x = FetchGlobal::f4(p[1]);
-        n.lo_y = FetchGlobal::f4(p[2]); n.hi_y = FetchGlobal::f4(p[3]);
-        n.lo_z = FetchGlobal::f4(p[4]); n.hi_z = FetchGlobal::f4(p[5]);
+        const uint32_t ix = near_row(r.inv.x, 0u) >> 4, iy = near_row(r.inv.y, 32u) >> 4, iz = near_row(r.inv.z, 64u) >> 4;
+        n.near_x = FetchGlobal::f4(p[ix]); n.far_x = FetchGlobal::f4(p[ix ^ 1u]);
+        n.near_y = FetchGlobal::f4(p[iy]); n.far_y = FetchGlobal::f4(p[iy ^ 1u]);
+        n.near_z = FetchGlobal::f4(p[iz]); n.far_z = FetchGlobal::f4(p[iz ^ 1u]);
         n.child = as_int4(FetchGlobal::f4(p[6]));
     }
     SOL_DEV void tri(uint32_t i, float4& p0, float4& p1, float4& p2) const {
@@ -397,12 +408,12 @@ SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, flo
     while (__ballot(more && leaf == 0) != 0) {
         if (more && ts.node >= 0) {
             Node4 n;
-            f.node(ts.node, n);
+            f.node(ts.node, r, n);
             float t0, t1, t2, t3;
-            bool h0 = intersect_box(r, n.lo_x.x, n.hi_x.x, n.lo_y.x, n.hi_y.x, n.lo_z.x, n.hi_z.x, tmin, ts.tmax, t0);
-            bool h1 = intersect_box(r, n.lo_x.y, n.hi_x.y, n.lo_y.y, n.hi_y.y, n.lo_z.y, n.hi_z.y, tmin, ts.tmax, t1);
-            bool h2 = intersect_box(r, n.lo_x.z, n.hi_x.z, n.lo_y.z, n.hi_y.z, n.lo_z.z, n.hi_z.z, tmin, ts.tmax, t2);
-            bool h3 = intersect_box(r, n.lo_x.w, n.hi_x.w, n.lo_y.w, n.hi_y.w, n.lo_z.w, n.hi_z.w, tmin, ts.tmax, t3);
+            bool h0 = intersect_box_nf(r, n.near_x.x, n.far_x.x, n.near_y.x, n.far_y.x, n.near_z.x, n.far_z.x, ts.tmax, t0);
+            bool h1 = intersect_box_nf(r, n.near_x.y, n.far_x.y, n.near_y.y, n.far_y.y, n.near_z.y, n.far_z.y, ts.tmax, t1);
+            bool h2 = intersect_box_nf(r, n.near_x.z, n.far_x.z, n.near_y.z, n.far_y.z, n.near_z.z, n.far_z.z, ts.tmax, t2);
+            bool h3 = intersect_box_nf(r, n.near_x.w, n.far_x.w, n.near_y.w, n.far_y.w, n.near_z.w, n.far_z.w, ts.tmax, t3);
             int c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
             h0 = h0 && c0 != SOL_GPU_EMPTY;
             h1 = h1 && c1 != SOL_GPU_EMPTY;
