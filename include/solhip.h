@@ -87,9 +87,9 @@ typedef struct solhip_tuning {
     uint32_t tile_h;
     uint32_t graph_iters;   /* bounce-loop iterations per captured graph                     */
     uint32_t bvh_width;     /* 2 or 4: node layout of the traversal in the wavefront renderer,
-                               the megakernel and the intersection kernel (4 needs the scene's
-                               4-wide layout; scenes without one use the binary). The adjoint
-                               replay uses the binary layout.                               */
+                               the megakernel, the adjoint replay and the intersection kernel
+                               (4 needs the scene's 4-wide layout; scenes without one use the
+                               binary).                                                     */
 } solhip_tuning;
 
 /* Counters accumulated by the context since the last solhip_reset_stats. */

@@ -76,7 +76,7 @@ struct GradSink {
 };
 
 // Replays one sample whose radiance is C, with the same operations as trace_path.
-template <int BLOCK, int LDS>
+template <int BLOCK, int LDS, bool WIDE>
 SOL_DEV void replay_path(const SceneView& s, const sol_pt_config& cfg, const float4* lds_nodes, const Stack& st,
                          Pcg& rng, V3 org, V3 dir, V3 C, V3 delta, const GradSink& g, unsigned& n_ext,
                          unsigned& n_shadow) {
@@ -89,7 +89,7 @@ SOL_DEV void replay_path(const SceneView& s, const sol_pt_config& cfg, const flo
         RayPre r = make_ray_pre(org, dir);
         HitRecord hit;
         ++n_ext;
-        if (!traverse<false, BLOCK, LDS>(s, lds_nodes, st, r, tmin, FLT_MAX, hit)) break;
+        if (!trace_ray<false, BLOCK, LDS, WIDE>(s, lds_nodes, st, r, tmin, FLT_MAX, hit)) break;
         Surface sf = decode_hit(s, org, dir, hit.slot, hit.t, hit.u, hit.v);
         if (sf.light >= 0 && sf.front) {
             V3 Le;
@@ -108,7 +108,8 @@ SOL_DEV void replay_path(const SceneView& s, const sol_pt_config& cfg, const flo
                 RayPre sr = make_ray_pre(sf.point, ns.shadow_dir);
                 HitRecord unused;
                 ++n_shadow;
-                if (!traverse<true, BLOCK, LDS>(s, lds_nodes, st, sr, cfg.ray_offset, 1.0f - cfg.ray_offset, unused)) {
+                if (!trace_ray<true, BLOCK, LDS, WIDE>(s, lds_nodes, st, sr, cfg.ray_offset, 1.0f - cfg.ray_offset,
+                                                       unused)) {
                     g.material(m, sf.mat, ns.rc, ns.f, delta * ns.contrib);
                     g.light(ns.light, delta * (T * ns.f * ns.scale));
                     R = R - ns.contrib;
@@ -131,7 +132,7 @@ SOL_DEV void replay_path(const SceneView& s, const sol_pt_config& cfg, const flo
 }
 
 // One lane per pixel in 8x8 tiles (as the megakernel); pixels with a zero adjoint are skipped.
-template <int BLOCK, int LDS>
+template <int BLOCK, int LDS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_config cfg, uint32_t w, uint32_t h,
                                                         uint64_t sample_index, uint32_t sample_count,
                                                         const float4* __restrict__ radiance,
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_conf
     const bool in_lds = n_grad <= kGradLds;
     if (in_lds)
         for (int i = threadIdx.x; i < n_grad; i += BLOCK) lds_grad[i] = 0.0f;
-    load_top_nodes<BLOCK>(s, lds_nodes);
+    stage_top_nodes<BLOCK, WIDE>(s, lds_nodes);
     __syncthreads();
     Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
     GradSink g{in_lds ? lds_grad : nullptr, d_mat, d_light, mat_floats};
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_conf
             rng.seed(pixel_seed(sample_index + i, x, y));
             V3 org, dir;
             camera_ray(s.camera, x, y, w, h, rng, org, dir);
-            replay_path<BLOCK, LDS>(s, cfg, lds_nodes, st, rng, org, dir, V3{c.x, c.y, c.z}, delta, g, n_ext, n_shadow);
+            replay_path<BLOCK, LDS, WIDE>(s, cfg, lds_nodes, st, rng, org, dir, V3{c.x, c.y, c.z}, delta, g, n_ext, n_shadow);
         }
         n_samples += sample_count;
     }
@@ -195,10 +196,14 @@ hipError_t adjoint_impl(const SceneView& s, const sol_pt_config& cfg, uint32_t w
                         uint32_t count, const float4* radiance, const float* d_image, uint32_t num_materials,
                         float* d_mat, float* d_light,
                         unsigned long long* counter, unsigned long long* stats, const TraversalScratch& sc, int grid,
-                        hipStream_t stream) {
+                        bool wide, hipStream_t stream) {
     (void)hipGetLastError();
-    hipLaunchKernelGGL((adjoint_kernel<BLOCK, LDS>), dim3(grid), dim3(BLOCK), 0, stream, s, cfg, w, h, si, count,
-                       radiance, d_image, num_materials, d_mat, d_light, counter, stats, sc);
+    if (wide)
+        hipLaunchKernelGGL((adjoint_kernel<BLOCK, LDS, true>), dim3(grid), dim3(BLOCK), 0, stream, s, cfg, w, h, si,
+                           count, radiance, d_image, num_materials, d_mat, d_light, counter, stats, sc);
+    else
+        hipLaunchKernelGGL((adjoint_kernel<BLOCK, LDS, false>), dim3(grid), dim3(BLOCK), 0, stream, s, cfg, w, h, si,
+                           count, radiance, d_image, num_materials, d_mat, d_light, counter, stats, sc);
     return hipGetLastError();
 }
 
@@ -210,15 +215,15 @@ hipError_t launch_adjoint(const SceneView& s, const sol_pt_config& cfg, uint32_t
                           float* d_mat, float* d_light, unsigned long long* work_counter, StatCounters stats,
                           const TraversalScratch& scratch, const LaunchShape& shape, hipStream_t stream) {
     switch (shape.block * 100 + shape.lds_stack) {
-        case 6408: return adjoint_impl<64, 8>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 6416: return adjoint_impl<64, 16>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 6432: return adjoint_impl<64, 32>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 12808: return adjoint_impl<128, 8>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 12816: return adjoint_impl<128, 16>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 12832: return adjoint_impl<128, 32>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 25608: return adjoint_impl<256, 8>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 25616: return adjoint_impl<256, 16>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
-        case 25632: return adjoint_impl<256, 32>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, stream);
+        case 6408: return adjoint_impl<64, 8>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream);
+        case 6416: return adjoint_impl<64, 16>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream);
+        case 6432: return adjoint_impl<64, 32>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream);
+        case 12808: return adjoint_impl<128, 8>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream);
+        case 12816: return adjoint_impl<128, 16>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream);
+        case 12832: return adjoint_impl<128, 32>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream);
+        case 25608: return adjoint_impl<256, 8>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream);
+        case 25616: return adjoint_impl<256, 16>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream);
+        case 25632: return adjoint_impl<256, 32>(s, cfg, width, height, sample_index, sample_count, radiance, d_image, num_materials, d_mat, d_light, work_counter, stats, scratch, shape.grid, shape.wide && s.nodes4 != nullptr, stream);
         default: return hipErrorInvalidValue;
     }
 }
