@@ -468,16 +468,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         int dest = -1;
         bool dead = false;
         uint32_t slot = 0;
-        float4 rec0, rec1, rec2, rec3, e1;
-        unsigned a = 0;
+        float4 rec0, rec1, rec2, rec3, e0, e1, h;
         if (valid) {
-            a = eqv.entry(i);
+            // the whole entry and its hit at once (a hole's are in bounds and unused): one memory
+            // round trip fewer before the shading record, which depends on the hit
+            const unsigned a = eqv.entry(i);
+            e0 = q[2 * a];
             e1 = q[2 * a + 1];
+            h = b.hits[i];
             slot = as_u(e1.w);
             valid = slot != kHole;
         }
         if (valid) {
-            float4 e0 = q[2 * a], h = b.hits[i];
             int tri = static_cast<int>(as_u(h.x));
             if (tri < 0) {
                 dead = true;
