@@ -209,14 +209,21 @@ __global__ void k_merge(const int* C, const int* nn, const unsigned long long* f
 
 // Bottom-up SAH pass: count and cost of every node; collapse[node] = 1 if the subtree becomes a
 // leaf. Triangles (nodes < n) are leaves of one triangle.
+// The same pass prepares the 4-wide collapse (the host layout's rule): each inner node becomes a
+// 4-wide node or is opened into its parent's so that the summed half area of the 4-wide nodes is
+// least. wcost[4 x + k - 1] is the least such sum for subtree x in at most k slots of its parent
+// (0 for a leaf; for k = 1 x is a 4-wide node, area(x) + fill(x, 4)); byte k - 2 of wsplit[x] is
+// how many of k slots the left subtree takes in the best fill(x, k).
 __global__ void k_collapse(uint32_t n, const float4* nlo, const float4* nhi, const int* left, const int* right,
                            const int* parent, unsigned* visits, unsigned* count, float* cost, unsigned* collapse,
-                           float ct, float ci, uint32_t max_leaf) {
+                           float* wcost, unsigned* wsplit, float ct, float ci, uint32_t max_leaf) {
     uint32_t t = blockIdx.x * kBB + threadIdx.x;
     if (t >= n) return;
     count[t] = 1;
     cost[t] = ci * half_area(nlo[t], nhi[t]);
     collapse[t] = 1;
+    for (int k = 0; k < 4; ++k) wcost[4 * size_t(t) + k] = 0.0f;
+    wsplit[t] = 0;
     __threadfence();
     int node = parent[t];
     while (node >= 0) {
@@ -234,6 +241,38 @@ __global__ void k_collapse(uint32_t n, const float4* nlo, const float4* nhi, con
         count[node] = cnt;
         cost[node] = leaf ? as_leaf : split;
         collapse[node] = leaf ? 1u : 0u;
+        float* wc = wcost + 4 * size_t(node);
+        if (leaf) {
+            wc[0] = wc[1] = wc[2] = wc[3] = 0.0f;
+            wsplit[node] = 0;
+        } else {
+            float A[4], B[4];
+            for (int k = 0; k < 4; ++k) {
+                A[k] = __hip_atomic_load(&wcost[4 * size_t(a) + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                B[k] = __hip_atomic_load(&wcost[4 * size_t(b) + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            float fill[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            unsigned sp = 0;
+            for (int k = 2; k <= 4; ++k) {
+                float best = HUGE_VALF;
+                int bj = 1;
+                for (int j = 1; j < k; ++j) {
+                    const float c = A[j - 1] + B[k - j - 1];
+                    if (c < best) {
+                        best = c;
+                        bj = j;
+                    }
+                }
+                fill[k] = best;
+                sp |= static_cast<unsigned>(bj) << (8 * (k - 2));
+            }
+            const float wide = area + fill[4];
+            wc[0] = wide;
+            wc[1] = fminf(wide, fill[2]);
+            wc[2] = fminf(wide, fill[3]);
+            wc[3] = fminf(wide, fill[4]);
+            wsplit[node] = sp;
+        }
         __threadfence();
         node = parent[node];
     }
@@ -316,40 +355,41 @@ __global__ void k_level_write(const int* F, uint32_t size, uint32_t n, uint32_t 
     out[level_base + k] = g;
 }
 
-// The up-to-four children of the 4-wide node made from binary inner node `node`: open the
-// inner child of largest half area until there are four (the host layout's rule).
-SOL_DEV int wide_children(int node, uint32_t n, const float4* nlo, const float4* nhi, const int* left,
-                          const int* right, const unsigned* collapse, int kids[4]) {
-    kids[0] = left[node];
-    kids[1] = right[node];
-    int cnt = 2;
-    while (cnt < 4) {
-        int best = -1;
-        float best_area = -1.0f;
-        for (int k = 0; k < cnt; ++k) {
-            if (!inner_node(kids[k], n, collapse)) continue;
-            float a = half_area(nlo[kids[k]], nhi[kids[k]]);
-            if (a > best_area) {
-                best_area = a;
-                best = k;
-            }
+// The up-to-four children of the 4-wide node made from binary inner node `node`, left to right,
+// as k_collapse's costs choose them: subtree x placed in k slots is opened into its two subtrees
+// (split by wsplit) when that costs less than making x a 4-wide node. The slots of the pending
+// subtrees and the children placed always sum to four, so at most four are pending.
+SOL_DEV int wide_children(int node, uint32_t n, const int* left, const int* right, const unsigned* collapse,
+                          const float* wcost, const unsigned* wsplit, int kids[4]) {
+    int px[4], pk[4];
+    int sp = 0, cnt = 0;
+    const int j4 = static_cast<int>((wsplit[node] >> 16) & 0xFFu);
+    px[sp] = right[node];
+    pk[sp++] = 4 - j4;
+    px[sp] = left[node];
+    pk[sp++] = j4;
+    while (sp > 0) {
+        --sp;
+        const int x = px[sp], k = pk[sp];
+        if (!inner_node(x, n, collapse) || k == 1 || !(wcost[4 * size_t(x) + k - 1] < wcost[4 * size_t(x)])) {
+            kids[cnt++] = x;
+            continue;
         }
-        if (best < 0) break;
-        const int open = kids[best];
-        for (int k = cnt; k > best + 1; --k) kids[k] = kids[k - 1];
-        kids[best] = left[open];
-        kids[best + 1] = right[open];
-        ++cnt;
+        const int j = static_cast<int>((wsplit[x] >> (8 * (k - 2))) & 0xFFu);
+        px[sp] = right[x];
+        pk[sp++] = k - j;
+        px[sp] = left[x];
+        pk[sp++] = j;
     }
     return cnt;
 }
 
-__global__ void k_wide_count(const int* F, uint32_t size, uint32_t n, const float4* nlo, const float4* nhi,
-                             const int* left, const int* right, const unsigned* collapse, unsigned* cnt) {
+__global__ void k_wide_count(const int* F, uint32_t size, uint32_t n, const int* left, const int* right,
+                             const unsigned* collapse, const float* wcost, const unsigned* wsplit, unsigned* cnt) {
     uint32_t k = blockIdx.x * kBB + threadIdx.x;
     if (k >= size) return;
     int kids[4];
-    int c = wide_children(F[k], n, nlo, nhi, left, right, collapse, kids);
+    int c = wide_children(F[k], n, left, right, collapse, wcost, wsplit, kids);
     unsigned inner = 0;
     for (int i = 0; i < c; ++i) inner += inner_node(kids[i], n, collapse) ? 1u : 0u;
     cnt[k] = inner;
@@ -357,14 +397,15 @@ __global__ void k_wide_count(const int* F, uint32_t size, uint32_t n, const floa
 
 __global__ void k_wide_write(const int* F, uint32_t size, uint32_t n, uint32_t level_base, uint32_t next_base,
                              const unsigned* offs, const unsigned* cnt, const float4* nlo, const float4* nhi,
-                             const int* left, const int* right, const unsigned* collapse, const int32_t* leafref,
-                             sol_gpu_node4* out, int* F_next, unsigned* totals) {
+                             const int* left, const int* right, const unsigned* collapse, const float* wcost,
+                             const unsigned* wsplit, const int32_t* leafref, sol_gpu_node4* out, int* F_next,
+                             unsigned* totals) {
     uint32_t k = blockIdx.x * kBB + threadIdx.x;
     if (k >= size) return;
     uint32_t off = offs[k];
     if (k == size - 1) totals[0] = off + cnt[k];
     int kids[4];
-    const int c = wide_children(F[k], n, nlo, nhi, left, right, collapse, kids);
+    const int c = wide_children(F[k], n, left, right, collapse, wcost, wsplit, kids);
     sol_gpu_node4 g;
     for (int i = 0; i < 4; ++i) {
         if (i >= c) {
@@ -512,8 +553,8 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     const size_t nodes_cap = 2 * size_t(n);
     Buf<float4> nlo, nhi;
     Buf<int> left, right, parent, C0, C1, nn;
-    Buf<unsigned> cb, totals, visits, count, collapse, ids0, ids1;
-    Buf<float> cost;
+    Buf<unsigned> cb, totals, visits, count, collapse, wsplit, ids0, ids1;
+    Buf<float> cost, wcost;
     Buf<unsigned long long> keys0, keys1, flags, offs;
     BUILD_TRY(nlo.alloc(nodes_cap));
     BUILD_TRY(nhi.alloc(nodes_cap));
@@ -529,6 +570,8 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     BUILD_TRY(count.alloc(nodes_cap));
     BUILD_TRY(collapse.alloc(nodes_cap));
     BUILD_TRY(cost.alloc(nodes_cap));
+    BUILD_TRY(wcost.alloc(4 * nodes_cap));
+    BUILD_TRY(wsplit.alloc(nodes_cap));
     BUILD_TRY(ids0.alloc(n));
     BUILD_TRY(ids1.alloc(n));
     BUILD_TRY(keys0.alloc(n));
@@ -584,7 +627,8 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     const uint32_t max_leaf = cfg.max_leaf_size;
     BUILD_TRY(hipMemsetAsync(visits.p, 0, sizeof(unsigned) * nodes_cap, st));
     hipLaunchKernelGGL(k_collapse, dim3(grid_for(n)), dim3(kBB), 0, st, n, nlo.p, nhi.p, left.p, right.p, parent.p,
-                       visits.p, count.p, cost.p, collapse.p, cfg.traversal_cost, cfg.intersection_cost, max_leaf);
+                       visits.p, count.p, cost.p, collapse.p, wcost.p, wsplit.p, cfg.traversal_cost,
+                       cfg.intersection_cost, max_leaf);
     BUILD_TRY(hipGetLastError());
     unsigned root_collapsed = 0;
     BUILD_TRY(hipMemcpyAsync(&root_collapsed, collapse.p + root, sizeof(unsigned), hipMemcpyDeviceToHost, st));
@@ -654,13 +698,13 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
         unsigned* cnt = reinterpret_cast<unsigned*>(flags.p);
         unsigned* cnt_off = reinterpret_cast<unsigned*>(offs.p);
         while (size > 0) {
-            hipLaunchKernelGGL(k_wide_count, dim3(grid_for(size)), dim3(kBB), 0, st, Fc, size, n, nlo.p, nhi.p, left.p,
-                               right.p, collapse.p, cnt);
+            hipLaunchKernelGGL(k_wide_count, dim3(grid_for(size)), dim3(kBB), 0, st, Fc, size, n, left.p, right.p,
+                               collapse.p, wcost.p, wsplit.p, cnt);
             BUILD_TRY(hipGetLastError());
             BUILD_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan32_bytes, cnt, cnt_off, static_cast<int>(size), st));
             hipLaunchKernelGGL(k_wide_write, dim3(grid_for(size)), dim3(kBB), 0, st, Fc, size, n, level_base,
-                               level_base + size, cnt_off, cnt, nlo.p, nhi.p, left.p, right.p, collapse.p, leafref.p,
-                               gnodes4.p, Fn, totals.p);
+                               level_base + size, cnt_off, cnt, nlo.p, nhi.p, left.p, right.p, collapse.p, wcost.p,
+                               wsplit.p, leafref.p, gnodes4.p, Fn, totals.p);
             BUILD_TRY(hipGetLastError());
             unsigned tot = 0;
             BUILD_TRY(hipMemcpyAsync(&tot, totals.p, sizeof(tot), hipMemcpyDeviceToHost, st));

@@ -54,9 +54,9 @@ struct GpuLayout {
 };
 GpuLayout to_gpu_layout(const Bvh& bvh, uint32_t top_levels);
 
-// The binary tree collapsed to 4-wide nodes: starting from a node's two children, the child
-// with the largest surface area that is an inner node is replaced by its two children until there
-// are four or only leaves remain. Top levels breadth-first, the rest depth-first.
+// The binary tree collapsed to 4-wide nodes: each inner binary node becomes a 4-wide node or is
+// opened into its parent's, chosen so that the summed surface area of the 4-wide nodes is least
+// (the leaves stay the binary tree's). Top levels breadth-first, the rest depth-first.
 struct GpuLayout4 {
     std::vector<sol_gpu_node4> nodes;
     uint32_t top_nodes = 0;

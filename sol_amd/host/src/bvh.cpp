@@ -5,7 +5,9 @@
 #include "sol/bvh.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <cmath>
 #include <deque>
 #include <unordered_map>
 #include <numeric>
@@ -444,26 +446,66 @@ GpuLayout4 to_gpu_layout4(const Bvh& bvh, uint32_t top_levels) {
         out.depth = 2;
         return out;
     }
-    // The children (binary node ids) of the 4-wide node made from binary inner node id.
-    auto collapse = [&](uint32_t id) {
-        std::vector<uint32_t> kids{bvh.nodes[id].index, bvh.nodes[id].index + 1};
-        while (kids.size() < 4) {
-            int best = -1;
-            float best_area = -1.0f;
-            for (size_t k = 0; k < kids.size(); ++k) {
-                const Node& nd = bvh.nodes[kids[k]];
-                if (nd.is_leaf()) continue;
-                float a = nd.box.half_area();
-                if (a > best_area) {
-                    best_area = a;
-                    best = static_cast<int>(k);
+    // Every inner binary node either becomes a 4-wide node or is opened into the 4-wide node of
+    // its parent. The leaves stay those of the binary tree, so the collapse only decides the
+    // summed area of the 4-wide nodes - the expected number of node visits of a ray, up to the
+    // root area - and it is chosen least (dynamic programming over the tree, bottom up):
+    //   fill(x, k) = min over j of cost(l, j) + cost(r, k - j)   x's two subtrees in k slots
+    //   wide(x)    = area(x) + fill(x, 4)                         x a 4-wide node
+    //   cost(x, k) = min(wide(x), fill(x, k)), 0 for a leaf       subtree x in at most k slots
+    const size_t n_nodes = bvh.nodes.size();
+    std::vector<std::array<double, 5>> cost(n_nodes), fill(n_nodes);
+    std::vector<std::array<uint8_t, 5>> split(n_nodes);   // best j of fill(x, k)
+    std::vector<double> wide(n_nodes, 0.0);
+    {
+        std::vector<std::pair<uint32_t, bool>> stack{{0u, false}};
+        while (!stack.empty()) {
+            auto [x, expanded] = stack.back();
+            stack.pop_back();
+            const Node& nd = bvh.nodes[x];
+            if (nd.is_leaf()) {
+                cost[x].fill(0.0);
+                continue;
+            }
+            if (!expanded) {
+                stack.push_back({x, true});
+                stack.push_back({nd.index, false});
+                stack.push_back({nd.index + 1, false});
+                continue;
+            }
+            const uint32_t l = nd.index, r = nd.index + 1;
+            for (int k = 2; k <= 4; ++k) {
+                fill[x][k] = HUGE_VAL;
+                for (int j = 1; j < k; ++j) {
+                    double c = cost[l][j] + cost[r][k - j];
+                    if (c < fill[x][k]) {
+                        fill[x][k] = c;
+                        split[x][k] = static_cast<uint8_t>(j);
+                    }
                 }
             }
-            if (best < 0) break;
-            uint32_t open = kids[best];
-            kids[best] = bvh.nodes[open].index;
-            kids.insert(kids.begin() + best + 1, bvh.nodes[open].index + 1);
+            wide[x] = static_cast<double>(nd.box.half_area()) + fill[x][4];
+            cost[x][0] = HUGE_VAL;
+            cost[x][1] = wide[x];
+            for (int k = 2; k <= 4; ++k) cost[x][k] = std::min(wide[x], fill[x][k]);
         }
+    }
+    // The children (binary node ids) of the 4-wide node made from binary inner node id.
+    auto collapse = [&](uint32_t id) {
+        std::vector<uint32_t> kids;
+        auto place = [&](auto&& self, uint32_t x, int k) -> void {
+            const Node& nd = bvh.nodes[x];
+            if (nd.is_leaf() || k == 1 || !(fill[x][k] < wide[x])) {
+                kids.push_back(x);
+                return;
+            }
+            const int j = split[x][k];
+            self(self, nd.index, j);
+            self(self, nd.index + 1, k - j);
+        };
+        const int j = split[id][4];
+        place(place, bvh.nodes[id].index, j);
+        place(place, bvh.nodes[id].index + 1, 4 - j);
         return kids;
     };
     // Order the 4-wide nodes (identified by their binary node id): top levels breadth-first, then
