@@ -13,7 +13,8 @@ Also reported:
                 triangle tests per ray of tests/golden/traversal_counts.json (counted once on the
                 same BVH and rays by tests/golden/make_traversal_counts.py). "traffic" is the HBM
                 bytes per launch read from the committed rocprofv3 counter summary of this
-                workload, if present. "copy_gbs" is a device-to-device copy timed here.
+                workload, if present; "dram_gbs" / "dram_frac" are that traffic over the measured
+                launch time, against the peak. "copy_gbs" is a device-to-device copy timed here.
   cpu_baseline  the CPU oracle (C++ restatement of sol's path tracer, OpenMP) on the host's cores,
                 on a bounded sample of the same frame (all pixels, fewer samples per pixel).
   intersect     the intersection microbenchmark: 1M-triangle soup, 4,194,304 closest-hit rays and
@@ -455,6 +456,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_trace", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic) if traffic else None, "copy_gbs": round(copy_gbs, 1),
+                     "dram_gbs": round(traffic / (launch_ms * 1e6), 1) if traffic else None,
+                     "dram_frac": round(traffic / (launch_ms * 1e6) / HBM_PEAK_GBS, 4) if traffic else None,
                      "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(launch_ms, 4),
                      "launches": int(launches), "profiled_ms_per_step": round(prof_s * 1e3, 2),
                      "kernel_ms_per_step": {"trace": round(wst.trace_ms / args.steps, 2),
