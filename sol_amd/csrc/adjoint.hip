@@ -139,7 +139,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_conf
                                                         const float* __restrict__ d_image, uint32_t num_materials,
                                                         float* d_mat, float* d_light, unsigned long long* counter,
                                                         unsigned long long* stats, TraversalScratch sc) {
-    __shared__ float4 lds_nodes[kTopNodesMax * 4];
+    __shared__ float4 lds_nodes[kTopLdsFloat4<WIDE>];
     __shared__ int lds_stack[LDS * BLOCK];
     __shared__ float lds_grad[kGradLds];
     const int mat_floats = static_cast<int>(num_materials) * 4;

@@ -13,7 +13,7 @@ template <bool ANY, int BLOCK, int LDS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol_ray* __restrict__ rays, uint64_t n,
                                                           sol_hit* __restrict__ hits, int32_t* __restrict__ occluded,
                                                           unsigned long long* counter, TraversalScratch sc) {
-    __shared__ float4 lds_nodes[kTopNodesMax * 4];
+    __shared__ float4 lds_nodes[kTopLdsFloat4<WIDE>];
     __shared__ int lds_stack[LDS * BLOCK];
     stage_top_nodes<BLOCK, WIDE>(s, lds_nodes);
     Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
                                                     uint64_t sample_index, uint32_t sample_count,
                                                     float* __restrict__ accum, unsigned long long* counter,
                                                     unsigned long long* stats, TraversalScratch sc) {
-    __shared__ float4 lds_nodes[kTopNodesMax * 4];
+    __shared__ float4 lds_nodes[kTopLdsFloat4<WIDE>];
     __shared__ int lds_stack[LDS * BLOCK];
     stage_top_nodes<BLOCK, WIDE>(s, lds_nodes);
     Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);

@@ -48,7 +48,7 @@ namespace {
 constexpr int kBB = 256;          // threads per block of the build kernels
 constexpr int kMaxRadius = 32;
 constexpr uint32_t kTopLevels = 6;    // binary levels laid out first (63 nodes at most)
-constexpr uint32_t kTopLevels4 = 3;   // 4-wide levels laid out first (21 nodes at most)
+constexpr uint32_t kTopLevels4 = 4;   // 4-wide levels laid out first (85 nodes at most)
 
 unsigned grid_for(size_t n) { return static_cast<unsigned>((n + kBB - 1) / kBB); }
 

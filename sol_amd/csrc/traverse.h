@@ -282,8 +282,14 @@ SOL_DEV bool traverse_with(const Fetch& f, const Stack& st, const RayPre& r, flo
 
 // ---- 4-wide BVH (sol_gpu_node4: 8 float4 per node) -------------------------------------------------
 
-// Maximum number of top 4-wide nodes copied to LDS (three levels, 21 nodes, 2.6 KB).
-constexpr int kTopNodes4Max = 21;
+// Maximum number of top 4-wide nodes copied to LDS (10 KB; the layouts put four levels, at most
+// 85 nodes, first). With a 16-entry stack a 256-lane workgroup of the traversal kernel then takes
+// 26.6 KB of LDS, so six such workgroups - six waves per SIMD - still fit a CU's 160 KB.
+constexpr int kTopNodes4Max = 80;
+
+// float4 entries of a kernel's LDS copy of the top levels of its layout.
+template <bool WIDE>
+constexpr int kTopLdsFloat4 = WIDE ? kTopNodes4Max * 8 : kTopNodesMax * 4;
 
 template <int BLOCK>
 SOL_DEV void load_top_nodes4(const SceneView& s, float4* lds_nodes) {

@@ -311,8 +311,7 @@ constexpr int kRefillLanes = 32;
 template <int BLOCK, int LDS, bool MULTI, bool WIDE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5 : 6))) void k_trace(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                  int cur, TraversalScratch sc) {
-    static_assert(kTopNodes4Max * 8 <= kTopNodesMax * 4, "LDS node cache too small for the 4-wide top levels");
-    __shared__ float4 lds_nodes[kTopNodesMax * 4];
+    __shared__ float4 lds_nodes[kTopLdsFloat4<WIDE>];
     __shared__ int lds_stack[LDS * BLOCK];
     const SegQueue eqv(b.counts, kQExtend0 + cur, b.cap_e), sqv(b.counts, kQShadow, b.cap_s);
     const unsigned n_ext = eqv.size();

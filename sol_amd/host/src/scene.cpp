@@ -291,8 +291,8 @@ FlatScene flatten_scene(const Scene& scene, uint32_t top_levels) {
     f.gpu_nodes = std::move(layout.nodes);
     f.top_nodes = layout.top_nodes;
     f.bvh_depth = layout.depth;
-    // 4-wide: three levels breadth-first (at most 21 nodes, the LDS-staged part)
-    bvh::GpuLayout4 layout4 = bvh::to_gpu_layout4(b, 3);
+    // 4-wide: four levels breadth-first (at most 85 nodes; the kernels stage up to 80 in LDS)
+    bvh::GpuLayout4 layout4 = bvh::to_gpu_layout4(b, 4);
     f.gpu_nodes4 = std::move(layout4.nodes);
     f.top_nodes4 = layout4.top_nodes;
     f.bvh4_depth = layout4.depth;

@@ -96,7 +96,7 @@ def check_structure(scene, built):
             else:
                 refs4.add(c)
     assert (reached == 1).all() and refs4 == leaves
-    assert info.top_nodes <= 63 and info.top_nodes4 <= 21
+    assert info.top_nodes <= 63 and info.top_nodes4 <= 85
 
 
 @pytest.mark.parametrize("name", ["cornell", "spheres", "soup", "water"])

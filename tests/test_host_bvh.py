@@ -121,7 +121,7 @@ def test_gpu_layout4_covers_every_leaf_and_bounds_its_subtrees(make):
         filled.append(used)
     assert np.all(covered == 1)
     assert referenced[0] == 0 and np.all(referenced[1:] == 1)
-    assert 1 <= d.top_nodes4 <= 21 and 2 <= d.bvh4_depth <= d.bvh_depth
+    assert 1 <= d.top_nodes4 <= 85 and 2 <= d.bvh4_depth <= d.bvh_depth
     if d.num_tris > 1000:
         assert np.mean(filled) > 3.0                    # collapsing fills most slots
 
