@@ -126,7 +126,7 @@ int solhip_create(int device, solhip_ctx** out) {
     HIP_TRY(hipMemset(ctx->scratch.overflow, 0, sizeof(int)));
     HIP_TRY(hipEventCreate(&ctx->ev_begin));
     HIP_TRY(hipEventCreate(&ctx->ev_end));
-    ctx->tuning = default_tuning();
+    ctx->tuning = default_tuning(ctx->info.total_memory);
     int st = ctx->ensure_scratch(persistent_grid(ctx->info.compute_units, 64) * 64);
     if (st) return st;
     *out = ctx.release();
@@ -167,7 +167,7 @@ int solhip_device_query(solhip_ctx* ctx, solhip_device_info* out) {
 
 int solhip_set_tuning(solhip_ctx* ctx, const solhip_tuning* t) {
     if (!ctx || !t) return fail(SOLHIP_E_INVALID, "null argument");
-    solhip_tuning d = default_tuning();
+    solhip_tuning d = default_tuning(ctx->info.total_memory);
     solhip_tuning n = *t;
     if (!n.block_size) n.block_size = d.block_size;
     if (!n.lds_stack) n.lds_stack = d.lds_stack;
@@ -178,7 +178,7 @@ int solhip_set_tuning(solhip_ctx* ctx, const solhip_tuning* t) {
     if (!n.bvh_width) n.bvh_width = d.bvh_width;
     if (n.bvh_width != 2 && n.bvh_width != 4) return fail(SOLHIP_E_INVALID, "bvh_width must be 2 or 4");
     if (!valid_shape_values(n.block_size, n.lds_stack)) return fail(SOLHIP_E_INVALID, "unsupported block size or LDS stack");
-    if (n.path_slots < 4096 || n.path_slots > (1u << 26)) return fail(SOLHIP_E_INVALID, "path_slots out of range");
+    if (n.path_slots < 4096 || n.path_slots > (1u << 27)) return fail(SOLHIP_E_INVALID, "path_slots out of range");
     if (n.graph_iters > 256) return fail(SOLHIP_E_INVALID, "graph_iters out of range");
     ctx->tuning = n;
     return SOLHIP_OK;

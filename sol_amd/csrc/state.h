@@ -37,11 +37,25 @@ struct DeviceBuffer {
 
 struct WavefrontState;  // defined in wavefront.hip
 
-inline solhip_tuning default_tuning() {
+// Device memory of the wavefront state per path slot: the path record and the hit (80 B) and the
+// queue segments (two extension queues 128 B, three shade queues 192 B, the shadow queue 48 B).
+constexpr unsigned long long kPathSlotBytes = 448;
+
+// Paths in flight by default: the largest power of two up to 2^27 whose state takes at most a
+// quarter of the device's memory (2^27 slots, 57 GB, on a 288 GB MI355X: a 1080p 64-spp frame is
+// then in flight at once; 2^24 slots where the memory size is not known).
+inline uint32_t default_path_slots(unsigned long long device_memory) {
+    if (device_memory == 0) return 1u << 24;
+    uint32_t s = 1u << 27;
+    while (s > 4096u && static_cast<unsigned long long>(s) * kPathSlotBytes > device_memory / 4) s >>= 1;
+    return s;
+}
+
+inline solhip_tuning default_tuning(unsigned long long device_memory) {
     solhip_tuning t{};
     t.block_size = 256;
     t.lds_stack = 16;
-    t.path_slots = 1u << 24;
+    t.path_slots = default_path_slots(device_memory);
     t.tile_w = 32;
     t.tile_h = 32;
     t.graph_iters = 16;

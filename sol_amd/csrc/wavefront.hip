@@ -476,10 +476,13 @@ int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
         return SOLHIP_OK;
     };
 
-    // Paths in flight above the work of one render only add idle slots.
+    // Paths in flight above the work of one render only add idle slots; none above the default's
+    // memory bound.
     const unsigned long long items = static_cast<unsigned long long>(plane) * budget_spp;
+    const uint32_t max_slots = default_path_slots(ctx.info.total_memory);
     std::vector<uint32_t> slot_cands;
-    for (uint32_t c : {1u << 21, 1u << 22, 1u << 23, 1u << 24}) {
+    for (uint32_t c : {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27}) {
+        if (c > max_slots) continue;
         uint32_t v = static_cast<uint32_t>(std::min<unsigned long long>(c, std::max<unsigned long long>(4096, items)));
         if (std::find(slot_cands.begin(), slot_cands.end(), v) == slot_cands.end()) slot_cands.push_back(v);
     }

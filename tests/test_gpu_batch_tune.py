@@ -47,7 +47,7 @@ def test_autotune_returns_valid_parameters_and_keeps_the_image(ctx):
     ctx.reset_stats()
     t = g.autotune(160, 90, budget_spp=2, config=cfg)
     assert t.block_size in (128, 256) and t.lds_stack in (8, 16)
-    assert 4096 <= t.path_slots <= (1 << 24) and t.graph_iters in (8, 16, 32)
+    assert 4096 <= t.path_slots <= (1 << 27) and t.graph_iters in (8, 16, 32)
     assert (t.tile_w, t.tile_h) in ((16, 16), (32, 32), (64, 16), (128, 8)) and t.bvh_width in (2, 4)
     cur = ctx.tuning()
     fields = ("block_size", "lds_stack", "path_slots", "graph_iters", "tile_w", "tile_h", "bvh_width")
