@@ -48,8 +48,9 @@ enum : int {
 };
 
 // Work items of the current chunk (64-bit words, 256 bytes apart): segment s hands out
-// items[item_next(s)] .. items[item_end(s)] - 1; items[kItemChunkBegin] is the first item of the
-// chunk (index 0 of the sample buffer).
+// items[item_next(s)] .. items[item_end(s)] - 1, and items[item_next(s) + 1] is where its range
+// began (the start fill's items run from there to the counter's first value);
+// items[kItemChunkBegin] is the first item of the chunk (index 0 of the sample buffer).
 constexpr int kItemStride = 32;
 SOL_HD constexpr int item_next(int seg) { return seg * kItemStride; }
 SOL_HD constexpr int item_end(int seg) { return (kNumSegments + seg) * kItemStride; }

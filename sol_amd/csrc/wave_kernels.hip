@@ -630,25 +630,38 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
 
 // ---- start / reset / reduce ------------------------------------------------------------------------
 
-// Starts a path in every slot (beginning of a chunk of work).
+// The start fill lays the slots out as the rounds of the other passes would: slot s is in round
+// s / kBlock, round r in segment r % kNumSegments, and a slot's place in its segment follows the
+// rounds in order. Every slot gets one extension entry (a path, or a hole if no item is left).
+SOL_HD unsigned start_segment(unsigned slot) { return (slot / kBlock) % kNumSegments; }
+SOL_HD unsigned start_position(unsigned slot) {
+    return (slot / kBlock / kNumSegments) * kBlock + slot % kBlock;
+}
+SOL_HD unsigned start_slots(unsigned slots, unsigned seg) {   // slots of segment seg
+    const unsigned full = slots / kBlock, rem = slots % kBlock;
+    unsigned c = full > seg ? ((full - 1 - seg) / kNumSegments + 1) * kBlock : 0u;
+    if (rem && full % kNumSegments == seg) c += rem;
+    return c;
+}
+
+// Starts a path in every slot (beginning of a chunk of work), in the layout above and without
+// atomics: k_set_chunk gave segment g the items from items[item_next(g) + 1], and the slot at
+// position p of segment g takes item items[item_next(g) + 1] + p, if below items[item_next(g)].
 __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, int nxt) {
-    __shared__ RoundLds<1> lds[2];
-    const int seg = my_segment();
-    unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
-    unsigned started = 0, parity = 0;
-    for (unsigned round = blockIdx.x; round * kBlock < b.slots; round += gridDim.x, parity ^= 1) {
-        unsigned slot = round * kBlock + threadIdx.x;
-        bool want = slot < b.slots;
-        RoundPlan plan = plan_round<1>(lds[parity], nullptr, ecnt, b.items, seg, -1, false, want);
-        if (plan.start) {
+    unsigned started = 0;
+    for (unsigned slot = blockIdx.x * kBlock + threadIdx.x; slot < b.slots; slot += gridDim.x * kBlock) {
+        const unsigned seg = start_segment(slot), pos = start_position(slot);
+        const unsigned long long item = b.items[item_next(static_cast<int>(seg)) + 1] + pos;
+        const unsigned entry = seg * b.cap_e + pos;
+        if (item < b.items[item_next(static_cast<int>(seg))]) {
             ++started;
             V3 org, dir;
-            init_path(tabs, b, plan.item, slot, org, dir);
-            float4* e = b.eq[nxt] + 2 * (seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
+            init_path(tabs, b, item, slot, org, dir);
+            float4* e = b.eq[nxt] + 2 * entry;
             e[0] = f4(org, 0.0f);
             e[1] = f4(dir, as_f(slot));
-        } else if (want) {
-            write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
+        } else {
+            write_hole(b, nxt, entry);
         }
     }
     wave_add_stat(&b.stats[2], started);
@@ -661,16 +674,29 @@ __global__ void k_reset(WaveBuf b, int cur) {
     if (i < kNumWaveCounters && i / kNumSegments != kQExtend0 + cur) b.counts[i * kCounterStride] = 0;
 }
 
-__global__ void k_set_chunk(WaveBuf b, unsigned long long begin, unsigned long long end) {
-    // segment s hands out the items [begin + n s / S, begin + n (s + 1) / S)
-    const unsigned long long n = end - begin;
-    if (threadIdx.x < kNumSegments) {
-        unsigned long long s = threadIdx.x;
-        b.items[item_next(static_cast<int>(s))] = begin + n * s / kNumSegments;
-        b.items[item_end(static_cast<int>(s))] = begin + n * (s + 1) / kNumSegments;
-    }
-    if (threadIdx.x == 0) b.items[kItemChunkBegin] = begin;
+// Work items of a chunk [begin, end) by segment, consecutive ranges in segment order: segment g
+// first gets as many items as the start fill has slots in it (k_start takes them, so its counter
+// starts past them and the range start is kept in the next word), then an even share of the rest
+// for the paths that restart there. The start fill's extension entries are counted in advance.
+__global__ void k_set_chunk(WaveBuf b, unsigned long long begin, unsigned long long end, int nxt) {
     if (threadIdx.x < kNumWaveCounters) b.counts[threadIdx.x * kCounterStride] = 0;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    unsigned long long left = end - begin, filled[kNumSegments];
+    for (int g = 0; g < kNumSegments; ++g) {
+        filled[g] = min(static_cast<unsigned long long>(start_slots(b.slots, g)), left);
+        left -= filled[g];
+    }
+    unsigned long long base = begin;
+    for (int g = 0; g < kNumSegments; ++g) {
+        const unsigned long long extra = left * (g + 1) / kNumSegments - left * g / kNumSegments;
+        b.items[item_next(g) + 1] = base;
+        b.items[item_next(g)] = base + filled[g];
+        b.items[item_end(g)] = base + filled[g] + extra;
+        b.counts[qcount(kQExtend0 + nxt, g)] = start_slots(b.slots, g);
+        base += filled[g] + extra;
+    }
+    b.items[kItemChunkBegin] = begin;
 }
 
 // Per pixel of the chunk's tiles: sum the samples in sample order, add to the image once.
@@ -805,7 +831,8 @@ hipError_t wave_launch_reset(const WaveLaunch& L, int cur) {
 
 hipError_t wave_launch_set_chunk(const WaveLaunch& L, unsigned long long begin, unsigned long long end) {
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(64), 0, L.stream, L.buf, begin, end);
+    // the start fill that follows writes extension queue 0
+    hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(64), 0, L.stream, L.buf, begin, end, 0);
     return hipGetLastError();
 }
 
