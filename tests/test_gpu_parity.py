@@ -141,16 +141,19 @@ def test_render_rejects_bad_arguments(ctx, cornell):
     assert not img.any()
 
 
-def test_wavefront_equals_megakernel_bitwise(ctx):
-    # Same streams, same operations, same per-pixel summation order: identical images.
+@pytest.mark.parametrize("slots", [5000, 8192, 70000])
+def test_wavefront_equals_megakernel_bitwise(ctx, slots):
+    # Same streams, same operations, same per-pixel summation order: identical images. 23,040 work
+    # items: 5000 and 8192 slots refill many times (5000: the start fill ends in a partial round of
+    # 392 slots), 70000 slots leave the last segments' start entries as holes.
     scene = sol_amd.Scene.sphere_field(num_spheres=30, subdivision=2, seed=8, extent=4.0, radius_max=1.0)
     cfg = _cfg(max_path_len=12)
     g = ctx.upload(scene)
     a, b = sol_amd.new_image(80, 48), sol_amd.new_image(80, 48)
     g.render(a, 0, 6, cfg, renderer=F.RENDERER_MEGAKERNEL)
-    ctx.set_tuning(path_slots=8192, tile_w=16, tile_h=8, graph_iters=4)   # many chunks of work, tiny queues
+    ctx.set_tuning(path_slots=slots, tile_w=16, tile_h=8, graph_iters=4)   # many chunks of work, tiny queues
     g.render(b, 0, 6, cfg, renderer=F.RENDERER_WAVEFRONT)
-    ctx.set_tuning(path_slots=1 << 20, tile_w=32, tile_h=32, graph_iters=16)
+    ctx.set_tuning(path_slots=0, tile_w=0, tile_h=0, graph_iters=0)       # back to the defaults
     np.testing.assert_array_equal(a, b)
 
 
