@@ -64,8 +64,8 @@ enum : int {
 // queues) and shade (one kernel per material queue: NEE, Russian roulette, BSDF sampling).
 // Per-path state is one 64-byte record per slot (random access costs one cache line); the data
 // that flows from one kernel to the next travels in queue entries written at compacted positions
-// (coalesced): extension rays (32 B), hits (16 B), shade entries (hit record, 64 B), shadow rays
-// (48 B). Appends are compacted per workgroup (one atomic per round of kWaveBlock entries, queue
+// (coalesced, field-major): extension rays (32 B), hits (16 B), shade entries (hit record, 64 B),
+// shadow rays (48 B). Appends are compacted per workgroup (one atomic per round of kWaveBlock entries, queue
 // and segment).
 // A path that ends writes its radiance to its sample's entry of the sample buffer and the slot
 // starts the next work item at once (in the kernel that ended it); a shadow ray still in flight
@@ -91,6 +91,7 @@ enum : int {
 struct WaveBuf {
     uint32_t slots;
     uint32_t cap_e, cap_m, cap_s;     // segment capacities: extension, shade, shadow queues
+    uint32_t stride_e, stride_m, stride_s;   // kNumSegments * cap_e, cap_m, cap_s (the queues' field strides)
     float4* path;                     // [slots * 4]
     float4* hits;                     // [slots]: closest hit of extension entry i (in queue order):
                                       // leaf-order triangle (int bits, -1 = miss), t, u, v
@@ -102,6 +103,17 @@ struct WaveBuf {
     unsigned long long* stats;        // [0] extension rays [1] shadow rays [2] samples
     float4* samples;                  // [sample_capacity]: radiance of each item of the chunk
     unsigned long long sample_capacity;
+
+    // Queue entries are stored field-major: 16-byte word k of entry a (its storage index, segment *
+    // capacity + position) follows every entry's word k - 1, so a wave reading or writing one word
+    // of consecutive entries touches consecutive cache lines, a quarter as many as with whole
+    // entries side by side (the texture path's cost is per line). Path records stay whole (one
+    // 64-byte line): they are reached by slot, which consecutive entries do not share in order.
+    // Word indices fit 32 bits (at most 4 x 2^26 queue words).
+    SOL_DEV float4* rec(uint32_t slot, uint32_t k) const { return path + (slot * 4u + k); }
+    SOL_DEV float4* ext(int q, uint32_t a, uint32_t k) const { return eq[q] + (k * stride_e + a); }
+    SOL_DEV float4* shade_entry(int q, uint32_t a, uint32_t k) const { return mq[q] + (k * stride_m + a); }
+    SOL_DEV float4* shadow(uint32_t a, uint32_t k) const { return sq + (k * stride_s + a); }
 };
 
 // Jobs are images; a job's work items are ordered band by band (tile_h rows), tile by tile

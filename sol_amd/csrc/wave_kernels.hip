@@ -168,7 +168,7 @@ SOL_DEV RoundPlan plan_round(RoundLds<NQ>& lds, unsigned* const* qcounters, unsi
 
 // A retired lane whose extension entry stays unused marks it as a hole (trace and sort skip it).
 SOL_DEV void write_hole(const WaveBuf& b, int nxt, unsigned entry) {
-    b.eq[nxt][2 * entry + 1] = make_float4(0.0f, 0.0f, 0.0f, as_f(kHole));
+    *b.ext(nxt, entry, 1) = make_float4(0.0f, 0.0f, 0.0f, as_f(kHole));
 }
 
 // The segment this workgroup appends to (grids are multiples of kNumSegments).
@@ -202,7 +202,7 @@ template <bool MULTI>
 SOL_DEV const SceneView& scene_of(const SceneView& single, const WaveTables& t, const WaveBuf& b, uint32_t slot,
                                   SceneView& tmp) {
     if (!MULTI) return single;
-    tmp = t.scenes[as_u(b.path[slot * 4 + 3].x)];
+    tmp = t.scenes[as_u(b.rec(slot, 3)->x)];
     return tmp;
 }
 
@@ -260,12 +260,11 @@ SOL_DEV void init_path(const WaveTables& t, const WaveBuf& b, unsigned long long
     Pcg rng;
     rng.seed(pixel_seed(sample, x, y));
     camera_ray(cam, x, y, t.job_w[job], t.job_h[job], rng, org, dir);
-    float4* p = b.path + slot * 4;
-    p[0] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    p[1] = make_float4(0.0f, 0.0f, 0.0f, as_f(0u));
-    p[2] = make_float4(as_f(static_cast<uint32_t>(rng.state)), as_f(static_cast<uint32_t>(rng.state >> 32)),
-                       as_f(static_cast<uint32_t>(item)), as_f(static_cast<uint32_t>(item >> 32)));
-    p[3] = make_float4(as_f(sc), 0.0f, 0.0f, 0.0f);
+    *b.rec(slot, 0) = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    *b.rec(slot, 1) = make_float4(0.0f, 0.0f, 0.0f, as_f(0u));
+    *b.rec(slot, 2) = make_float4(as_f(static_cast<uint32_t>(rng.state)), as_f(static_cast<uint32_t>(rng.state >> 32)),
+                                  as_f(static_cast<uint32_t>(item)), as_f(static_cast<uint32_t>(item >> 32)));
+    *b.rec(slot, 3) = make_float4(as_f(sc), 0.0f, 0.0f, 0.0f);
 }
 
 // The radiance of an ended path goes to its sample entry.
@@ -324,7 +323,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
     }
     const Stack st = make_stack_w<BLOCK>(lds_stack, sc, LDS);
     const unsigned waves = gridDim.x * (BLOCK / 64);
-    const float4* q = b.eq[cur];
     const FetchUniform fu(single, lds_nodes);
     const FetchUniform4 fu4(single, lds_nodes);
     FetchGlobal fg(single);
@@ -360,7 +358,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                 else if (base2 + (rank - avail) < end2) e = base2 + (rank - avail);
                 if (e < n_ext) {
                     const unsigned a = eqv.entry(e);
-                    float4 e0 = q[2 * a], e1 = q[2 * a + 1];
+                    float4 e0 = *b.ext(cur, a, 0), e1 = *b.ext(cur, a, 1);
                     uint32_t slot = as_u(e1.w);
                     if (slot != kHole) {
                         idx = static_cast<int>(e);
@@ -369,15 +367,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         r = make_ray_pre(xyz(e0), xyz(e1));
                         ts.start(FLT_MAX);
                         if (MULTI) {
-                            const SceneView& sv = tabs.scenes[as_u(b.path[slot * 4 + 3].x)];
+                            const SceneView& sv = tabs.scenes[as_u(b.rec(slot, 3)->x)];
                             fg = FetchGlobal(sv);
                             fg4 = FetchGlobal4(sv);
                         }
                         ++n_closest;
                     }
                 } else if (e != 0xFFFFFFFFu) {
-                    const float4* se = b.sq + 3 * sqv.entry(e - n_ext);
-                    float4 e0 = se[0], e1 = se[1];
+                    const unsigned sa = sqv.entry(e - n_ext);
+                    float4 e0 = *b.shadow(sa, 0), e1 = *b.shadow(sa, 1);
                     idx = static_cast<int>(e);
                     any = true;
                     tmin = cfg.ray_offset;
@@ -387,10 +385,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         uint32_t target = as_u(e0.w);
                         uint32_t sid;
                         if (target == kSampleTarget) {
-                            unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) | as_u(se[2].w);
+                            unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) |
+                                                      as_u(b.shadow(sa, 2)->w);
                             sid = tabs.job_scene[find_job(tabs, item)];
                         } else {
-                            sid = as_u(b.path[target * 4 + 3].x);
+                            sid = as_u(b.rec(target, 3)->x);
                         }
                         fg = FetchGlobal(tabs.scenes[sid]);
                         fg4 = FetchGlobal4(tabs.scenes[sid]);
@@ -425,8 +424,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                                     : make_float4(as_f(0xFFFFFFFFu), 0.0f, 0.0f, 0.0f);
                 } else if (status == kStepDone) {
                     // unoccluded: add the light sample's contribution
-                    const float4* se = b.sq + 3 * sqv.entry(e - n_ext);
-                    float4 e0 = se[0], e1 = se[1], ct = se[2];
+                    const unsigned sa = sqv.entry(e - n_ext);
+                    float4 e0 = *b.shadow(sa, 0), e1 = *b.shadow(sa, 1), ct = *b.shadow(sa, 2);
                     uint32_t target = as_u(e0.w);
                     if (target == kSampleTarget) {
                         unsigned long long item = (static_cast<unsigned long long>(as_u(e1.w)) << 32) | as_u(ct.w);
@@ -434,8 +433,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         float4 v = *d;
                         *d = make_float4(v.x + ct.x, v.y + ct.y, v.z + ct.z, 0.0f);
                     } else {
-                        float4 p1 = b.path[target * 4 + 1];
-                        b.path[target * 4 + 1] = make_float4(p1.x + ct.x, p1.y + ct.y, p1.z + ct.z, p1.w);
+                        float4 p1 = *b.rec(target, 1);
+                        *b.rec(target, 1) = make_float4(p1.x + ct.x, p1.y + ct.y, p1.z + ct.z, p1.w);
                     }
                 }
                 idx = -1;
@@ -459,7 +458,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     unsigned* qcnt[kNumShadeQueues] = {&b.counts[qcount(kQShade0, seg)], &b.counts[qcount(kQShade0 + 1, seg)],
                                        &b.counts[qcount(kQShade0 + 2, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
-    const float4* q = b.eq[cur];
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
         unsigned i = round * kBlock + threadIdx.x;
@@ -472,8 +470,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             // the whole entry and its hit at once (a hole's are in bounds and unused): one memory
             // round trip fewer before the shading record, which depends on the hit
             const unsigned a = eqv.entry(i);
-            e0 = q[2 * a];
-            e1 = q[2 * a + 1];
+            e0 = *b.ext(cur, a, 0);
+            e1 = *b.ext(cur, a, 1);
             h = b.hits[i];
             slot = as_u(e1.w);
             valid = slot != kHole;
@@ -488,12 +486,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 V3 org = xyz(e0), dir = xyz(e1);
                 Surface sf = decode_hit(s, org, dir, tri, h.y, h.z, h.w);
                 if (sf.light >= 0 && sf.front) {
-                    float4 p0 = b.path[slot * 4 + 0], p1 = b.path[slot * 4 + 1];
+                    float4 p0 = *b.rec(slot, 0), p1 = *b.rec(slot, 1);
                     V3 Le;
                     float w;
                     V3 C = xyz(p1) + hit_emission(s, cfg.flags, sf, dir, h.y, p0.w,
                                                   1.0f / static_cast<float>(s.num_lights), xyz(p0), Le, w);
-                    b.path[slot * 4 + 1] = f4(C, p1.w);
+                    *b.rec(slot, 1) = f4(C, p1.w);
                 }
                 if (sf.mat < 0) {
                     dead = true;
@@ -505,23 +503,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                     dest = queue_of(sf.bsdf);
                 }
             }
-            if (dead) retire(b, b.path[slot * 4 + 1], b.path[slot * 4 + 2]);
+            if (dead) retire(b, *b.rec(slot, 1), *b.rec(slot, 2));
         }
         RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, dest, false, dead);
         if (plan.pos >= 0) {
-            float4* e = b.mq[dest] + 4 * (seg * b.cap_m + static_cast<unsigned>(plan.pos));
-            e[0] = rec0;
-            e[1] = rec1;
-            e[2] = rec2;
-            e[3] = rec3;
+            const unsigned a = seg * b.cap_m + static_cast<unsigned>(plan.pos);
+            *b.shade_entry(dest, a, 0) = rec0;
+            *b.shade_entry(dest, a, 1) = rec1;
+            *b.shade_entry(dest, a, 2) = rec2;
+            *b.shade_entry(dest, a, 3) = rec3;
         }
         if (plan.start) {
             ++started;
             V3 org, dir;
             init_path(tabs, b, plan.item, slot, org, dir);
-            float4* e = b.eq[nxt] + 2 * (seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
-            e[0] = f4(org, 0.0f);
-            e[1] = f4(dir, as_f(slot));
+            const unsigned a = seg * b.cap_e + static_cast<unsigned>(plan.ext_pos);
+            *b.ext(nxt, a, 0) = f4(org, 0.0f);
+            *b.ext(nxt, a, 1) = f4(dir, as_f(slot));
         } else if (dead) {
             write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
         }
@@ -532,9 +530,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 // ---- shade -----------------------------------------------------------------------------------------
 
 // One kernel per queue; the material type of the queue is a compile-time constant (glass and
-// mirror share the specular queue), so each kernel carries only its BSDF's code.
+// mirror share the specular queue), so each kernel carries only its BSDF's code. The glossy
+// kernel (Phong and the diffuse-Phong mix) is held to three waves per SIMD (168 VGPRs).
 template <int QUEUE, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE == kQueuePhong ? 3 : 1))) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                   int nxt) {
     __shared__ RoundLds<1> lds[2];
     const SegQueue mqv(b.counts, kQShade0 + QUEUE, b.cap_m);
@@ -542,7 +541,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
     const int seg = my_segment();
     unsigned* scnt[1] = {&b.counts[qcount(kQShadow, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
-    const float4* mq = b.mq[QUEUE];
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
         unsigned i = round * kBlock + threadIdx.x;
@@ -553,7 +551,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
         unsigned long long item = 0;
         if (valid) {
             const unsigned a = mqv.entry(i);
-            float4 r0 = mq[4 * a], r1 = mq[4 * a + 1], r2 = mq[4 * a + 2], r3 = mq[4 * a + 3];
+            float4 r0 = *b.shade_entry(QUEUE, a, 0), r1 = *b.shade_entry(QUEUE, a, 1), r2 = *b.shade_entry(QUEUE, a, 2),
+                   r3 = *b.shade_entry(QUEUE, a, 3);
             slot = as_u(r1.w);
             SceneView tmp;
             const SceneView& s = scene_of<MULTI>(single, tabs, b, slot, tmp);
@@ -571,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
             if (QUEUE == kQueueDiffuse) m.type = SOL_BSDF_DIFFUSE;
             if (QUEUE == kQueuePhong && m.type != SOL_BSDF_INTERP) m.type = SOL_BSDF_PHONG;
             if (QUEUE == kQueueSpecular && m.type != SOL_BSDF_GLASS) m.type = SOL_BSDF_MIRROR;
-            float4 p0 = b.path[slot * 4 + 0], p1 = b.path[slot * 4 + 1], p2 = b.path[slot * 4 + 2];
+            float4 p0 = *b.rec(slot, 0), p1 = *b.rec(slot, 1), p2 = *b.rec(slot, 2);
             item = item_of(p2);
             V3 T = xyz(p0);
             Pcg rng;
@@ -592,9 +591,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
                 cont = depth < cfg.max_path_len;
                 ndir = bs.in;
                 if (cont) {
-                    b.path[slot * 4 + 0] = f4(T, skip_nee ? 0.0f : bs.pdf);
-                    b.path[slot * 4 + 1] = make_float4(p1.x, p1.y, p1.z, as_f(depth));
-                    b.path[slot * 4 + 2] = make_float4(as_f(static_cast<uint32_t>(rng.state)),
+                    *b.rec(slot, 0) = f4(T, skip_nee ? 0.0f : bs.pdf);
+                    *b.rec(slot, 1) = make_float4(p1.x, p1.y, p1.z, as_f(depth));
+                    *b.rec(slot, 2) = make_float4(as_f(static_cast<uint32_t>(rng.state)),
                                                        as_f(static_cast<uint32_t>(rng.state >> 32)), p2.z, p2.w);
                 }
             }
@@ -603,10 +602,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
         bool retired = valid && !cont;
         RoundPlan plan = plan_round<1>(lds[parity], scnt, ecnt, b.items, seg, shoot ? 0 : -1, cont, retired);
         if (plan.pos >= 0) {
-            float4* e = b.sq + 3 * (seg * b.cap_s + static_cast<unsigned>(plan.pos));
-            e[0] = f4(point, as_f(cont ? slot : kSampleTarget));
-            e[1] = f4(sdir, as_f(static_cast<uint32_t>(item >> 32)));
-            e[2] = f4(contrib, as_f(static_cast<uint32_t>(item)));
+            const unsigned a = seg * b.cap_s + static_cast<unsigned>(plan.pos);
+            *b.shadow(a, 0) = f4(point, as_f(cont ? slot : kSampleTarget));
+            *b.shadow(a, 1) = f4(sdir, as_f(static_cast<uint32_t>(item >> 32)));
+            *b.shadow(a, 2) = f4(contrib, as_f(static_cast<uint32_t>(item)));
         }
         if (plan.ext_pos >= 0) {
             if (cont || plan.start) {
@@ -617,9 +616,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView single, WaveTables t
                     init_path(tabs, b, plan.item, slot, org, dir);
                     tmin = 0.0f;
                 }
-                float4* e = b.eq[nxt] + 2 * (seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
-                e[0] = f4(org, tmin);
-                e[1] = f4(dir, as_f(slot));
+                const unsigned a = seg * b.cap_e + static_cast<unsigned>(plan.ext_pos);
+                *b.ext(nxt, a, 0) = f4(org, tmin);
+                *b.ext(nxt, a, 1) = f4(dir, as_f(slot));
             } else {
                 write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
             }
@@ -657,9 +656,8 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
             ++started;
             V3 org, dir;
             init_path(tabs, b, item, slot, org, dir);
-            float4* e = b.eq[nxt] + 2 * entry;
-            e[0] = f4(org, 0.0f);
-            e[1] = f4(dir, as_f(slot));
+            *b.ext(nxt, entry, 0) = f4(org, 0.0f);
+            *b.ext(nxt, entry, 1) = f4(dir, as_f(slot));
         } else {
             write_hole(b, nxt, entry);
         }

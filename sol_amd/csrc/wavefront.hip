@@ -80,6 +80,9 @@ struct WavefrontState {
         b.cap_e = static_cast<uint32_t>(cap_e);
         b.cap_m = static_cast<uint32_t>(cap_m);
         b.cap_s = static_cast<uint32_t>(cap_s);
+        b.stride_e = static_cast<uint32_t>(kNumSegments * cap_e);
+        b.stride_m = static_cast<uint32_t>(kNumSegments * cap_m);
+        b.stride_s = static_cast<uint32_t>(kNumSegments * cap_s);
         b.path = f4s(size_t(n) * 4);
         b.hits = f4s(n);
         for (int q = 0; q < 2; ++q) b.eq[q] = f4s(kNumSegments * cap_e * 2);
