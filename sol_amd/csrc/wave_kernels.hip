@@ -249,9 +249,8 @@ SOL_DEV void decode_item(const WaveTables& t, unsigned long long item, uint32_t&
     sample = t.job_sample0[job] + s;
 }
 
-// Path start of one lane: seed, camera ray, fresh path record.
-SOL_DEV void init_path(const WaveTables& t, const WaveBuf& b, unsigned long long item, uint32_t slot, V3& org,
-                       V3& dir) {
+// Path start of one lane: seed, camera ray and the fresh path record (its four words in rec).
+SOL_DEV void start_path(const WaveTables& t, unsigned long long item, V3& org, V3& dir, float4 rec[4]) {
     uint32_t job, x, y;
     unsigned long long sample;
     decode_item(t, item, job, x, y, sample);
@@ -260,11 +259,19 @@ SOL_DEV void init_path(const WaveTables& t, const WaveBuf& b, unsigned long long
     Pcg rng;
     rng.seed(pixel_seed(sample, x, y));
     camera_ray(cam, x, y, t.job_w[job], t.job_h[job], rng, org, dir);
-    *b.rec(slot, 0) = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    *b.rec(slot, 1) = make_float4(0.0f, 0.0f, 0.0f, as_f(0u));
-    *b.rec(slot, 2) = make_float4(as_f(static_cast<uint32_t>(rng.state)), as_f(static_cast<uint32_t>(rng.state >> 32)),
-                                  as_f(static_cast<uint32_t>(item)), as_f(static_cast<uint32_t>(item >> 32)));
-    *b.rec(slot, 3) = make_float4(as_f(sc), 0.0f, 0.0f, 0.0f);
+    rec[0] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    rec[1] = make_float4(0.0f, 0.0f, 0.0f, as_f(0u));
+    rec[2] = make_float4(as_f(static_cast<uint32_t>(rng.state)), as_f(static_cast<uint32_t>(rng.state >> 32)),
+                         as_f(static_cast<uint32_t>(item)), as_f(static_cast<uint32_t>(item >> 32)));
+    rec[3] = make_float4(as_f(sc), 0.0f, 0.0f, 0.0f);
+}
+
+// The same, with the record stored in the slot.
+SOL_DEV void init_path(const WaveTables& t, const WaveBuf& b, unsigned long long item, uint32_t slot, V3& org,
+                       V3& dir) {
+    float4 rec[4];
+    start_path(t, item, org, dir, rec);
+    for (uint32_t k = 0; k < 4; ++k) *b.rec(slot, k) = rec[k];
 }
 
 // The radiance of an ended path goes to its sample entry.
@@ -646,21 +653,44 @@ SOL_HD unsigned start_slots(unsigned slots, unsigned seg) {   // slots of segmen
 // Starts a path in every slot (beginning of a chunk of work), in the layout above and without
 // atomics: k_set_chunk gave segment g the items from items[item_next(g) + 1], and the slot at
 // position p of segment g takes item items[item_next(g) + 1] + p, if below items[item_next(g)].
+// A wave fills 64 consecutive slots at a time. Their path records are one contiguous 4 KB block:
+// written from each lane directly, a 16-byte store of all lanes would touch one line per lane, so
+// the wave stages its records in LDS and stores the block in four 1 KB runs, lane l writing word
+// l % 4 of record l / 4 of each run (records of holes are skipped). The wave's own LDS writes
+// and reads complete in order, so no barrier is needed.
 __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, int nxt) {
+    __shared__ float4 stage[kWaves][256];
+    float4* st = stage[threadIdx.x >> 6];
+    const unsigned lane = __lane_id();
+    const unsigned wave_step = gridDim.x * kBlock;
     unsigned started = 0;
-    for (unsigned slot = blockIdx.x * kBlock + threadIdx.x; slot < b.slots; slot += gridDim.x * kBlock) {
-        const unsigned seg = start_segment(slot), pos = start_position(slot);
-        const unsigned long long item = b.items[item_next(static_cast<int>(seg)) + 1] + pos;
-        const unsigned entry = seg * b.cap_e + pos;
-        if (item < b.items[item_next(static_cast<int>(seg))]) {
-            ++started;
-            V3 org, dir;
-            init_path(tabs, b, item, slot, org, dir);
-            *b.ext(nxt, entry, 0) = f4(org, 0.0f);
-            *b.ext(nxt, entry, 1) = f4(dir, as_f(slot));
-        } else {
-            write_hole(b, nxt, entry);
+    for (unsigned base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < b.slots; base += wave_step) {
+        const unsigned slot = base + lane;
+        bool start = false;
+        if (slot < b.slots) {
+            const unsigned seg = start_segment(slot), pos = start_position(slot);
+            const unsigned long long item = b.items[item_next(static_cast<int>(seg)) + 1] + pos;
+            const unsigned entry = seg * b.cap_e + pos;
+            if (item < b.items[item_next(static_cast<int>(seg))]) {
+                start = true;
+                ++started;
+                V3 org, dir;
+                float4 rec[4];
+                start_path(tabs, item, org, dir, rec);
+                for (int k = 0; k < 4; ++k) st[lane * 4 + k] = rec[k];
+                *b.ext(nxt, entry, 0) = f4(org, 0.0f);
+                *b.ext(nxt, entry, 1) = f4(dir, as_f(slot));
+            } else {
+                write_hole(b, nxt, entry);
+            }
         }
+        const unsigned long long starts = __ballot(start);
+        __builtin_amdgcn_wave_barrier();
+        for (unsigned k = 0; k < 4; ++k) {
+            const unsigned r = 16 * k + lane / 4;   // record r of the wave's 64
+            if ((starts >> r) & 1ull) b.path[(base + r) * 4 + lane % 4] = st[64 * k + lane];
+        }
+        __builtin_amdgcn_wave_barrier();
     }
     wave_add_stat(&b.stats[2], started);
 }
