@@ -37,14 +37,18 @@ enum : int {
 };
 
 // Device counters: the length of segment s of queue q is counter q * kNumSegments + s, then the
-// work-fetch counter of the traversal kernel. Counters sit kCounterStride words (256 bytes) apart:
-// device atomics on one cache line serialise, so each counter gets a line of its own.
+// work-fetch counter of the traversal kernel (these kNumWaveCounters are cleared for every pass),
+// then the start flag: nonzero while extension queue 0 holds the start fill of a chunk, whose
+// entries leave out their first word (the ray origin is the camera's eye, tmin 0). Counters sit
+// kCounterStride words (256 bytes) apart: device atomics on one cache line serialise, so each
+// counter gets a line of its own.
 constexpr int kCounterStride = 64;
 SOL_HD constexpr int qcount(int queue, int seg) { return (queue * kNumSegments + seg) * kCounterStride; }
 enum : int {
     kNumWaveCounters = kNumQueues * kNumSegments + 1,
     kCntTraceWork = kNumQueues * kNumSegments * kCounterStride,
-    kCounterWords = kNumWaveCounters * kCounterStride
+    kCntStartFill = kNumWaveCounters * kCounterStride,
+    kCounterWords = (kNumWaveCounters + 1) * kCounterStride
 };
 
 // Work items of the current chunk (64-bit words, 256 bytes apart): segment s hands out

@@ -324,6 +324,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
     const unsigned n_shadow = sqv.size();
     const unsigned total = n_ext + n_shadow;
     if (total == 0) return;
+    // extension queue 0 holding a chunk's start fill: origins are the camera's eye
+    const bool start_fill = cur == 0 && b.counts[kCntStartFill] != 0;
     if (!MULTI) {
         if (WIDE) load_top_nodes4<BLOCK>(single, lds_nodes);
         else load_top_nodes<BLOCK>(single, lds_nodes);
@@ -365,11 +367,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                 else if (base2 + (rank - avail) < end2) e = base2 + (rank - avail);
                 if (e < n_ext) {
                     const unsigned a = eqv.entry(e);
-                    float4 e0 = *b.ext(cur, a, 0), e1 = *b.ext(cur, a, 1);
+                    float4 e1 = *b.ext(cur, a, 1);
                     uint32_t slot = as_u(e1.w);
                     if (slot != kHole) {
                         idx = static_cast<int>(e);
                         any = false;
+                        float4 e0;
+                        if (start_fill) {
+                            const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot, 3)->x)] : single;
+                            e0 = make_float4(sv.camera.eye[0], sv.camera.eye[1], sv.camera.eye[2], 0.0f);
+                        } else {
+                            e0 = *b.ext(cur, a, 0);
+                        }
                         tmin = e0.w;
                         r = make_ray_pre(xyz(e0), xyz(e1));
                         ts.start(FLT_MAX);
@@ -465,6 +474,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     unsigned* qcnt[kNumShadeQueues] = {&b.counts[qcount(kQShade0, seg)], &b.counts[qcount(kQShade0 + 1, seg)],
                                        &b.counts[qcount(kQShade0 + 2, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
+    const bool start_fill = cur == 0 && b.counts[kCntStartFill] != 0;   // see k_trace
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
         unsigned i = round * kBlock + threadIdx.x;
@@ -477,11 +487,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             // the whole entry and its hit at once (a hole's are in bounds and unused): one memory
             // round trip fewer before the shading record, which depends on the hit
             const unsigned a = eqv.entry(i);
-            e0 = *b.ext(cur, a, 0);
+            if (!start_fill) e0 = *b.ext(cur, a, 0);
             e1 = *b.ext(cur, a, 1);
             h = b.hits[i];
             slot = as_u(e1.w);
             valid = slot != kHole;
+            if (start_fill && valid) {
+                const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot, 3)->x)] : single;
+                e0 = make_float4(sv.camera.eye[0], sv.camera.eye[1], sv.camera.eye[2], 0.0f);
+            }
         }
         if (valid) {
             int tri = static_cast<int>(as_u(h.x));
@@ -543,6 +557,8 @@ template <int QUEUE, bool MULTI>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE == kQueuePhong ? 3 : 1))) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                   int nxt) {
     __shared__ RoundLds<1> lds[2];
+    // after the first sort of a chunk, extension queue 0 no longer holds its start fill
+    if (QUEUE == 0 && blockIdx.x == 0 && threadIdx.x == 0) b.counts[kCntStartFill] = 0;
     const SegQueue mqv(b.counts, kQShade0 + QUEUE, b.cap_m);
     const unsigned count = mqv.size();
     const int seg = my_segment();
@@ -678,7 +694,7 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
                 float4 rec[4];
                 start_path(tabs, item, org, dir, rec);
                 for (int k = 0; k < 4; ++k) st[lane * 4 + k] = rec[k];
-                *b.ext(nxt, entry, 0) = f4(org, 0.0f);
+                // no first word: while the start flag is set, the origin is the camera's eye
                 *b.ext(nxt, entry, 1) = f4(dir, as_f(slot));
             } else {
                 write_hole(b, nxt, entry);
@@ -725,6 +741,7 @@ __global__ void k_set_chunk(WaveBuf b, unsigned long long begin, unsigned long l
         base += filled[g] + extra;
     }
     b.items[kItemChunkBegin] = begin;
+    b.counts[kCntStartFill] = 1;
 }
 
 // Per pixel of the chunk's tiles: sum the samples in sample order, add to the image once.
