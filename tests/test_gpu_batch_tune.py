@@ -59,11 +59,25 @@ def test_autotune_returns_valid_parameters_and_keeps_the_image(ctx):
     ctx.set_tuning(**{k: 0 for k in fields})
 
 
+def test_default_paths_in_flight_follow_device_memory(ctx):
+    # the largest power of two up to 2^27 whose 448 bytes a slot fit in a quarter of the memory
+    mem = ctx.device_info().total_memory
+    ctx.set_tuning(path_slots=0)
+    want = 1 << 27
+    while want > 4096 and want * 448 > mem // 4:
+        want //= 2
+    assert ctx.tuning().path_slots == want
+    if mem >= 4 * 448 * (1 << 27):          # an MI355X (288 GB): a 1080p 64-spp frame in flight at once
+        assert want == 1 << 27
+
+
 def test_tuning_rejects_unsupported_values(ctx):
     with pytest.raises(F.SolError):
         ctx.set_tuning(block_size=96)
     with pytest.raises(F.SolError):
         ctx.set_tuning(path_slots=10)
+    with pytest.raises(F.SolError):
+        ctx.set_tuning(path_slots=(1 << 27) + 1)
     with pytest.raises(F.SolError):
         ctx.set_tuning(bvh_width=3)
     g = ctx.upload(sol_amd.Scene.cornell_box())
