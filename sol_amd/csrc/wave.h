@@ -66,22 +66,27 @@ enum : int {
 // One bounce of all paths in flight is four launches: trace (closest hit of the extension rays
 // and any hit of the previous bounce's shadow rays), reset, sort (hit decoding, emission, material
 // queues) and shade (one kernel per material queue: NEE, Russian roulette, BSDF sampling).
-// Per-path state is one 64-byte record per slot (random access costs one cache line); the data
-// that flows from one kernel to the next travels in queue entries written at compacted positions
-// (coalesced, field-major): extension rays (32 B), hits (16 B), shade entries (hit record, 64 B),
-// shadow rays (48 B). Appends are compacted per workgroup (one atomic per round of kWaveBlock entries, queue
-// and segment).
+// A path's state travels with it in its queue entries, written at compacted positions
+// (coalesced, field-major): extension rays (64 B: the ray and the path state), hits (16 B), shade
+// entries (the hit record and where the path state is, 64 B), shadow rays (48 B). A path's
+// throughput and PCG state stay in its extension entry until the shading kernel has read them
+// (nothing writes that queue during the bounce), so the sort does not copy them. Only what other kernels add to
+// out of order stays in a 32-byte record per slot (random access: one cache line): the radiance
+// (emission added by the sort, unoccluded light samples by the trace) and the scene. Appends are
+// compacted per workgroup (one atomic per round of kWaveBlock entries, queue and segment).
 // A path that ends writes its radiance to its sample's entry of the sample buffer and the slot
 // starts the next work item at once (in the kernel that ended it); a shadow ray still in flight
-// for the ended path adds its contribution to that sample entry instead of the path record.
+// for the ended path adds its contribution to that sample entry instead of the slot record.
 //
-// path record (4 x float4):   [0] throughput xyz, pdf of the previous bounce
-//                             [1] radiance xyz, depth (int bits)
-//                             [2] PCG state lo, hi, work item lo, hi (uint bits)
-//                             [3] scene index (uint bits), unused
-// extension entry (2 x float4): [0] org xyz, tmin  [1] dir xyz, slot
+// slot record (2 x float4):   [0] radiance xyz, -  [1] scene index (uint bits), -
+// extension entry (4 x float4): [0] org xyz, depth (uint bits; 0: a new path, traced from tmin 0,
+//                             else from the configured ray offset)  [1] dir xyz, slot
+//                             [2] throughput xyz, pdf of the previous bounce (not stored at depth
+//                             0: throughput 1, pdf 0)
+//                             [3] PCG state lo, hi, work item lo, hi (uint bits)
 // shade entry (4 x float4):   [0] point xyz, material | front bit  [1] shading normal xyz, slot
-//                             [2] geometric normal xyz, -           [3] ray direction xyz, -
+//                             [2] geometric normal xyz, depth
+//                             [3] ray direction xyz, storage index of the extension entry
 // shadow entry (3 x float4):  [0] org xyz, target  [1] dir xyz, item hi  [2] contribution xyz, item lo
 //                             target = slot, or kSampleTarget | 0 when the path has ended (the
 //                             contribution then goes to the sample entry of the item)
@@ -96,10 +101,10 @@ struct WaveBuf {
     uint32_t slots;
     uint32_t cap_e, cap_m, cap_s;     // segment capacities: extension, shade, shadow queues
     uint32_t stride_e, stride_m, stride_s;   // kNumSegments * cap_e, cap_m, cap_s (the queues' field strides)
-    float4* path;                     // [slots * 4]
+    float4* path;                     // [slots * 2]: slot records
     float4* hits;                     // [slots]: closest hit of extension entry i (in queue order):
                                       // leaf-order triangle (int bits, -1 = miss), t, u, v
-    float4* eq[2];                    // [kNumSegments * cap_e * 2] each
+    float4* eq[2];                    // [kNumSegments * cap_e * 4] each
     float4* mq[kNumShadeQueues];      // [kNumSegments * cap_m * 4] each
     float4* sq;                       // [kNumSegments * cap_s * 3]
     unsigned* counts;                 // kCounterWords
@@ -111,10 +116,10 @@ struct WaveBuf {
     // Queue entries are stored field-major: 16-byte word k of entry a (its storage index, segment *
     // capacity + position) follows every entry's word k - 1, so a wave reading or writing one word
     // of consecutive entries touches consecutive cache lines, a quarter as many as with whole
-    // entries side by side (the texture path's cost is per line). Path records stay whole (one
-    // 64-byte line): they are reached by slot, which consecutive entries do not share in order.
-    // Word indices fit 32 bits (at most 4 x 2^26 queue words).
-    SOL_DEV float4* rec(uint32_t slot, uint32_t k) const { return path + (slot * 4u + k); }
+    // entries side by side (the texture path's cost is per line). Slot records stay whole (one
+    // line): they are reached by slot, which consecutive entries do not share in order. Word
+    // indices fit 32 bits: at 2^27 slots an extension queue has 4 words of 2^28 entries.
+    SOL_DEV float4* rec(uint32_t slot, uint32_t k) const { return path + (slot * 2u + k); }
     SOL_DEV float4* ext(int q, uint32_t a, uint32_t k) const { return eq[q] + (k * stride_e + a); }
     SOL_DEV float4* shade_entry(int q, uint32_t a, uint32_t k) const { return mq[q] + (k * stride_m + a); }
     SOL_DEV float4* shadow(uint32_t a, uint32_t k) const { return sq + (k * stride_s + a); }

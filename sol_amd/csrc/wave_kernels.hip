@@ -202,7 +202,7 @@ template <bool MULTI>
 SOL_DEV const SceneView& scene_of(const SceneView& single, const WaveTables& t, const WaveBuf& b, uint32_t slot,
                                   SceneView& tmp) {
     if (!MULTI) return single;
-    tmp = t.scenes[as_u(b.rec(slot, 3)->x)];
+    tmp = t.scenes[as_u(b.rec(slot, 1)->x)];
     return tmp;
 }
 
@@ -249,34 +249,65 @@ SOL_DEV void decode_item(const WaveTables& t, unsigned long long item, uint32_t&
     sample = t.job_sample0[job] + s;
 }
 
-// Path start of one lane: seed, camera ray and the fresh path record (its four words in rec).
-SOL_DEV void start_path(const WaveTables& t, unsigned long long item, V3& org, V3& dir, float4 rec[4]) {
+// Path start of one work item: seed, camera ray, the PCG state after the camera's draws.
+struct PathStart {
+    V3 org, dir;
+    unsigned long long rng;
+    uint32_t scene;
+};
+
+SOL_DEV PathStart start_path(const WaveTables& t, unsigned long long item) {
     uint32_t job, x, y;
     unsigned long long sample;
     decode_item(t, item, job, x, y, sample);
-    uint32_t sc = t.job_scene[job];
-    const sol_camera cam = t.scenes[sc].camera;
+    PathStart ps;
+    ps.scene = t.job_scene[job];
+    const sol_camera cam = t.scenes[ps.scene].camera;
     Pcg rng;
     rng.seed(pixel_seed(sample, x, y));
-    camera_ray(cam, x, y, t.job_w[job], t.job_h[job], rng, org, dir);
-    rec[0] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    rec[1] = make_float4(0.0f, 0.0f, 0.0f, as_f(0u));
-    rec[2] = make_float4(as_f(static_cast<uint32_t>(rng.state)), as_f(static_cast<uint32_t>(rng.state >> 32)),
-                         as_f(static_cast<uint32_t>(item)), as_f(static_cast<uint32_t>(item >> 32)));
-    rec[3] = make_float4(as_f(sc), 0.0f, 0.0f, 0.0f);
+    camera_ray(cam, x, y, t.job_w[job], t.job_h[job], rng, ps.org, ps.dir);
+    ps.rng = rng.state;
+    return ps;
 }
 
-// The same, with the record stored in the slot.
-SOL_DEV void init_path(const WaveTables& t, const WaveBuf& b, unsigned long long item, uint32_t slot, V3& org,
-                       V3& dir) {
-    float4 rec[4];
-    start_path(t, item, org, dir, rec);
-    for (uint32_t k = 0; k < 4; ++k) *b.rec(slot, k) = rec[k];
+SOL_DEV float4 rng_item_word(unsigned long long rng, unsigned long long item) {
+    return make_float4(as_f(static_cast<uint32_t>(rng)), as_f(static_cast<uint32_t>(rng >> 32)),
+                       as_f(static_cast<uint32_t>(item)), as_f(static_cast<uint32_t>(item >> 32)));
+}
+SOL_DEV unsigned long long rng_of(float4 w) {
+    return static_cast<unsigned long long>(as_u(w.x)) | (static_cast<unsigned long long>(as_u(w.y)) << 32);
+}
+
+// A path continues with the extension entry `a` of queue q: origin and depth (depth 0, a new
+// path, is traced from tmin 0, deeper rays from the configured offset), direction and slot,
+// throughput and the pdf of the bounce that chose the direction, PCG state and work item. At
+// depth 0 the throughput is 1 and the pdf 0, so that word is neither written nor read.
+SOL_DEV void write_ext(const WaveBuf& b, int q, uint32_t a, V3 org, uint32_t depth, V3 dir, uint32_t slot, V3 T,
+                       float pdf, unsigned long long rng, unsigned long long item) {
+    *b.ext(q, a, 0) = f4(org, as_f(depth));
+    *b.ext(q, a, 1) = f4(dir, as_f(slot));
+    if (depth != 0) *b.ext(q, a, 2) = f4(T, pdf);
+    *b.ext(q, a, 3) = rng_item_word(rng, item);
+}
+
+// Throughput and previous pdf of the path of extension entry `a` of queue q at `depth`.
+SOL_DEV float4 ext_throughput(const WaveBuf& b, int q, uint32_t a, uint32_t depth) {
+    return depth == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : *b.ext(q, a, 2);
+}
+
+// A new path for work item `item` in `slot`: its slot record (no radiance yet, its scene) and its
+// first extension entry.
+SOL_DEV void init_path(const WaveTables& t, const WaveBuf& b, unsigned long long item, uint32_t slot, int q,
+                       uint32_t a) {
+    const PathStart ps = start_path(t, item);
+    *b.rec(slot, 0) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    *b.rec(slot, 1) = make_float4(as_f(ps.scene), 0.0f, 0.0f, 0.0f);
+    write_ext(b, q, a, ps.org, 0u, ps.dir, slot, V3{1.0f, 1.0f, 1.0f}, 0.0f, ps.rng, item);
 }
 
 // The radiance of an ended path goes to its sample entry.
-SOL_DEV void retire(const WaveBuf& b, float4 p1, float4 p2) {
-    b.samples[item_of(p2) - b.items[kItemChunkBegin]] = make_float4(p1.x, p1.y, p1.z, 0.0f);
+SOL_DEV void retire(const WaveBuf& b, float4 radiance, unsigned long long item) {
+    b.samples[item - b.items[kItemChunkBegin]] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
 }
 
 template <int BLOCK>
@@ -374,16 +405,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         any = false;
                         float4 e0;
                         if (start_fill) {
-                            const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot, 3)->x)] : single;
-                            e0 = make_float4(sv.camera.eye[0], sv.camera.eye[1], sv.camera.eye[2], 0.0f);
+                            const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot, 1)->x)] : single;
+                            e0 = make_float4(sv.camera.eye[0], sv.camera.eye[1], sv.camera.eye[2], as_f(0u));
                         } else {
                             e0 = *b.ext(cur, a, 0);
                         }
-                        tmin = e0.w;
+                        tmin = as_u(e0.w) == 0 ? 0.0f : cfg.ray_offset;   // depth 0: a new path
                         r = make_ray_pre(xyz(e0), xyz(e1));
                         ts.start(FLT_MAX);
                         if (MULTI) {
-                            const SceneView& sv = tabs.scenes[as_u(b.rec(slot, 3)->x)];
+                            const SceneView& sv = tabs.scenes[as_u(b.rec(slot, 1)->x)];
                             fg = FetchGlobal(sv);
                             fg4 = FetchGlobal4(sv);
                         }
@@ -405,7 +436,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                                                       as_u(b.shadow(sa, 2)->w);
                             sid = tabs.job_scene[find_job(tabs, item)];
                         } else {
-                            sid = as_u(b.rec(target, 3)->x);
+                            sid = as_u(b.rec(target, 1)->x);
                         }
                         fg = FetchGlobal(tabs.scenes[sid]);
                         fg4 = FetchGlobal4(tabs.scenes[sid]);
@@ -449,8 +480,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         float4 v = *d;
                         *d = make_float4(v.x + ct.x, v.y + ct.y, v.z + ct.z, 0.0f);
                     } else {
-                        float4 p1 = *b.rec(target, 1);
-                        *b.rec(target, 1) = make_float4(p1.x + ct.x, p1.y + ct.y, p1.z + ct.z, p1.w);
+                        float4 p = *b.rec(target, 0);
+                        *b.rec(target, 0) = make_float4(p.x + ct.x, p.y + ct.y, p.z + ct.z, p.w);
                     }
                 }
                 idx = -1;
@@ -483,18 +514,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         bool dead = false;
         uint32_t slot = 0;
         float4 rec0, rec1, rec2, rec3, e0, e1, h;
+        unsigned a = 0;   // the entry's storage index: the shading kernel reads the path state there
         if (valid) {
-            // the whole entry and its hit at once (a hole's are in bounds and unused): one memory
-            // round trip fewer before the shading record, which depends on the hit
-            const unsigned a = eqv.entry(i);
+            // the ray and its hit at once (a hole's are in bounds and unused): one memory round
+            // trip fewer before the shading record, which depends on the hit
+            a = eqv.entry(i);
             if (!start_fill) e0 = *b.ext(cur, a, 0);
             e1 = *b.ext(cur, a, 1);
             h = b.hits[i];
             slot = as_u(e1.w);
             valid = slot != kHole;
-            if (start_fill && valid) {
-                const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot, 3)->x)] : single;
-                e0 = make_float4(sv.camera.eye[0], sv.camera.eye[1], sv.camera.eye[2], 0.0f);
+            if (start_fill && valid) {   // a new path: the camera's eye, depth 0
+                const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot, 1)->x)] : single;
+                e0 = make_float4(sv.camera.eye[0], sv.camera.eye[1], sv.camera.eye[2], as_f(0u));
             }
         }
         if (valid) {
@@ -507,24 +539,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 V3 org = xyz(e0), dir = xyz(e1);
                 Surface sf = decode_hit(s, org, dir, tri, h.y, h.z, h.w);
                 if (sf.light >= 0 && sf.front) {
-                    float4 p0 = *b.rec(slot, 0), p1 = *b.rec(slot, 1);
+                    const float4 e2 = ext_throughput(b, cur, a, as_u(e0.w));
+                    float4 rad = *b.rec(slot, 0);
                     V3 Le;
                     float w;
-                    V3 C = xyz(p1) + hit_emission(s, cfg.flags, sf, dir, h.y, p0.w,
-                                                  1.0f / static_cast<float>(s.num_lights), xyz(p0), Le, w);
-                    *b.rec(slot, 1) = f4(C, p1.w);
+                    V3 C = xyz(rad) + hit_emission(s, cfg.flags, sf, dir, h.y, e2.w,
+                                                   1.0f / static_cast<float>(s.num_lights), xyz(e2), Le, w);
+                    *b.rec(slot, 0) = f4(C, rad.w);
                 }
                 if (sf.mat < 0) {
                     dead = true;
                 } else {
                     rec0 = f4(sf.point, as_f(static_cast<uint32_t>(sf.mat) | (sf.front ? kFrontBit : 0u)));
                     rec1 = f4(sf.n, as_f(slot));
-                    rec2 = f4(sf.face_n, 0.0f);
-                    rec3 = f4(dir, 0.0f);
+                    rec2 = f4(sf.face_n, e0.w);   // depth
+                    rec3 = f4(dir, as_f(a));      // where the path state is
                     dest = queue_of(sf.bsdf);
                 }
             }
-            if (dead) retire(b, *b.rec(slot, 1), *b.rec(slot, 2));
+            if (dead) retire(b, *b.rec(slot, 0), item_of(*b.ext(cur, a, 3)));
         }
         RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, dest, false, dead);
         if (plan.pos >= 0) {
@@ -536,11 +569,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         }
         if (plan.start) {
             ++started;
-            V3 org, dir;
-            init_path(tabs, b, plan.item, slot, org, dir);
-            const unsigned a = seg * b.cap_e + static_cast<unsigned>(plan.ext_pos);
-            *b.ext(nxt, a, 0) = f4(org, 0.0f);
-            *b.ext(nxt, a, 1) = f4(dir, as_f(slot));
+            init_path(tabs, b, plan.item, slot, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
         } else if (dead) {
             write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
         }
@@ -570,12 +599,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
         bool valid = i < count;
         uint32_t slot = 0;
         bool shoot = false, cont = false;
-        V3 point{0, 0, 0}, sdir{0, 0, 0}, contrib{0, 0, 0}, ndir{0, 0, 0};
-        unsigned long long item = 0;
+        V3 point{0, 0, 0}, sdir{0, 0, 0}, contrib{0, 0, 0}, ndir{0, 0, 0}, T_next{0, 0, 0};
+        float pdf_next = 0.0f;
+        uint32_t depth_next = 0;
+        unsigned long long item = 0, rng_next = 0;
         if (valid) {
             const unsigned a = mqv.entry(i);
             float4 r0 = *b.shade_entry(QUEUE, a, 0), r1 = *b.shade_entry(QUEUE, a, 1), r2 = *b.shade_entry(QUEUE, a, 2),
                    r3 = *b.shade_entry(QUEUE, a, 3);
+            // the path state stays in the extension entry the sort read (queue 1 - nxt)
+            const uint32_t depth0 = as_u(r2.w), at = as_u(r3.w);
+            const float4 r4 = ext_throughput(b, 1 - nxt, at, depth0), r5 = *b.ext(1 - nxt, at, 3);
             slot = as_u(r1.w);
             SceneView tmp;
             const SceneView& s = scene_of<MULTI>(single, tabs, b, slot, tmp);
@@ -593,12 +627,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
             if (QUEUE == kQueueDiffuse) m.type = SOL_BSDF_DIFFUSE;
             if (QUEUE == kQueuePhong && m.type != SOL_BSDF_INTERP) m.type = SOL_BSDF_PHONG;
             if (QUEUE == kQueueSpecular && m.type != SOL_BSDF_GLASS) m.type = SOL_BSDF_MIRROR;
-            float4 p0 = *b.rec(slot, 0), p1 = *b.rec(slot, 1), p2 = *b.rec(slot, 2);
-            item = item_of(p2);
-            V3 T = xyz(p0);
+            item = item_of(r5);
+            V3 T = xyz(r4);
             Pcg rng;
-            rng.state = static_cast<unsigned long long>(as_u(p2.x)) | (static_cast<unsigned long long>(as_u(p2.y)) << 32);
-            uint32_t depth = as_u(p1.w);
+            rng.state = rng_of(r5);
+            uint32_t depth = depth0;
             bool skip_nee = QUEUE == kQueueSpecular || (cfg.flags & SOL_PT_DISABLE_NEE);
             if (!skip_nee) {
                 NeeSample ns;
@@ -613,14 +646,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
                 depth += 1;
                 cont = depth < cfg.max_path_len;
                 ndir = bs.in;
-                if (cont) {
-                    *b.rec(slot, 0) = f4(T, skip_nee ? 0.0f : bs.pdf);
-                    *b.rec(slot, 1) = make_float4(p1.x, p1.y, p1.z, as_f(depth));
-                    *b.rec(slot, 2) = make_float4(as_f(static_cast<uint32_t>(rng.state)),
-                                                       as_f(static_cast<uint32_t>(rng.state >> 32)), p2.z, p2.w);
-                }
+                T_next = T;
+                pdf_next = skip_nee ? 0.0f : bs.pdf;
+                depth_next = depth;
+                rng_next = rng.state;
             }
-            if (!cont) retire(b, p1, p2);
+            if (!cont) retire(b, *b.rec(slot, 0), item);
         }
         bool retired = valid && !cont;
         RoundPlan plan = plan_round<1>(lds[parity], scnt, ecnt, b.items, seg, shoot ? 0 : -1, cont, retired);
@@ -631,19 +662,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
             *b.shadow(a, 2) = f4(contrib, as_f(static_cast<uint32_t>(item)));
         }
         if (plan.ext_pos >= 0) {
-            if (cont || plan.start) {
-                V3 org = point, dir = ndir;
-                float tmin = cfg.ray_offset;
-                if (plan.start) {
-                    ++started;
-                    init_path(tabs, b, plan.item, slot, org, dir);
-                    tmin = 0.0f;
-                }
-                const unsigned a = seg * b.cap_e + static_cast<unsigned>(plan.ext_pos);
-                *b.ext(nxt, a, 0) = f4(org, tmin);
-                *b.ext(nxt, a, 1) = f4(dir, as_f(slot));
+            const unsigned a = seg * b.cap_e + static_cast<unsigned>(plan.ext_pos);
+            if (cont) {
+                write_ext(b, nxt, a, point, depth_next, ndir, slot, T_next, pdf_next, rng_next, item);
+            } else if (plan.start) {
+                ++started;
+                init_path(tabs, b, plan.item, slot, nxt, a);
             } else {
-                write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
+                write_hole(b, nxt, a);
             }
         }
     }
@@ -669,13 +695,15 @@ SOL_HD unsigned start_slots(unsigned slots, unsigned seg) {   // slots of segmen
 // Starts a path in every slot (beginning of a chunk of work), in the layout above and without
 // atomics: k_set_chunk gave segment g the items from items[item_next(g) + 1], and the slot at
 // position p of segment g takes item items[item_next(g) + 1] + p, if below items[item_next(g)].
-// A wave fills 64 consecutive slots at a time. Their path records are one contiguous 4 KB block:
+// A wave fills 64 consecutive slots at a time. Their slot records are one contiguous 2 KB block:
 // written from each lane directly, a 16-byte store of all lanes would touch one line per lane, so
-// the wave stages its records in LDS and stores the block in four 1 KB runs, lane l writing word
-// l % 4 of record l / 4 of each run (records of holes are skipped). The wave's own LDS writes
-// and reads complete in order, so no barrier is needed.
+// the wave stages its records in LDS and stores the block in two 1 KB runs, lane l writing word
+// l % 2 of record l / 2 of each run (records of holes are skipped). The wave's own LDS writes and
+// reads complete in order, so no barrier is needed. Of the extension entries only the direction
+// and slot and the PCG state and item are written: while the start flag is set, the origin is
+// the camera's eye, the depth 0 and the throughput 1.
 __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, int nxt) {
-    __shared__ float4 stage[kWaves][256];
+    __shared__ float4 stage[kWaves][128];
     float4* st = stage[threadIdx.x >> 6];
     const unsigned lane = __lane_id();
     const unsigned wave_step = gridDim.x * kBlock;
@@ -690,21 +718,20 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
             if (item < b.items[item_next(static_cast<int>(seg))]) {
                 start = true;
                 ++started;
-                V3 org, dir;
-                float4 rec[4];
-                start_path(tabs, item, org, dir, rec);
-                for (int k = 0; k < 4; ++k) st[lane * 4 + k] = rec[k];
-                // no first word: while the start flag is set, the origin is the camera's eye
-                *b.ext(nxt, entry, 1) = f4(dir, as_f(slot));
+                const PathStart ps = start_path(tabs, item);
+                st[lane * 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                st[lane * 2 + 1] = make_float4(as_f(ps.scene), 0.0f, 0.0f, 0.0f);
+                *b.ext(nxt, entry, 1) = f4(ps.dir, as_f(slot));
+                *b.ext(nxt, entry, 3) = rng_item_word(ps.rng, item);
             } else {
                 write_hole(b, nxt, entry);
             }
         }
         const unsigned long long starts = __ballot(start);
         __builtin_amdgcn_wave_barrier();
-        for (unsigned k = 0; k < 4; ++k) {
-            const unsigned r = 16 * k + lane / 4;   // record r of the wave's 64
-            if ((starts >> r) & 1ull) b.path[(base + r) * 4 + lane % 4] = st[64 * k + lane];
+        for (unsigned k = 0; k < 2; ++k) {
+            const unsigned r = 32 * k + lane / 2;   // record r of the wave's 64
+            if ((starts >> r) & 1ull) b.path[(base + r) * 2 + lane % 2] = st[64 * k + lane];
         }
         __builtin_amdgcn_wave_barrier();
     }

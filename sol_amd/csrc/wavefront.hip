@@ -67,9 +67,9 @@ struct WavefrontState {
         const size_t R = kWaveBlock;
         auto round64 = [](size_t v) { return (v + 63) & ~size_t(63); };
         const size_t cap_e = round64(n / 4 + 8 * R), cap_m = round64(n / 8 + 2 * R), cap_s = round64(n / 8 + 6 * R);
-        // path record 64 B and hit 16 B per slot; queue entries 32 B (extension), 64 B (shade),
+        // slot record 32 B and hit 16 B per slot; queue entries 64 B (extension), 96 B (shade),
         // 48 B (shadow) per segment entry
-        size_t f4_total = size_t(n) * (4 + 1) + kNumSegments * (2 * cap_e * 2 + kNumShadeQueues * cap_m * 4 + cap_s * 3);
+        size_t f4_total = size_t(n) * (2 + 1) + kNumSegments * (2 * cap_e * 4 + kNumShadeQueues * cap_m * 4 + cap_s * 3);
         size_t bytes = sizeof(float4) * f4_total + 16 * 4096;
         HIP_OK(arena.alloc(bytes));
         HIP_OK(counters.alloc(sizeof(unsigned) * kCounterWords + align_up(sizeof(unsigned long long) * kItemWords) + 256));
@@ -83,9 +83,9 @@ struct WavefrontState {
         b.stride_e = static_cast<uint32_t>(kNumSegments * cap_e);
         b.stride_m = static_cast<uint32_t>(kNumSegments * cap_m);
         b.stride_s = static_cast<uint32_t>(kNumSegments * cap_s);
-        b.path = f4s(size_t(n) * 4);
+        b.path = f4s(size_t(n) * 2);
         b.hits = f4s(n);
-        for (int q = 0; q < 2; ++q) b.eq[q] = f4s(kNumSegments * cap_e * 2);
+        for (int q = 0; q < 2; ++q) b.eq[q] = f4s(kNumSegments * cap_e * 4);
         for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = f4s(kNumSegments * cap_m * 4);
         b.sq = f4s(kNumSegments * cap_s * 3);
         char* c = counters.as<char>();
