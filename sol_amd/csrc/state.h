@@ -37,12 +37,12 @@ struct DeviceBuffer {
 
 struct WavefrontState;  // defined in wavefront.hip
 
-// Device memory of the wavefront state per path slot: the slot record and the hit (48 B) and the
+// Device memory of the wavefront state per path slot: the slot record and the hit (32 B) and the
 // queue segments (two extension queues 256 B, three shade queues 192 B, the shadow queue 48 B).
-constexpr unsigned long long kPathSlotBytes = 544;
+constexpr unsigned long long kPathSlotBytes = 528;
 
 // Paths in flight by default: the largest power of two up to 2^27 whose state takes at most a
-// quarter of the device's memory (2^27 slots, 73 GB, on a 288 GB MI355X: a 1080p 64-spp frame is
+// quarter of the device's memory (2^27 slots, 71 GB, on a 288 GB MI355X: a 1080p 64-spp frame is
 // then in flight at once; 2^24 slots where the memory size is not known).
 inline uint32_t default_path_slots(unsigned long long device_memory) {
     if (device_memory == 0) return 1u << 24;

@@ -71,14 +71,14 @@ enum : int {
 // entries (the hit record and where the path state is, 64 B), shadow rays (48 B). A path's
 // throughput and PCG state stay in its extension entry until the shading kernel has read them
 // (nothing writes that queue during the bounce), so the sort does not copy them. Only what other kernels add to
-// out of order stays in a 32-byte record per slot (random access: one cache line): the radiance
-// (emission added by the sort, unoccluded light samples by the trace) and the scene. Appends are
+// out of order stays in a 16-byte record per slot (random access): the radiance (emission added by
+// the sort, unoccluded light samples by the trace) and the scene. Appends are
 // compacted per workgroup (one atomic per round of kWaveBlock entries, queue and segment).
 // A path that ends writes its radiance to its sample's entry of the sample buffer and the slot
 // starts the next work item at once (in the kernel that ended it); a shadow ray still in flight
 // for the ended path adds its contribution to that sample entry instead of the slot record.
 //
-// slot record (2 x float4):   [0] radiance xyz, -  [1] scene index (uint bits), -
+// slot record (float4):       radiance xyz, scene index (uint bits; kept by every update)
 // extension entry (4 x float4): [0] org xyz, depth (uint bits; 0: a new path, traced from tmin 0,
 //                             else from the configured ray offset)  [1] dir xyz, slot
 //                             [2] throughput xyz, pdf of the previous bounce (not stored at depth
@@ -101,7 +101,7 @@ struct WaveBuf {
     uint32_t slots;
     uint32_t cap_e, cap_m, cap_s;     // segment capacities: extension, shade, shadow queues
     uint32_t stride_e, stride_m, stride_s;   // kNumSegments * cap_e, cap_m, cap_s (the queues' field strides)
-    float4* path;                     // [slots * 2]: slot records
+    float4* path;                     // [slots]: slot records
     float4* hits;                     // [slots]: closest hit of extension entry i (in queue order):
                                       // leaf-order triangle (int bits, -1 = miss), t, u, v
     float4* eq[2];                    // [kNumSegments * cap_e * 4] each
@@ -116,10 +116,10 @@ struct WaveBuf {
     // Queue entries are stored field-major: 16-byte word k of entry a (its storage index, segment *
     // capacity + position) follows every entry's word k - 1, so a wave reading or writing one word
     // of consecutive entries touches consecutive cache lines, a quarter as many as with whole
-    // entries side by side (the texture path's cost is per line). Slot records stay whole (one
-    // line): they are reached by slot, which consecutive entries do not share in order. Word
+    // entries side by side (the texture path's cost is per line). Slot records are reached by
+    // slot, which consecutive entries do not share in order. Word
     // indices fit 32 bits: at 2^27 slots an extension queue has 4 words of 2^28 entries.
-    SOL_DEV float4* rec(uint32_t slot, uint32_t k) const { return path + (slot * 2u + k); }
+    SOL_DEV float4* rec(uint32_t slot) const { return path + slot; }
     SOL_DEV float4* ext(int q, uint32_t a, uint32_t k) const { return eq[q] + (k * stride_e + a); }
     SOL_DEV float4* shade_entry(int q, uint32_t a, uint32_t k) const { return mq[q] + (k * stride_m + a); }
     SOL_DEV float4* shadow(uint32_t a, uint32_t k) const { return sq + (k * stride_s + a); }

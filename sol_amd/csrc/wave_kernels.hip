@@ -202,7 +202,7 @@ template <bool MULTI>
 SOL_DEV const SceneView& scene_of(const SceneView& single, const WaveTables& t, const WaveBuf& b, uint32_t slot,
                                   SceneView& tmp) {
     if (!MULTI) return single;
-    tmp = t.scenes[as_u(b.rec(slot, 1)->x)];
+    tmp = t.scenes[as_u(b.rec(slot)->w)];
     return tmp;
 }
 
@@ -300,8 +300,7 @@ SOL_DEV float4 ext_throughput(const WaveBuf& b, int q, uint32_t a, uint32_t dept
 SOL_DEV void init_path(const WaveTables& t, const WaveBuf& b, unsigned long long item, uint32_t slot, int q,
                        uint32_t a) {
     const PathStart ps = start_path(t, item);
-    *b.rec(slot, 0) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    *b.rec(slot, 1) = make_float4(as_f(ps.scene), 0.0f, 0.0f, 0.0f);
+    *b.rec(slot) = make_float4(0.0f, 0.0f, 0.0f, as_f(ps.scene));
     write_ext(b, q, a, ps.org, 0u, ps.dir, slot, V3{1.0f, 1.0f, 1.0f}, 0.0f, ps.rng, item);
 }
 
@@ -405,7 +404,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         any = false;
                         float4 e0;
                         if (start_fill) {
-                            const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot, 1)->x)] : single;
+                            const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot)->w)] : single;
                             e0 = make_float4(sv.camera.eye[0], sv.camera.eye[1], sv.camera.eye[2], as_f(0u));
                         } else {
                             e0 = *b.ext(cur, a, 0);
@@ -414,7 +413,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         r = make_ray_pre(xyz(e0), xyz(e1));
                         ts.start(FLT_MAX);
                         if (MULTI) {
-                            const SceneView& sv = tabs.scenes[as_u(b.rec(slot, 1)->x)];
+                            const SceneView& sv = tabs.scenes[as_u(b.rec(slot)->w)];
                             fg = FetchGlobal(sv);
                             fg4 = FetchGlobal4(sv);
                         }
@@ -436,7 +435,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                                                       as_u(b.shadow(sa, 2)->w);
                             sid = tabs.job_scene[find_job(tabs, item)];
                         } else {
-                            sid = as_u(b.rec(target, 1)->x);
+                            sid = as_u(b.rec(target)->w);
                         }
                         fg = FetchGlobal(tabs.scenes[sid]);
                         fg4 = FetchGlobal4(tabs.scenes[sid]);
@@ -480,8 +479,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         float4 v = *d;
                         *d = make_float4(v.x + ct.x, v.y + ct.y, v.z + ct.z, 0.0f);
                     } else {
-                        float4 p = *b.rec(target, 0);
-                        *b.rec(target, 0) = make_float4(p.x + ct.x, p.y + ct.y, p.z + ct.z, p.w);
+                        float4 p = *b.rec(target);
+                        *b.rec(target) = make_float4(p.x + ct.x, p.y + ct.y, p.z + ct.z, p.w);
                     }
                 }
                 idx = -1;
@@ -525,7 +524,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             slot = as_u(e1.w);
             valid = slot != kHole;
             if (start_fill && valid) {   // a new path: the camera's eye, depth 0
-                const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot, 1)->x)] : single;
+                const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot)->w)] : single;
                 e0 = make_float4(sv.camera.eye[0], sv.camera.eye[1], sv.camera.eye[2], as_f(0u));
             }
         }
@@ -540,12 +539,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 Surface sf = decode_hit(s, org, dir, tri, h.y, h.z, h.w);
                 if (sf.light >= 0 && sf.front) {
                     const float4 e2 = ext_throughput(b, cur, a, as_u(e0.w));
-                    float4 rad = *b.rec(slot, 0);
+                    float4 rad = *b.rec(slot);
                     V3 Le;
                     float w;
                     V3 C = xyz(rad) + hit_emission(s, cfg.flags, sf, dir, h.y, e2.w,
                                                    1.0f / static_cast<float>(s.num_lights), xyz(e2), Le, w);
-                    *b.rec(slot, 0) = f4(C, rad.w);
+                    *b.rec(slot) = f4(C, rad.w);
                 }
                 if (sf.mat < 0) {
                     dead = true;
@@ -557,7 +556,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                     dest = queue_of(sf.bsdf);
                 }
             }
-            if (dead) retire(b, *b.rec(slot, 0), item_of(*b.ext(cur, a, 3)));
+            if (dead) retire(b, *b.rec(slot), item_of(*b.ext(cur, a, 3)));
         }
         RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, dest, false, dead);
         if (plan.pos >= 0) {
@@ -651,7 +650,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
                 depth_next = depth;
                 rng_next = rng.state;
             }
-            if (!cont) retire(b, *b.rec(slot, 0), item);
+            if (!cont) retire(b, *b.rec(slot), item);
         }
         bool retired = valid && !cont;
         RoundPlan plan = plan_round<1>(lds[parity], scnt, ecnt, b.items, seg, shoot ? 0 : -1, cont, retired);
@@ -695,45 +694,28 @@ SOL_HD unsigned start_slots(unsigned slots, unsigned seg) {   // slots of segmen
 // Starts a path in every slot (beginning of a chunk of work), in the layout above and without
 // atomics: k_set_chunk gave segment g the items from items[item_next(g) + 1], and the slot at
 // position p of segment g takes item items[item_next(g) + 1] + p, if below items[item_next(g)].
-// A wave fills 64 consecutive slots at a time. Their slot records are one contiguous 2 KB block:
-// written from each lane directly, a 16-byte store of all lanes would touch one line per lane, so
-// the wave stages its records in LDS and stores the block in two 1 KB runs, lane l writing word
-// l % 2 of record l / 2 of each run (records of holes are skipped). The wave's own LDS writes and
-// reads complete in order, so no barrier is needed. Of the extension entries only the direction
-// and slot and the PCG state and item are written: while the start flag is set, the origin is
-// the camera's eye, the depth 0 and the throughput 1.
+// A wave fills 64 consecutive slots at a time, so its slot records are one contiguous 1 KB run. Of
+// the extension entries only the direction and slot and the PCG state and item are written: while
+// the start flag is set, the origin is the camera's eye, the depth 0 and the throughput 1.
 __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, int nxt) {
-    __shared__ float4 stage[kWaves][128];
-    float4* st = stage[threadIdx.x >> 6];
-    const unsigned lane = __lane_id();
     const unsigned wave_step = gridDim.x * kBlock;
     unsigned started = 0;
     for (unsigned base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < b.slots; base += wave_step) {
-        const unsigned slot = base + lane;
-        bool start = false;
+        const unsigned slot = base + __lane_id();
         if (slot < b.slots) {
             const unsigned seg = start_segment(slot), pos = start_position(slot);
             const unsigned long long item = b.items[item_next(static_cast<int>(seg)) + 1] + pos;
             const unsigned entry = seg * b.cap_e + pos;
             if (item < b.items[item_next(static_cast<int>(seg))]) {
-                start = true;
                 ++started;
                 const PathStart ps = start_path(tabs, item);
-                st[lane * 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                st[lane * 2 + 1] = make_float4(as_f(ps.scene), 0.0f, 0.0f, 0.0f);
+                *b.rec(slot) = make_float4(0.0f, 0.0f, 0.0f, as_f(ps.scene));
                 *b.ext(nxt, entry, 1) = f4(ps.dir, as_f(slot));
                 *b.ext(nxt, entry, 3) = rng_item_word(ps.rng, item);
             } else {
                 write_hole(b, nxt, entry);
             }
         }
-        const unsigned long long starts = __ballot(start);
-        __builtin_amdgcn_wave_barrier();
-        for (unsigned k = 0; k < 2; ++k) {
-            const unsigned r = 32 * k + lane / 2;   // record r of the wave's 64
-            if ((starts >> r) & 1ull) b.path[(base + r) * 2 + lane % 2] = st[64 * k + lane];
-        }
-        __builtin_amdgcn_wave_barrier();
     }
     wave_add_stat(&b.stats[2], started);
 }

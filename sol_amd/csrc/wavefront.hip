@@ -69,7 +69,7 @@ struct WavefrontState {
         const size_t cap_e = round64(n / 4 + 8 * R), cap_m = round64(n / 8 + 2 * R), cap_s = round64(n / 8 + 6 * R);
         // slot record 32 B and hit 16 B per slot; queue entries 64 B (extension), 96 B (shade),
         // 48 B (shadow) per segment entry
-        size_t f4_total = size_t(n) * (2 + 1) + kNumSegments * (2 * cap_e * 4 + kNumShadeQueues * cap_m * 4 + cap_s * 3);
+        size_t f4_total = size_t(n) * (1 + 1) + kNumSegments * (2 * cap_e * 4 + kNumShadeQueues * cap_m * 4 + cap_s * 3);
         size_t bytes = sizeof(float4) * f4_total + 16 * 4096;
         HIP_OK(arena.alloc(bytes));
         HIP_OK(counters.alloc(sizeof(unsigned) * kCounterWords + align_up(sizeof(unsigned long long) * kItemWords) + 256));
@@ -83,7 +83,7 @@ struct WavefrontState {
         b.stride_e = static_cast<uint32_t>(kNumSegments * cap_e);
         b.stride_m = static_cast<uint32_t>(kNumSegments * cap_m);
         b.stride_s = static_cast<uint32_t>(kNumSegments * cap_s);
-        b.path = f4s(size_t(n) * 2);
+        b.path = f4s(size_t(n));
         b.hits = f4s(n);
         for (int q = 0; q < 2; ++q) b.eq[q] = f4s(kNumSegments * cap_e * 4);
         for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = f4s(kNumSegments * cap_m * 4);

@@ -60,14 +60,14 @@ def test_autotune_returns_valid_parameters_and_keeps_the_image(ctx):
 
 
 def test_default_paths_in_flight_follow_device_memory(ctx):
-    # the largest power of two up to 2^27 whose 544 bytes a slot fit in a quarter of the memory
+    # the largest power of two up to 2^27 whose 528 bytes a slot fit in a quarter of the memory
     mem = ctx.device_info().total_memory
     ctx.set_tuning(path_slots=0)
     want = 1 << 27
-    while want > 4096 and want * 544 > mem // 4:
+    while want > 4096 and want * 528 > mem // 4:
         want //= 2
     assert ctx.tuning().path_slots == want
-    if mem >= 4 * 544 * (1 << 27):          # an MI355X (288 GB): a 1080p 64-spp frame in flight at once
+    if mem >= 4 * 528 * (1 << 27):          # an MI355X (288 GB): a 1080p 64-spp frame in flight at once
         assert want == 1 << 27
 
 
