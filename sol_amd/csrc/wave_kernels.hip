@@ -513,13 +513,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         bool dead = false;
         uint32_t slot = 0;
         float4 rec0, rec1, rec2, rec3, e0, e1, h;
-        unsigned a = 0;   // the entry's storage index: the shading kernel reads the path state there
+        unsigned ea = 0;   // the entry's storage index: the shading kernel reads the path state there
         if (valid) {
             // the ray and its hit at once (a hole's are in bounds and unused): one memory round
             // trip fewer before the shading record, which depends on the hit
-            a = eqv.entry(i);
-            if (!start_fill) e0 = *b.ext(cur, a, 0);
-            e1 = *b.ext(cur, a, 1);
+            ea = eqv.entry(i);
+            if (!start_fill) e0 = *b.ext(cur, ea, 0);
+            e1 = *b.ext(cur, ea, 1);
             h = b.hits[i];
             slot = as_u(e1.w);
             valid = slot != kHole;
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 V3 org = xyz(e0), dir = xyz(e1);
                 Surface sf = decode_hit(s, org, dir, tri, h.y, h.z, h.w);
                 if (sf.light >= 0 && sf.front) {
-                    const float4 e2 = ext_throughput(b, cur, a, as_u(e0.w));
+                    const float4 e2 = ext_throughput(b, cur, ea, as_u(e0.w));
                     float4 rad = *b.rec(slot);
                     V3 Le;
                     float w;
@@ -552,11 +552,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                     rec0 = f4(sf.point, as_f(static_cast<uint32_t>(sf.mat) | (sf.front ? kFrontBit : 0u)));
                     rec1 = f4(sf.n, as_f(slot));
                     rec2 = f4(sf.face_n, e0.w);   // depth
-                    rec3 = f4(dir, as_f(a));      // where the path state is
+                    rec3 = f4(dir, as_f(ea));     // where the path state is
                     dest = queue_of(sf.bsdf);
                 }
             }
-            if (dead) retire(b, *b.rec(slot), item_of(*b.ext(cur, a, 3)));
+            if (dead) retire(b, *b.rec(slot), item_of(*b.ext(cur, ea, 3)));
         }
         RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, dest, false, dead);
         if (plan.pos >= 0) {
