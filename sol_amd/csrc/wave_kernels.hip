@@ -580,7 +580,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 
 // One kernel per queue; the material type of the queue is a compile-time constant (glass and
 // mirror share the specular queue), so each kernel carries only its BSDF's code. The glossy
-// kernel (Phong and the diffuse-Phong mix) is held to three waves per SIMD (168 VGPRs).
+// kernel (Phong and the diffuse-Phong mix) is held to three waves per SIMD (at most 168 VGPRs).
 template <int QUEUE, bool MULTI>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE == kQueuePhong ? 3 : 1))) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                   int nxt) {
