@@ -16,7 +16,9 @@ Also reported:
                 workload, if present; "dram_gbs" / "dram_frac" are that traffic over the measured
                 launch time, against the peak. "copy_gbs" is a device-to-device copy timed here.
   cpu_baseline  the CPU oracle (C++ restatement of sol's path tracer, OpenMP) on the host's cores,
-                on a bounded sample of the same frame (all pixels, fewer samples per pixel).
+                on a bounded sample of the same frame (all pixels; by default all 64 samples, about
+                11 s on 16 threads, and then the oracle's image is also compared with the wavefront
+                image: "oracle_identical_pixels", as "megakernel_identical_pixels" for the megakernel).
   intersect     the intersection microbenchmark: 1M-triangle soup, 4,194,304 closest-hit rays and
                 4,194,304 any-hit rays (device buffers), Mrays/s; the oracle on 262,144 of each.
   cornell       the Cornell box at 256x256, 16 spp, depth 8: both GPU renderers and the oracle.
@@ -371,7 +373,7 @@ def main():
     log(f"wavefront (HIP-event profiled): {prof_s * 1e3:.1f} ms/frame; per frame trace {wst.trace_ms / args.steps:.1f} ms, "
         f"sort {wst.sort_ms / args.steps:.1f} ms, shade {wst.shade_ms / args.steps:.1f} ms")
 
-    mega = None
+    mega, identical = None, None
     if not args.no_megakernel:
         mega_s, mst = timed("mega", profile=False)
         mega = samples_per_step / mega_s / 1e6
@@ -423,11 +425,21 @@ def main():
         t = time.perf_counter()
         osc.render(one, 0, 1, cfg, threads=1)
         dt1 = time.perf_counter() - t
+        # The oracle's frame, accumulated as often as the timed wavefront frames were (the same fp32
+        # additions in the same order), against the wavefront image: the checker at full size.
+        oracle_identical = None
+        if args.cpu_spp == args.spp:
+            acc = np.zeros_like(cimg)
+            for _ in range(args.steps):
+                acc = acc + cimg
+            oracle_identical = float(np.all(img["wave"].cpu().numpy() == acc, axis=0).mean())
+            log(f"wavefront vs oracle: identical pixels {oracle_identical:.6f}")
         cpu = {"value": WIDTH * HEIGHT * args.cpu_spp / dt / 1e6, "unit": "Msamples/s", "cores": threads,
                "kind": "port",
                "sample": f"1920x1080 at {args.cpu_spp} spp (samples 0..{args.cpu_spp - 1}) of the same frame, "
                          f"oracle over the same BVH, {dt:.1f} s",
                "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+               "oracle_identical_pixels": oracle_identical,
                "one_thread": {"value": round(WIDTH * HEIGHT / dt1 / 1e6, 3),
                               "sample": f"1920x1080 at 1 spp (sample 0), {dt1:.1f} s"}}
         log(f"cpu oracle: {cpu['value']:.2f} Msamples/s on {threads} threads; one thread "
@@ -451,6 +463,7 @@ def main():
                    "triangles": scene.num_tris, "renderer": "wavefront"},
         "megakernel_msamples_s": round(mega, 2) if mega else None,
         "speedup_vs_megakernel": round(value / mega, 3) if mega else None,
+        "megakernel_identical_pixels": identical,
         "mrays_s": round(mrays, 1),
         "device": info.name.decode(errors="replace"),
         "roofline": {"bound": "hbm", "kernel": "k_trace", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
