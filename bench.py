@@ -315,8 +315,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. A frame renders on one MI355X; "
-              "across GPUs the work splits by sample ranges or image tiles with one final image sum, which this "
-              "build does not run", file=sys.stderr)
+              "across GPUs the work splits by sample ranges or image tiles with one final image sum, which is "
+              "not implemented", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

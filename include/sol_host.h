@@ -163,7 +163,7 @@ int sol_scene_set_lights(sol_scene* scene, const sol_light* l, uint32_t n);
 
 /* Image files: sol::Image::save (reference src/image.cpp:26-37). format: 0 = auto (EXR, sol's
    default), 4 = EXR (uncompressed FLOAT scanlines, channels B, G, R as src/formats/exr.cpp:48-100
-   writes them), 5 = PFM; 1-3 (PNG, JPEG, TIFF) have no codec in this build and fail. pixels is
+   writes them), 5 = PFM; 1-3 (PNG, JPEG, TIFF) have no codec in this library and fail. pixels is
    planar fp32, channel by channel, rows top to bottom. */
 enum {
     SOL_IMAGE_AUTO = 0, SOL_IMAGE_PNG = 1, SOL_IMAGE_JPEG = 2, SOL_IMAGE_TIFF = 3, SOL_IMAGE_EXR = 4,

@@ -1,7 +1,7 @@
 // Planar fp32 image, as sol::Image (reference include/sol/image.h:18-98, src/image.cpp:13-41).
 // Pixel (0, 0) is the top-left corner. Files: OpenEXR (sol's default format, written as sol's
 // exr::save writes it: uncompressed 32-bit float scanlines, channels B, G, R) and PFM; PNG, JPEG
-// and TIFF need codec libraries this build does not have, so saving in them fails, as sol's does
+// and TIFF need codec libraries this library does not link, so saving in them fails, as sol's does
 // when it is built without them.
 #ifndef SOL_IMAGE_H
 #define SOL_IMAGE_H

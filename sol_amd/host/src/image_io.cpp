@@ -116,7 +116,7 @@ bool Image::save(std::string_view path, Format format) const {
         case Format::Auto:
         case Format::Exr: return save_exr(path);
         case Format::Pfm: return save_pfm(path);
-        default: return false;    // PNG, JPEG, TIFF: no codec library in this build
+        default: return false;    // PNG, JPEG, TIFF: no codec library is linked
     }
 }
 

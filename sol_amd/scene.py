@@ -217,7 +217,7 @@ _IMAGE_FORMATS = {"auto": F.IMAGE_AUTO, "png": F.IMAGE_PNG, "jpeg": F.IMAGE_JPEG
 
 def save_image(path, image, format="auto"):
     """Writes a planar (channels, height, width) float32 image as sol::Image::save does: EXR by
-    default, or PFM; PNG / JPEG / TIFF have no codec in this build and raise."""
+    default, or PFM; PNG / JPEG / TIFF have no codec in this library and raise."""
     img = np.ascontiguousarray(image, np.float32)
     if img.ndim != 3:
         raise ValueError("image must be (channels, height, width)")
