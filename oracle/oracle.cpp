@@ -9,13 +9,17 @@
 //                                      generate_canonical<float,24>              KATs; float edge case
 //   pixel_seed (FNV-1a 32, 8 LE bytes) include/sol/renderer.h:44-46 (proto::fnv   FNV-1a published test
 //                                      not vendored: FNV-1a assumed)             vectors
-//   sample_pixel / camera ray          renderer.h:50-54, cameras.cpp:5-25        closed forms (centre, corners)
+//   sample_pixel / camera ray          renderer.h:50-54, cameras.cpp:5-25        float64 closed forms: centre,
+//                                                                               corners, fov and aspect angles,
+//                                                                               footprint, jitter draw order,
+//                                                                               an emitter's projection
 //   watertight ray/triangle            JCGT 2013 (Woop et al.); replaces          hand-computed cases,
 //                                      proto::PrecomputedTriangle::intersect     watertightness, brute force
 //   BVH traversal (closest / any)      triangle_mesh.cpp:64-117                  brute force over all triangles
 //   Diffuse / Phong / Glass / Mirror   bsdfs.cpp:11-165, bsdfs.h:89-94           closed forms: pdf integrals,
 //                                                                               Fresnel at normal incidence,
-//                                                                               TIR, reflection law
+//                                                                               TIR, mirror reflection law
+//                                                                               and rejection below the face
 //   Interp (diffuse + Phong mix)       bsdfs.cpp:179-230, color.h:66-71           mix 0 / 1 = the single lobe,
 //                                      (proto::lerp not vendored:                 pdf integral, lobe-pick
 //                                      a (1 - t) + b t assumed)                   frequency
@@ -732,6 +736,25 @@ void accumulate_stats(oracle_stats* out, const Tally& t, uint64_t samples) {
     out->samples += samples;
 }
 
+// Renderer::sample_pixel (renderer.h:50-54): the pixel's area in camera space [-1, 1]^2, y down
+// (pixel (0, 0) is the top-left corner, v = +1); the x jitter is drawn first.
+struct Uv {
+    float u, v;
+};
+template <typename Rng>
+Uv sample_pixel(Rng& sampler, uint32_t x, uint32_t y, uint32_t w, uint32_t h) {
+    float jx = sampler();
+    float jy = sampler();
+    return {(static_cast<float>(x) + jx) * (2.0f / static_cast<float>(w)) - 1.0f,
+            1.0f - (static_cast<float>(y) + jy) * (2.0f / static_cast<float>(h))};
+}
+
+// PerspectiveCamera::generate_ray (cameras.cpp:23-25) over the vectors its constructor scaled
+// (cameras.cpp:5-21, in oracle_scene_create).
+Ray camera_ray(const oracle_scene& s, Uv uv) {
+    return Ray{s.eye, normalize(s.cam_dir + s.cam_right * uv.u + s.cam_up * uv.v), 0.0f, FLT_MAX};
+}
+
 // Scripted random sequence for the BSDF closed-form checks.
 struct Script {
     const float* values;
@@ -854,13 +877,7 @@ int oracle_render(const oracle_scene* s, const sol_pt_config* cfg, uint32_t w, u
             Color color{0.0f, 0.0f, 0.0f};
             for (uint32_t i = 0; i < sample_count; ++i) {
                 Pcg sampler(pixel_seed(sample_index + i, x, y));
-                // sample_pixel (renderer.h:50-54), x jitter drawn first.
-                float jx = sampler();
-                float jy = sampler();
-                float u = (static_cast<float>(x) + jx) * (2.0f / static_cast<float>(w)) - 1.0f;
-                float v = 1.0f - (static_cast<float>(y) + jy) * (2.0f / static_cast<float>(h));
-                // generate_ray (cameras.cpp:23-25)
-                Ray ray{s->eye, normalize(s->cam_dir + s->cam_right * u + s->cam_up * v), 0.0f, FLT_MAX};
+                Ray ray = camera_ray(*s, sample_pixel(sampler, static_cast<uint32_t>(x), y, w, h));
                 color = color + trace_path(*s, *cfg, sampler, ray, tally);
             }
             size_t p = size_t(y) * w + x;
@@ -922,11 +939,7 @@ int oracle_backward(const oracle_scene* s, const sol_pt_config* cfg, uint32_t w,
             std::vector<Color> suffix;
             for (uint32_t i = 0; i < sample_count; ++i) {
                 Pcg sampler(pixel_seed(sample_index + i, x, y));
-                float jx = sampler();
-                float jy = sampler();
-                float u = (static_cast<float>(x) + jx) * (2.0f / static_cast<float>(w)) - 1.0f;
-                float v = 1.0f - (static_cast<float>(y) + jy) * (2.0f / static_cast<float>(h));
-                Ray ray{s->eye, normalize(s->cam_dir + s->cam_right * u + s->cam_up * v), 0.0f, FLT_MAX};
+                Ray ray = camera_ray(*s, sample_pixel(sampler, static_cast<uint32_t>(x), static_cast<uint32_t>(y), w, h));
                 path.clear();
                 trace_path(*s, *cfg, sampler, ray, tally, &path);
                 const size_t K = path.size();
@@ -1011,5 +1024,20 @@ int oracle_bsdf_sample(const sol_material* m, const float* n3, const float* out3
 }
 
 float oracle_fresnel(float eta, float cos_i, float cos_t) { return fresnel_factor(eta, cos_i, cos_t); }
+
+void oracle_sample_pixel(uint64_t sample_index, uint32_t x, uint32_t y, uint32_t w, uint32_t h, float* uv) {
+    Pcg sampler(pixel_seed(sample_index, x, y));
+    Uv p = sample_pixel(sampler, x, y, w, h);
+    uv[0] = p.u;
+    uv[1] = p.v;
+}
+
+int oracle_camera_ray(const oracle_scene* s, float u, float v, float* org3, float* dir3) {
+    if (!s || !org3 || !dir3) return -1;
+    Ray r = camera_ray(*s, Uv{u, v});
+    org3[0] = r.org.x; org3[1] = r.org.y; org3[2] = r.org.z;
+    dir3[0] = r.dir.x; dir3[1] = r.dir.y; dir3[2] = r.dir.z;
+    return 0;
+}
 
 } // extern "C"

@@ -81,6 +81,13 @@ int oracle_ray_triangle(const float* org3, const float* dir3, const float* v0, c
 int oracle_bsdf_sample(const sol_material* m, const float* n3, const float* out3, int front, const float* randoms,
                        float* in3, float* pdf, float* cos, float* rgb);
 float oracle_fresnel(float eta, float cos_i, float cos_t);
+/* Renderer::sample_pixel (include/sol/renderer.h:50-54) of pixel (x, y) of a w x h image for the
+   absolute sample index: the first two draws of the pixel's stream, as oracle_render makes them.
+   uv[0] = u (left -1, right +1), uv[1] = v (top +1, bottom -1). */
+void oracle_sample_pixel(uint64_t sample_index, uint32_t x, uint32_t y, uint32_t w, uint32_t h, float* uv);
+/* PerspectiveCamera::generate_ray (src/cameras.cpp:23-25) of the scene's camera at (u, v), as
+   oracle_render makes it: origin and unit direction. */
+int oracle_camera_ray(const oracle_scene* s, float u, float v, float* org3, float* dir3);
 
 #ifdef __cplusplus
 }

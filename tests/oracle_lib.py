@@ -44,6 +44,9 @@ lib.oracle_ray_triangle.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
 lib.oracle_bsdf_sample.argtypes = [C.POINTER(F.Material), _P, _P, C.c_int, _P, _P, _P, _P, _P]
 lib.oracle_fresnel.restype = C.c_float
 lib.oracle_fresnel.argtypes = [C.c_float, C.c_float, C.c_float]
+lib.oracle_sample_pixel.restype = None
+lib.oracle_sample_pixel.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]
+lib.oracle_camera_ray.argtypes = [_P, C.c_float, C.c_float, _P, _P]
 
 
 def f32(*v):
@@ -96,6 +99,64 @@ class OracleScene:
         self._h = h
 
     @classmethod
+    def with_bvh(cls, scene, bounds, index, count, prims):
+        """The oracle's copy of scene traversing the given bvh v2-layout tree: bounds (N, 6) float32
+        (min x, max x, min y, max y, min z, max z), index and count (N,) (count 0: inner node with
+        children at index, index + 1; else a leaf of prims[index, index + count)), prims the
+        original triangle index of every leaf entry."""
+        self = cls.__new__(cls)
+        rows = np.zeros((len(index), 8), np.uint32)
+        rows[:, :6] = np.ascontiguousarray(bounds, np.float32).view(np.uint32)
+        rows[:, 6] = index
+        rows[:, 7] = count
+        prims = np.ascontiguousarray(prims, np.uint32)
+        bv = F.BvhView()
+        bv.num_nodes = rows.shape[0]
+        bv.nodes = C.cast(rows.ctypes.data, C.POINTER(F.BvhNode))
+        bv.num_prims = prims.shape[0]
+        bv.prim_indices = prims.ctypes.data_as(C.POINTER(C.c_uint32))
+        self._create(scene.mesh_view(), bv)       # the oracle copies the arrays
+        return self
+
+    @classmethod
+    def with_gpu_nodes(cls, scene, nodes, prims):
+        """The oracle's copy of scene traversing a tree given in the GPU's binary layout (sol_gpu_node:
+        per node both child boxes and child words, a leaf as ~(first << 3 | count - 1) over the
+        leaf-order triangles, whose original indices are prims), rewritten breadth-first into the
+        bvh v2 layout: root box = union of the root's child boxes, an inner node's children at
+        index and index + 1."""
+        assert len(nodes) >= 1 and nodes[0].child[0] != nodes[0].child[1]
+
+        def child_boxes(g):
+            return [(g.b0[0], g.b0[1], g.b0[2], g.b0[3], g.bz[0], g.bz[1]),
+                    (g.b1[0], g.b1[1], g.b1[2], g.b1[3], g.bz[2], g.bz[3])]
+
+        c0, c1 = child_boxes(nodes[0])
+        bounds = [(min(c0[0], c1[0]), max(c0[1], c1[1]), min(c0[2], c1[2]), max(c0[3], c1[3]),
+                   min(c0[4], c1[4]), max(c0[5], c1[5]))]
+        index, count = [0], [0]
+        work = [(0, 0)]                       # (GPU node, row of the rewritten tree)
+        head = 0
+        while head < len(work):
+            gi, row = work[head]
+            head += 1
+            g = nodes[gi]
+            first = len(bounds)
+            index[row] = first
+            for c, box in enumerate(child_boxes(g)):
+                bounds.append(box)
+                ref = int(g.child[c])
+                if ref >= 0:
+                    index.append(0)
+                    count.append(0)
+                    work.append((ref, first + c))
+                else:
+                    code = (~ref) & 0xFFFFFFFF
+                    index.append(code >> 3)
+                    count.append((code & 7) + 1)
+        return cls.with_bvh(scene, np.float32(bounds), np.uint32(index), np.uint32(count), np.uint32(prims))
+
+    @classmethod
     def from_tables(cls, t):
         """t: the dict oracle.scene_files.load_scene returns (camera included)."""
         self = cls.__new__(cls)
@@ -138,6 +199,12 @@ class OracleScene:
         arr = (F.Light * len(lights))(*lights)
         assert lib.oracle_scene_set_lights(self._h, arr, len(lights)) == 0
 
+    def camera_ray(self, u, v):
+        """(origin, unit direction) of the scene camera's ray through (u, v) in [-1, 1]^2."""
+        org, d = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        assert lib.oracle_camera_ray(self._h, float(u), float(v), org.ctypes.data, d.ctypes.data) == 0
+        return org, d
+
     def intersect(self, rays, any_hit=False, brute_force=False, stats=None):
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
@@ -174,6 +241,13 @@ def pcg(seed, n):
     f = np.empty(n, np.float32)
     lib.oracle_pcg(seed, n, u.ctypes.data, f.ctypes.data)
     return u, f
+
+
+def sample_pixel(sample_index, x, y, w, h):
+    """(u, v) of sample_pixel for pixel (x, y) of a w x h image."""
+    uv = np.zeros(2, np.float32)
+    lib.oracle_sample_pixel(sample_index, x, y, w, h, uv.ctypes.data)
+    return float(uv[0]), float(uv[1])
 
 
 def ray_triangle(org, d, v0, v1, v2):

@@ -4,8 +4,7 @@
 The wavefront renderer runs with its default tuning (on a 288 GB MI355X every one of the frame's
 132,710,400 paths is in flight at once, the configuration the throughput figure is measured in)
 and must equal the megakernel bit for bit. Both are then held to the oracle rendering the same
-frame over the same BVH on the host's cores (about 11 s on 16 threads), to the pixel budget of the
-other parity tests.
+frame over the same BVH on the host's cores (about 11 s on 16 threads): every pixel identical.
 """
 import gc
 import os
