@@ -1,10 +1,14 @@
 """Differentiable rendering over solhip_backward (adjoint path replay).
 
 Parameters, per material and per light of an uploaded scene:
-  albedo    (M, 3)  diffuse albedo kd (used by diffuse materials)
-  log_ns    (M,)    natural log of the Phong exponent ns used to evaluate the BSDF (Phong materials);
-                    sampling keeps the scene's ns_sample, so sampling decisions do not move with it
+  albedo    (M, 3)  diffuse albedo kd (diffuse materials, and the diffuse lobe of the mixed
+                    diffuse-Phong material)
+  log_ns    (M,)    natural log of the Phong exponent ns used to evaluate the BSDF (Phong materials,
+                    and the Phong lobe of the mixed material); sampling keeps the scene's
+                    ns_sample, so sampling decisions do not move with it
   emission  (L, 3)  intensity of every light
+The other entries (albedo of a Phong or specular material, log_ns of a diffuse one) are not read
+by the renderer; their gradients are zero.
 
 `render(albedo, log_ns, emission)` is a torch.autograd.Function: the forward pass writes the
 tables and renders the image on the GPU (solhip_render into a device tensor); the backward pass
@@ -34,18 +38,19 @@ def base_parameters(materials, lights):
 
 
 def apply_parameters(materials, lights, albedo, log_ns, emission):
-    """Copies of the tables with kd = albedo (diffuse), ns = exp(log_ns) (Phong; ns_sample unchanged)
-    and intensity = emission."""
+    """Copies of the tables with kd = albedo (diffuse and mixed), ns = exp(log_ns) (Phong and mixed;
+    ns_sample unchanged) and intensity = emission: exactly the entries solhip_backward writes
+    gradients for."""
     albedo = np.asarray(albedo, np.float32)
     log_ns = np.asarray(log_ns, np.float32)
     emission = np.asarray(emission, np.float32)
     mats = []
     for i, m in enumerate(materials):
         m2 = copy.copy(m)
-        if m.type == F.BSDF_DIFFUSE:
+        if m.type in (F.BSDF_DIFFUSE, F.BSDF_INTERP):
             for c in range(3):
                 m2.kd[c] = float(albedo[i, c])
-        elif m.type == F.BSDF_PHONG:
+        if m.type in (F.BSDF_PHONG, F.BSDF_INTERP):
             m2.ns = float(np.float32(math.exp(float(log_ns[i]))))
         mats.append(m2)
     ls = []
@@ -96,7 +101,8 @@ class DiffRenderer:
         torch.cuda.synchronize()
         self.gscene.backward_device(d_image.data_ptr(), self.width, self.height, self.sample_index,
                                     self.sample_count, dm.data_ptr(), dl.data_ptr(), self.config)
-        # kd gradients exist for diffuse materials only, log ns for Phong only (the kernel writes no others)
+        # kd gradients exist for diffuse and mixed materials, log ns for Phong and mixed ones: the
+        # entries apply_parameters moves (the kernel writes no others)
         return dm[:, :3].contiguous(), dm[:, 3].contiguous(), dl
 
     def __call__(self, albedo, log_ns, emission):

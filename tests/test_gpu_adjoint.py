@@ -69,13 +69,13 @@ def test_adjoint_host_and_device_pointers_agree_and_accumulate(ctx):
     compare(tl.cpu().numpy() - 1.0, dl, rtol=1e-4)
 
 
-def test_adjoint_matches_central_differences(ctx):
-    scene = small_spheres()
-    cfg = sol_amd.PathTracerConfig(max_path_len=8, disable_rr=True)
-    w, h, spp = 128, 72, 8
-    dr = DiffRenderer(ctx.upload(scene), w, h, 0, spp, cfg)
+def fd_check(dr, seed, kinds_wanted, per_kind=6, min_checked=6, types=None):
+    """Adjoint gradients of an L2 loss against a target rendered with parameters perturbed by
+    N(0, 0.1^2), against central differences (h = H, common random numbers) of GPU renders through
+    DiffRenderer: the largest `per_kind` gradients of each kind, each parameter of a material or
+    light the renderer reads (of materials of the given BSDF types, if types is given)."""
     albedo, log_ns, le = dr.parameters()
-    rng = np.random.default_rng(4)
+    rng = np.random.default_rng(seed)
     t_albedo = np.clip(albedo + rng.normal(0, 0.1, albedo.shape), 0.02, 0.98).astype(np.float32)
     t_log_ns = (log_ns + rng.normal(0, 0.1, log_ns.shape)).astype(np.float32)
     t_le = np.maximum(le + rng.normal(0, 0.1, le.shape), 0).astype(np.float32)
@@ -87,21 +87,25 @@ def test_adjoint_matches_central_differences(ctx):
     img = dr.render_with(albedo, log_ns, le)
     da, dn, de = dr.gradients(2 * (img.double() - target).float(), albedo, log_ns, le)
     da, dn, de = da.cpu().numpy(), dn.cpu().numpy(), de.cpu().numpy()
-    mats = dr.materials
     cand = []
-    for i, m in enumerate(mats):
-        if m.type == F.BSDF_DIFFUSE:
+    for i, m in enumerate(dr.materials):
+        picked = types is None or m.type in types
+        if m.type in (F.BSDF_DIFFUSE, F.BSDF_INTERP) and picked:
             cand += [("albedo", (i, c), da[i, c]) for c in range(3)]
-        elif m.type == F.BSDF_PHONG:
+        if m.type in (F.BSDF_PHONG, F.BSDF_INTERP) and picked:
             cand.append(("log_ns", (i,), dn[i]))
+        if m.type not in (F.BSDF_DIFFUSE, F.BSDF_INTERP):
+            assert not da[i].any()            # parameters the renderer does not read get no gradient
+        if m.type not in (F.BSDF_PHONG, F.BSDF_INTERP):
+            assert dn[i] == 0
     cand += [("emission", (i, c), de[i, c]) for i in range(len(dr.lights)) for c in range(3)]
     kinds = {}
     for p in cand:
         kinds.setdefault(p[0], []).append(p)
-    assert set(kinds) == {"albedo", "log_ns", "emission"}
+    assert set(kinds) == set(kinds_wanted)
     checked = 0
     for kind, ps in kinds.items():
-        ps = sorted(ps, key=lambda p: -abs(p[2]))[:6]
+        ps = sorted(ps, key=lambda p: -abs(p[2]))[:per_kind]
         k = {"albedo": 0, "log_ns": 1, "emission": 2}[kind]
         fds = []
         for _, idx, adj in ps:
@@ -116,7 +120,26 @@ def test_adjoint_matches_central_differences(ctx):
             if abs(fd) >= 0.01 * scale:
                 checked += 1
                 assert abs(adj - fd) <= 0.01 * abs(fd), (kind, idx, adj, fd)
-    assert checked >= 6
+    assert checked >= min_checked
+
+
+def test_adjoint_matches_central_differences(ctx):
+    cfg = sol_amd.PathTracerConfig(max_path_len=8, disable_rr=True)
+    dr = DiffRenderer(ctx.upload(small_spheres()), 128, 72, 0, 8, cfg)
+    fd_check(dr, 4, {"albedo", "log_ns", "emission"})
+
+
+def test_adjoint_matches_central_differences_on_mixed_bsdfs(ctx):
+    # sol's glossy Cornell box: diffuse walls, Phong materials and two diffuse-Phong mixes (the
+    # floor's and the tall box's), whose albedo and Phong exponent DiffRenderer moves and
+    # solhip_backward differentiates.
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cornell", "cornell_box_glossy.toml")
+    scene = sol_amd.Scene.load(path, width=64, height=48)
+    assert any(m.type == F.BSDF_INTERP for m in scene.materials())
+    cfg = sol_amd.PathTracerConfig(max_path_len=8, disable_rr=True)
+    dr = DiffRenderer(ctx.upload(scene), 64, 48, 0, 16, cfg)
+    fd_check(dr, 9, {"albedo", "log_ns", "emission"}, per_kind=6, min_checked=6, types={F.BSDF_INTERP})
 
 
 def test_autograd_function_matches_direct_call(ctx):
