@@ -92,6 +92,9 @@ typedef struct solhip_tuning {
                                the megakernel, the adjoint replay and the intersection kernel
                                (4 needs the scene's 4-wide layout; scenes without one use the
                                binary).                                                     */
+    uint32_t chunk_items;   /* wavefront: work items (path samples) per chunk of work, the
+                               capacity of the per-sample radiance buffer (16 B an item);
+                               65536..2^28, default 192 Mi (3 GiB). Whole tiles make a chunk. */
 } solhip_tuning;
 
 /* Counters accumulated by the context since the last solhip_reset_stats. */
@@ -221,7 +224,9 @@ int solhip_render(solhip_scene* scene, const sol_pt_config* cfg, int renderer,
                   uint32_t sample_count, float* accum, int device_ptr);
 
 /* Many scenes and image tiles in one wavefront run fed by a global work queue. All scenes
-   must belong to ctx. Accumulation semantics per job as solhip_render. */
+   must belong to ctx. Accumulation semantics per job as solhip_render. Jobs may share an image
+   (or overlapping memory), e.g. two sample ranges of one frame: their sums are added in job order,
+   exactly as successive solhip_render calls would add them. */
 int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t num_jobs,
                         const sol_pt_config* cfg);
 

@@ -176,6 +176,8 @@ int solhip_set_tuning(solhip_ctx* ctx, const solhip_tuning* t) {
     if (!n.tile_h) n.tile_h = d.tile_h;
     if (!n.graph_iters) n.graph_iters = d.graph_iters;
     if (!n.bvh_width) n.bvh_width = d.bvh_width;
+    if (!n.chunk_items) n.chunk_items = d.chunk_items;
+    if (n.chunk_items < (1u << 16) || n.chunk_items > (1u << 28)) return fail(SOLHIP_E_INVALID, "chunk_items out of range");
     if (n.bvh_width != 2 && n.bvh_width != 4) return fail(SOLHIP_E_INVALID, "bvh_width must be 2 or 4");
     if (!valid_shape_values(n.block_size, n.lds_stack)) return fail(SOLHIP_E_INVALID, "unsupported block size or LDS stack");
     if (n.path_slots < 4096 || n.path_slots > (1u << 27)) return fail(SOLHIP_E_INVALID, "path_slots out of range");

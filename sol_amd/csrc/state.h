@@ -51,6 +51,9 @@ inline uint32_t default_path_slots(unsigned long long device_memory) {
     return s;
 }
 
+// Work items per chunk by default: the per-sample radiance buffer holds 192 Mi items (3 GiB).
+constexpr uint32_t kDefaultChunkItems = 192u << 20;
+
 inline solhip_tuning default_tuning(unsigned long long device_memory) {
     solhip_tuning t{};
     t.block_size = 256;
@@ -60,6 +63,7 @@ inline solhip_tuning default_tuning(unsigned long long device_memory) {
     t.tile_h = 32;
     t.graph_iters = 16;
     t.bvh_width = 4;
+    t.chunk_items = kDefaultChunkItems;
     return t;
 }
 

@@ -140,6 +140,10 @@ struct WaveTables {
     float4* const* job_sout;
     const uint32_t* job_sstride;
     const uint32_t* job_soff;
+    // Reduction layer of each job: jobs whose images overlap get increasing layers in job order,
+    // and the layers of a chunk are reduced one launch after the other, so that overlapping jobs
+    // add into the image in job order and never race.
+    const uint32_t* job_layer;
     const uint32_t* job_w;
     const uint32_t* job_h;
     const uint32_t* job_spp;

@@ -753,10 +753,12 @@ __global__ void k_set_chunk(WaveBuf b, unsigned long long begin, unsigned long l
     b.counts[kCntStartFill] = 1;
 }
 
-// Per pixel of the chunk's tiles: sum the samples in sample order, add to the image once.
+// Per pixel of the chunk's tiles of jobs in reduction layer `layer`: sum the samples in sample
+// order, add to the image once.
 __global__ __launch_bounds__(256) void k_reduce(WaveTables tabs, WaveBuf b, uint32_t first_tile,
-                                                unsigned long long chunk_begin) {
+                                                unsigned long long chunk_begin, uint32_t layer) {
     uint32_t tile = first_tile + blockIdx.x;
+    if (tabs.job_layer[tabs.tile_job[tile]] != layer) return;
     uint32_t tw = tabs.tile_w[tile], th = tabs.tile_h[tile];
     uint32_t per_sample = tw * th;
     uint32_t pix = blockIdx.y * 256 + threadIdx.x;
@@ -891,10 +893,10 @@ hipError_t wave_launch_set_chunk(const WaveLaunch& L, unsigned long long begin, 
 }
 
 hipError_t wave_launch_reduce(const WaveLaunch& L, uint32_t first_tile, uint32_t num_tiles, uint32_t max_tile_pixels,
-                              unsigned long long chunk_begin) {
+                              unsigned long long chunk_begin, uint32_t layer) {
     dim3 g(num_tiles, (max_tile_pixels + 255) / 256);
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_reduce, g, dim3(256), 0, L.stream, L.tabs, L.buf, first_tile, chunk_begin);
+    hipLaunchKernelGGL(k_reduce, g, dim3(256), 0, L.stream, L.tabs, L.buf, first_tile, chunk_begin, layer);
     return hipGetLastError();
 }
 

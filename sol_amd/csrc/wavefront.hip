@@ -22,9 +22,6 @@ namespace {
         }                                                                                        \
     } while (0)
 
-// Items (path samples) whose colours the sample buffer can hold at once: 12 bytes each.
-constexpr unsigned long long kDefaultSampleCapacity = 192ull << 20;
-
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Job {
@@ -43,7 +40,7 @@ struct WavefrontState {
     DeviceBuffer arena, counters, samples;
     unsigned long long sample_capacity = 0;
     WaveBuf buf{};
-    DeviceBuffer t_job_sout, t_job_sstride, t_job_soff;
+    DeviceBuffer t_job_sout, t_job_sstride, t_job_soff, t_job_layer;
     DeviceBuffer t_scenes, t_job_item0, t_job_sample0, t_job_scene, t_job_accum, t_job_w, t_job_h, t_job_spp,
         t_job_tw, t_job_th, t_tile_job, t_tile_x0, t_tile_y0, t_tile_w, t_tile_h, t_tile_item0;
     uint64_t generation = 0;
@@ -168,6 +165,39 @@ std::string graph_key(const WaveLaunch& L, const WavefrontState& st, int iters) 
     return k;
 }
 
+// Reduction layer of each job (planar RGB images of 3 w h floats at accum): 0 unless the job's
+// image overlaps the image of an earlier job, else one more than the largest layer of the earlier
+// jobs it overlaps. Layers are reduced in increasing order, so overlapping jobs add in job order
+// (as successive renders would) and jobs of one layer never share a pixel. Intervals are swept in
+// address order; only jobs in a run of overlapping intervals are compared pairwise.
+std::vector<uint32_t> reduction_layers(const std::vector<float*>& accum, const std::vector<uint32_t>& w,
+                                       const std::vector<uint32_t>& h, uint32_t& num_layers) {
+    const size_t n = accum.size();
+    std::vector<uint32_t> layer(n, 0);
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    auto lo = [&](size_t i) { return reinterpret_cast<uintptr_t>(accum[i]); };
+    auto hi = [&](size_t i) { return lo(i) + sizeof(float) * 3 * static_cast<uintptr_t>(w[i]) * h[i]; };
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return lo(a) < lo(b) || (lo(a) == lo(b) && a < b); });
+    num_layers = 1;
+    for (size_t r = 0; r < n;) {
+        size_t e = r + 1;
+        uintptr_t end = hi(order[r]);
+        while (e < n && lo(order[e]) < end) end = std::max(end, hi(order[e++]));
+        if (e - r > 1) {
+            std::vector<size_t> run(order.begin() + r, order.begin() + e);
+            std::sort(run.begin(), run.end());   // job order
+            for (size_t a = 0; a < run.size(); ++a)
+                for (size_t b = 0; b < a; ++b)
+                    if (lo(run[b]) < hi(run[a]) && lo(run[a]) < hi(run[b]))
+                        layer[run[a]] = std::max(layer[run[a]], layer[run[b]] + 1);
+            for (size_t a : run) num_layers = std::max(num_layers, layer[a] + 1);
+        }
+        r = e;
+    }
+    return layer;
+}
+
 // Runs the jobs to completion. Jobs index `views` through Job::scene.
 int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::vector<Job>& jobs,
                   const sol_pt_config& cfg, uint32_t max_depth, uint32_t max_depth4) {
@@ -179,7 +209,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     // Jobs in passes of samples: a pass keeps a job under 2^32 items and one tile's samples
     // under the sample-buffer capacity (far above any ordinary sample count); each pass is
     // accumulated into the image on its own.
-    const unsigned long long cap = kDefaultSampleCapacity;
+    const unsigned long long cap = tu.chunk_items;
     std::vector<unsigned long long> job_item0, job_sample0;
     std::vector<uint32_t> job_scene, job_w, job_h, job_spp, job_tw, job_th;
     std::vector<float*> job_accum;
@@ -232,13 +262,15 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     tile_item0.push_back(total);
     const uint32_t num_tiles = static_cast<uint32_t>(tile_job.size());
     const uint32_t num_jobs = static_cast<uint32_t>(job_scene.size());
+    uint32_t num_layers = 1;
+    const std::vector<uint32_t> job_layer = reduction_layers(job_accum, job_w, job_h, num_layers);
 
     hipStream_t s = ctx.stream;
     if ((rc = st.upload(st.t_scenes, views, s)) || (rc = st.upload(st.t_job_item0, job_item0, s)) ||
         (rc = st.upload(st.t_job_sample0, job_sample0, s)) || (rc = st.upload(st.t_job_scene, job_scene, s)) ||
         (rc = st.upload(st.t_job_accum, job_accum, s)) || (rc = st.upload(st.t_job_w, job_w, s)) ||
         (rc = st.upload(st.t_job_sout, job_sout, s)) || (rc = st.upload(st.t_job_sstride, job_sstride, s)) ||
-        (rc = st.upload(st.t_job_soff, job_soff, s)) ||
+        (rc = st.upload(st.t_job_soff, job_soff, s)) || (rc = st.upload(st.t_job_layer, job_layer, s)) ||
         (rc = st.upload(st.t_job_h, job_h, s)) || (rc = st.upload(st.t_job_spp, job_spp, s)) ||
         (rc = st.upload(st.t_job_tw, job_tw, s)) || (rc = st.upload(st.t_job_th, job_th, s)) ||
         (rc = st.upload(st.t_tile_job, tile_job, s)) || (rc = st.upload(st.t_tile_x0, tile_x0, s)) ||
@@ -278,6 +310,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     L.tabs.job_sout = st.t_job_sout.as<float4*>();
     L.tabs.job_sstride = st.t_job_sstride.as<uint32_t>();
     L.tabs.job_soff = st.t_job_soff.as<uint32_t>();
+    L.tabs.job_layer = st.t_job_layer.as<uint32_t>();
     L.tabs.job_w = st.t_job_w.as<uint32_t>();
     L.tabs.job_h = st.t_job_h.as<uint32_t>();
     L.tabs.job_spp = st.t_job_spp.as<uint32_t>();
@@ -375,8 +408,9 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
                 pending += counts[qcount(kQExtend0, k)] + counts[qcount(kQShadow, k)];
             if (pending == 0) break;
         }
-        HIP_OK(wave_launch_reduce(L, c.first, c.count, max_tile_pixels, c.begin));
-        launches += 1;
+        for (uint32_t layer = 0; layer < num_layers; ++layer)
+            HIP_OK(wave_launch_reduce(L, c.first, c.count, max_tile_pixels, c.begin, layer));
+        launches += num_layers;
     }
     HIP_OK(hipEventRecord(ctx.ev_end, s));
     unsigned long long stats[3];
