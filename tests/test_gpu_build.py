@@ -12,6 +12,7 @@ import pytest
 
 import sol_amd
 from sol_amd import _ffi as F
+from tests import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 
@@ -130,6 +131,50 @@ def test_tiny_and_degenerate_meshes(ctx, n):
                                       np.zeros(n, np.int32), -np.ones(n, np.int32), [mat], [light], cam)
     built = ctx.build(scene)
     check_structure(scene, built)
+
+
+def oracle_of_built(scene, built):
+    """The oracle over the GPU-built tree itself (not the host's, not its own)."""
+    d = built.download()
+    return O.OracleScene.with_gpu_nodes(scene, d["nodes"], [t.prim for t in d["tris"]])
+
+
+@pytest.mark.parametrize("name", ["cornell", "spheres", "water"])
+def test_gpu_built_scene_renders_like_the_oracle_on_its_tree(ctx, name):
+    # The oracle traverses the GPU-built tree (not the host's, not its own): renders of both GPU
+    # renderers over the GPU-built tree equal the oracle's bit for bit.
+    scene = scenes()[name]()
+    built = ctx.build(scene)
+    oracle = oracle_of_built(scene, built)
+    cfg = sol_amd.PathTracerConfig(max_path_len=8)
+    ref = sol_amd.new_image(48, 32)
+    oracle.render(ref, 2, 4, cfg)
+    assert ref.any()
+    for kind in (F.RENDERER_MEGAKERNEL, F.RENDERER_WAVEFRONT):
+        for width in (2, 4):
+            ctx.set_tuning(bvh_width=width)
+            print(f"render {name} renderer {kind} bvh width {width} vs oracle", flush=True)
+            img = sol_amd.new_image(48, 32)
+            built.render(img, 2, 4, cfg, renderer=kind)
+            np.testing.assert_array_equal(img, ref)
+    ctx.set_tuning(bvh_width=0)
+
+
+def test_gpu_built_scene_intersects_like_the_oracle_on_its_tree(ctx):
+    scene = sol_amd.Scene.triangle_soup(30000, seed=12)
+    built = ctx.build(scene)
+    oracle = oracle_of_built(scene, built)
+    rays = sol_amd.gen_rays(0, 1 << 15, seed=5)
+    shadow = sol_amd.gen_rays(1, 1 << 15, seed=6)
+    want = oracle.intersect(rays)
+    want_any = oracle.intersect(shadow, any_hit=True)
+    assert (want[0] >= 0).mean() > 0.05 and 0.05 < want_any.mean() < 0.95
+    for width in (2, 4):
+        ctx.set_tuning(bvh_width=width)
+        for x, y in zip(built.intersect(rays), want):
+            np.testing.assert_array_equal(x, y)
+        np.testing.assert_array_equal(built.intersect(shadow, any_hit=True), want_any)
+    ctx.set_tuning(bvh_width=0)
 
 
 @pytest.mark.parametrize("name", ["cornell", "spheres", "water"])

@@ -229,3 +229,27 @@ def test_invalid_arguments_are_rejected():
         sol_amd.Scene.cornell_box(aspect=-1.0)
     with pytest.raises(F.SolError):
         sol_amd.Scene.sphere_field(num_spheres=0)
+
+
+def test_oracle_over_the_gpu_node_layout_equals_the_product_bvh():
+    # The rewrite of a binary GPU-layout tree into the oracle's bvh v2 layout (used to run the
+    # oracle over GPU-built trees) checked on the host's own flattening of the same tree: the
+    # oracle then renders and intersects exactly as over the product BVH.
+    s = sol_amd.Scene.sphere_field(num_spheres=20, subdivision=2, seed=4, extent=3.0, radius_max=1.0)
+    d = s.gpu_desc()
+    nodes = [d.nodes[i] for i in range(d.num_nodes)]
+    prims = [d.tris[i].prim for i in range(d.num_tris)]
+    conv = O.OracleScene.with_gpu_nodes(s, nodes, prims)
+    prod = O.OracleScene(s, use_product_bvh=True)
+    rays = sol_amd.gen_rays(0, 4096, seed=9)
+    for a, b in zip(conv.intersect(rays), prod.intersect(rays)):
+        np.testing.assert_array_equal(a, b)
+    st_conv, st_prod = O.OracleStats(), O.OracleStats()
+    cfg = sol_amd.PathTracerConfig(max_path_len=6)
+    a, b = sol_amd.new_image(24, 16), sol_amd.new_image(24, 16)
+    conv.render(a, 0, 2, cfg, stats=st_conv)
+    prod.render(b, 0, 2, cfg, stats=st_prod)
+    assert a.any()
+    np.testing.assert_array_equal(a, b)
+    # the same tree: every ray visits the same nodes
+    assert st_conv.ext_inner_visits == st_prod.ext_inner_visits and st_conv.ext_tri_tests == st_prod.ext_tri_tests
