@@ -9,12 +9,15 @@ and arithmetic, so their images are identical (equal RMSE by construction; check
 Also reported:
   roofline      the traversal kernel k_trace (closest hits of the extension rays and any hits of the
                 shadow rays), timed with HIP events around each launch over a second timed pass
-                of the same frames; its algorithmic bytes per ray use the inner-node visits and
-                triangle tests per ray of tests/golden/traversal_counts.json (counted once on the
-                same BVH and rays by tests/golden/make_traversal_counts.py). "traffic" is the HBM
-                bytes per launch read from the committed rocprofv3 counter summary of this
-                workload, if present; "dram_gbs" / "dram_frac" are that traffic over the measured
-                launch time, against the peak. "copy_gbs" is a device-to-device copy timed here.
+                of the same frames. "achieved" is its HBM traffic per launch ("traffic", FETCH_SIZE
+                + WRITE_SIZE from the committed rocprofv3 counter summary of this workload, scaled
+                by the counters' reading of a copy of known size) over the measured launch time;
+                "frac" = achieved / 8 TB/s. "request" is the rate of the survey's algorithmic bytes
+                per ray (64 B per inner-node visit and 48 B per triangle test of the binary tree,
+                counted on the same BVH and rays in tests/golden/traversal_counts.json): a request
+                rate that caches serve mostly, so it can exceed HBM peak. "issue" names the binding
+                limit: the texture-address unit's busy fraction (memory-instruction issue) from the
+                committed counter summary. "copy_gbs" is a device-to-device copy timed here.
   cpu_baseline  the CPU oracle (C++ restatement of sol's path tracer, OpenMP) on the host's cores,
                 on a bounded sample of the same frame (all pixels; by default all 64 samples, about
                 11 s on 16 threads, and then the oracle's image is also compared with the wavefront
@@ -23,8 +26,9 @@ Also reported:
                 4,194,304 any-hit rays (device buffers), Mrays/s; the oracle on 262,144 of each.
   cornell       the Cornell box at 256x256, 16 spp, depth 8: both GPU renderers and the oracle.
   batched       64 uneven sphere-field scenes (sol_amd.workloads.batch_specs) rendered by one
-                solhip_render_batch call after solhip_autotune on the largest job, against the same
-                jobs rendered one by one with the megakernel.
+                solhip_render_batch call, with the default launch parameters and after
+                solhip_autotune, against the same jobs rendered one by one with the megakernel
+                (all three timed alike: one warm-up, then the mean of the timed steps).
   adjoint       solhip_backward on the sphere field with 256 material groups at 512x512, 32 spp
                 (Russian roulette off), with a central-difference check (h = 1e-3, common random
                 numbers) of the largest albedo, Phong log-exponent and emission gradients.
@@ -46,6 +50,7 @@ HBM_PEAK_GBS = 8000.0
 B_RAY, B_HIT, B_FLAG, B_NODE, B_TRI = 32, 16, 4, 64, 48
 COUNTS = os.path.join(ROOT, "tests", "golden", "traversal_counts.json")
 TRAFFIC = os.path.join(ROOT, "profiles", "k_trace_sphere_field_1080p.pmc.txt")
+COUNTERS = os.path.join(ROOT, "profiles", "wavefront_sphere_field_1080p.counters.md")
 COPY_BYTES = 1 << 31
 
 
@@ -66,15 +71,20 @@ def copy_bandwidth(torch):
     return 2 * COPY_BYTES / (best * 1e-3) / 1e9
 
 
-def read_traffic():
-    """HBM bytes per k_trace launch from the committed counter summary ("traffic_bytes_per_launch: N")."""
-    if not os.path.exists(TRAFFIC):
+def read_key(path, key):
+    """The value of a "key: value" line of a committed profile summary, or None."""
+    if not os.path.exists(path):
         return None
-    with open(TRAFFIC) as f:
+    with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     return None
+
+
+def read_traffic():
+    """HBM bytes per k_trace launch from the committed counter summary."""
+    return read_key(TRAFFIC, "traffic_bytes_per_launch")
 
 
 def cpu_model():
@@ -168,26 +178,31 @@ def bench_batched(sol_amd, F, torch, steps):
     ctx = sol_amd.Context(0)
     gs = [ctx.upload(sc) for sc in scenes]
     cfg = sol_amd.PathTracerConfig(max_path_len=BATCH_DEPTH)
-    big = max(range(len(specs)), key=lambda i: specs[i].width * specs[i].height * specs[i].spp)
-    t = time.perf_counter()
-    tuned = gs[big].autotune(specs[big].width, specs[big].height, budget_spp=2, config=cfg)
-    tune_s = time.perf_counter() - t
     bufs = [torch.zeros((3, sp.height, sp.width), dtype=torch.float32, device="cuda") for sp in specs]
     jobs = [(g, sp.width, sp.height, 0, sp.spp, b.data_ptr()) for sp, g, b in zip(specs, gs, bufs)]
     samples = sum(sp.width * sp.height * sp.spp for sp in specs)
     torch.cuda.synchronize()
-    ctx.render_batch(jobs, cfg)
-    torch.cuda.synchronize()
+
+    def timed(fn):
+        fn()                                   # warm-up: arenas, graph capture, code objects
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    default_s = timed(lambda: ctx.render_batch(jobs, cfg))
     t = time.perf_counter()
-    for _ in range(steps):
-        ctx.render_batch(jobs, cfg)
-    torch.cuda.synchronize()
-    batch_s = (time.perf_counter() - t) / steps
-    t = time.perf_counter()
-    for sp, g, b in zip(specs, gs, bufs):
-        g.render_device(b.data_ptr(), sp.width, sp.height, 0, sp.spp, cfg, renderer=F.RENDERER_MEGAKERNEL)
-    torch.cuda.synchronize()
-    mega_s = time.perf_counter() - t
+    tuned = ctx.autotune_batch(jobs, cfg)
+    tune_s = time.perf_counter() - t
+    batch_s = timed(lambda: ctx.render_batch(jobs, cfg))
+
+    def one_by_one():
+        for sp, g, b in zip(specs, gs, bufs):
+            g.render_device(b.data_ptr(), sp.width, sp.height, 0, sp.spp, cfg, renderer=F.RENDERER_MEGAKERNEL)
+
+    mega_s = timed(one_by_one)
     # the same batch with every BVH built on the GPU
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -207,9 +222,11 @@ def bench_batched(sol_amd, F, torch, steps):
         g.close()
     out = {"scenes": len(specs), "triangles": int(sum(sc.num_tris for sc in scenes)), "samples": samples,
            "msamples_s": round(samples / batch_s / 1e6, 2), "ms": round(batch_s * 1e3, 2),
+           "default_msamples_s": round(samples / default_s / 1e6, 2), "default_ms": round(default_s * 1e3, 2),
            "megakernel_one_by_one_msamples_s": round(samples / mega_s / 1e6, 2),
-           "tuned": {k: getattr(tuned, k) for k in ("block_size", "lds_stack", "path_slots", "tile_w", "tile_h",
-                                                   "graph_iters", "bvh_width")},
+           "speedup_vs_megakernel_one_by_one": round(mega_s / batch_s, 3),
+           "timing": f"each: one warm-up, then the mean of {steps} calls",
+           "tuned": {k: getattr(tuned, k) for k, _ in tuned._fields_},
            "build_s": round(build_s, 1), "autotune_s": round(tune_s, 1),
            "host_bvh_build_s": round(host_build_s, 2), "gpu_bvh_build_s": round(gpu_build_s, 3),
            "gpu_bvh_msamples_s": round(samples / gbatch_s / 1e6, 2)}
@@ -403,7 +420,11 @@ def main():
         f"shadow {b_shadow:.0f} B/ray) -> {achieved:.0f} GB/s")
     copy_gbs = copy_bandwidth(torch)
     traffic = read_traffic()
-    log(f"device copy: {copy_gbs:.0f} GB/s; k_trace HBM traffic per launch: {traffic}")
+    dram_gbs = traffic / (launch_ms * 1e6) if traffic and launch_ms > 0 else None
+    ta_busy = read_key(COUNTERS, "k_trace_ta_busy_frac")
+    valu_busy = read_key(COUNTERS, "k_trace_valu_busy_frac")
+    log(f"device copy: {copy_gbs:.0f} GB/s; k_trace HBM traffic per launch: {traffic} -> {dram_gbs} GB/s; "
+        f"TA busy {ta_busy}, VALU busy {valu_busy}")
 
     gpu_build = None if args.no_small else bench_gpu_build(sol_amd, F, torch, ctx, scene, cfg, args.spp)
     intersect = None if args.no_small else bench_intersect(sol_amd, torch, not args.no_cpu)
@@ -466,12 +487,18 @@ def main():
         "megakernel_identical_pixels": identical,
         "mrays_s": round(mrays, 1),
         "device": info.name.decode(errors="replace"),
-        "roofline": {"bound": "hbm", "kernel": "k_trace", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": "k_trace", "achieved": round(dram_gbs, 1) if dram_gbs else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dram_gbs / HBM_PEAK_GBS, 4) if dram_gbs else None,
                      "traffic": round(traffic) if traffic else None, "copy_gbs": round(copy_gbs, 1),
-                     "dram_gbs": round(traffic / (launch_ms * 1e6), 1) if traffic else None,
-                     "dram_frac": round(traffic / (launch_ms * 1e6) / HBM_PEAK_GBS, 4) if traffic else None,
-                     "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(launch_ms, 4),
+                     "request": {"gbs": round(achieved, 1), "bytes_per_launch": round(bytes_per_launch),
+                                 "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4),
+                                 "model": "survey bytes per ray: 32 ray + 16 / 4 result + 64 per binary inner-node "
+                                          "visit + 48 per triangle test; served mostly by LDS, L2 and MALL"},
+                     "issue": {"limit": "texture-address unit (vector memory instruction issue)",
+                               "ta_busy_frac": ta_busy, "valu_busy_frac": valu_busy,
+                               "source": os.path.relpath(COUNTERS, ROOT)},
+                     "avg_launch_ms": round(launch_ms, 4),
                      "launches": int(launches), "profiled_ms_per_step": round(prof_s * 1e3, 2),
                      "kernel_ms_per_step": {"trace": round(wst.trace_ms / args.steps, 2),
                                             "sort": round(wst.sort_ms / args.steps, 2),

@@ -246,6 +246,12 @@ int solhip_backward(solhip_scene* scene, const sol_pt_config* cfg, uint32_t widt
 int solhip_autotune(solhip_scene* scene, const sol_pt_config* cfg, uint32_t width,
                     uint32_t height, uint32_t budget_spp, solhip_tuning* out);
 
+/* The same search on a batch: every trial is solhip_render_batch of the given jobs, rendered into
+   scratch images (the jobs' own images are not written). Keeps the fastest parameters in the
+   context (also written to *out if not null). No counterpart in sol. */
+int solhip_autotune_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t num_jobs,
+                          const sol_pt_config* cfg, solhip_tuning* out);
+
 #ifdef __cplusplus
 }
 #endif

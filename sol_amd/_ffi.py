@@ -212,6 +212,7 @@ _SIGS = {
                                       _P, _P, _P, C.c_int]),
         "solhip_autotune": (C.c_int, [_P, C.POINTER(PtConfig), C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.POINTER(Tuning)]),
+        "solhip_autotune_batch": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(PtConfig), C.POINTER(Tuning)]),
     },
     host: {
         "sol_host_last_error": (C.c_char_p, []),

@@ -477,45 +477,29 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
     return ctx->check_overflow();
 }
 
-// Launch-parameter search for the wavefront renderer on one scene and image size: coordinate
-// descent from the current parameters over paths in flight, traversal workgroup size, LDS stack
-// depth, bounce iterations per graph, BVH node width and tile shape. Every candidate renders budget_spp samples
-// per pixel twice into a scratch image (the first render builds the slot arena and captures the
-// graph) and the second is timed with HIP events. A candidate replaces the current best if it is
-// at least 1 % faster. The statistics and the profiling switch of the context are restored.
-int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, uint32_t height, uint32_t budget_spp,
-                    solhip_tuning* out) {
-    if (!s || !cfg) return set_error(SOLHIP_E_INVALID, "null argument");
-    if (width == 0 || height == 0) return set_error(SOLHIP_E_INVALID, "empty image");
-    if (cfg->min_survival_prob < 0 || cfg->max_survival_prob > 1 || cfg->min_survival_prob > cfg->max_survival_prob)
-        return set_error(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
-    if (budget_spp == 0) budget_spp = 1;
-    Context& ctx = *s->ctx;
-    HIP_OK(hipSetDevice(ctx.device));
-    const size_t plane = static_cast<size_t>(width) * height;
-    DeviceBuffer img;
-    HIP_OK(img.alloc(sizeof(float) * 3 * plane));
+} // extern "C"
+
+namespace {
+
+// Coordinate descent over the wavefront launch parameters from the context's current ones: paths in
+// flight (candidates above `items`, the work of one trial, only add idle slots and are left out),
+// traversal workgroup size, LDS stack depth, bounce iterations per graph, BVH node width and tile
+// shape. `trial` renders with ctx.tuning and returns its duration in ms (the first call of a
+// candidate builds the slot arena and captures the graph, so each candidate runs twice and the
+// second run counts). A candidate replaces the current best if it is at least 1 % faster. The
+// statistics and the profiling switch of the context are restored; ctx.tuning ends as the best.
+template <typename Trial>
+int tune(Context& ctx, unsigned long long items, Trial trial, solhip_tuning* out) {
     const solhip_stats saved_stats = ctx.stats;
     const bool saved_profiling = ctx.profiling;
     ctx.profiling = false;
     const solhip_tuning start = ctx.tuning;
-
-    auto trial = [&](const solhip_tuning& t, float& ms) -> int {
+    auto timed = [&](const solhip_tuning& t, float& ms) -> int {
         ctx.tuning = t;
-        int rc = render_wavefront(*s, *cfg, width, height, 0, budget_spp, img.as<float>());
+        int rc = trial(ms);
         if (rc) return rc;
-        HIP_OK(hipEventRecord(ctx.ev_begin, ctx.stream));
-        rc = render_wavefront(*s, *cfg, width, height, 0, budget_spp, img.as<float>());
-        if (rc) return rc;
-        HIP_OK(hipEventRecord(ctx.ev_end, ctx.stream));
-        HIP_OK(hipEventSynchronize(ctx.ev_end));
-        HIP_OK(hipEventElapsedTime(&ms, ctx.ev_begin, ctx.ev_end));
-        return SOLHIP_OK;
+        return trial(ms);
     };
-
-    // Paths in flight above the work of one render only add idle slots; none above the default's
-    // memory bound.
-    const unsigned long long items = static_cast<unsigned long long>(plane) * budget_spp;
     const uint32_t max_slots = default_path_slots(ctx.info.total_memory);
     std::vector<uint32_t> slot_cands;
     for (uint32_t c : {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27}) {
@@ -525,7 +509,7 @@ int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
     }
     solhip_tuning best = start;
     float best_ms = 0.0f;
-    int rc = trial(best, best_ms);
+    int rc = timed(best, best_ms);
     for (int dim = 0; rc == SOLHIP_OK && dim < 6; ++dim) {
         std::vector<solhip_tuning> cands;
         solhip_tuning t = best;
@@ -545,7 +529,7 @@ int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
         for (const solhip_tuning& c : cands) {
             if (std::memcmp(&c, &best, sizeof(c)) == 0) continue;
             float ms = 0.0f;
-            if ((rc = trial(c, ms)) != SOLHIP_OK) break;
+            if ((rc = timed(c, ms)) != SOLHIP_OK) break;
             if (ms < 0.99f * best_ms) {
                 best = c;
                 best_ms = ms;
@@ -558,6 +542,76 @@ int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
     if (rc) return rc;
     if (out) *out = best;
     return SOLHIP_OK;
+}
+
+// Duration in ms of fn(), measured with the context's HIP events on its stream.
+template <typename Fn>
+int event_time(Context& ctx, Fn fn, float& ms) {
+    HIP_OK(hipEventRecord(ctx.ev_begin, ctx.stream));
+    int rc = fn();
+    if (rc) return rc;
+    HIP_OK(hipEventRecord(ctx.ev_end, ctx.stream));
+    HIP_OK(hipEventSynchronize(ctx.ev_end));
+    HIP_OK(hipEventElapsedTime(&ms, ctx.ev_begin, ctx.ev_end));
+    return SOLHIP_OK;
+}
+
+bool valid_survival(const sol_pt_config& c) {
+    return c.min_survival_prob >= 0 && c.max_survival_prob <= 1 && c.min_survival_prob <= c.max_survival_prob;
+}
+
+} // namespace
+
+extern "C" {
+
+// Launch-parameter search on one scene and image size: every trial renders budget_spp samples per
+// pixel of the image into a scratch image.
+int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, uint32_t height, uint32_t budget_spp,
+                    solhip_tuning* out) {
+    if (!s || !cfg) return set_error(SOLHIP_E_INVALID, "null argument");
+    if (width == 0 || height == 0) return set_error(SOLHIP_E_INVALID, "empty image");
+    if (!valid_survival(*cfg)) return set_error(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    if (budget_spp == 0) budget_spp = 1;
+    Context& ctx = *s->ctx;
+    HIP_OK(hipSetDevice(ctx.device));
+    const size_t plane = static_cast<size_t>(width) * height;
+    DeviceBuffer img;
+    HIP_OK(img.alloc(sizeof(float) * 3 * plane));
+    auto trial = [&](float& ms) {
+        return event_time(ctx, [&] { return render_wavefront(*s, *cfg, width, height, 0, budget_spp, img.as<float>()); },
+                          ms);
+    };
+    return tune(ctx, static_cast<unsigned long long>(plane) * budget_spp, trial, out);
+}
+
+// Launch-parameter search on a batch: every trial is the whole solhip_render_batch of the jobs, with
+// their images replaced by scratch images (the jobs' own images are not touched).
+int solhip_autotune_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t num_jobs, const sol_pt_config* cfg,
+                          solhip_tuning* out) {
+    if (!ctx || !cfg || !jobs || num_jobs == 0) return set_error(SOLHIP_E_INVALID, "null argument or no job");
+    if (!valid_survival(*cfg)) return set_error(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    size_t floats = 0;
+    unsigned long long items = 0;
+    for (uint32_t i = 0; i < num_jobs; ++i) {
+        const solhip_batch_job& j = jobs[i];
+        if (!j.scene || j.width == 0 || j.height == 0) return set_error(SOLHIP_E_INVALID, "invalid batch job");
+        if (j.scene->ctx != ctx) return set_error(SOLHIP_E_CONTEXT, "scene belongs to another context");
+        floats += 3 * static_cast<size_t>(j.width) * j.height;
+        items += static_cast<unsigned long long>(j.width) * j.height * j.sample_count;
+    }
+    HIP_OK(hipSetDevice(ctx->device));
+    DeviceBuffer img;
+    HIP_OK(img.alloc(sizeof(float) * floats));
+    std::vector<solhip_batch_job> local(jobs, jobs + num_jobs);
+    float* p = img.as<float>();
+    for (solhip_batch_job& j : local) {
+        j.accum = p;
+        p += 3 * static_cast<size_t>(j.width) * j.height;
+    }
+    auto trial = [&](float& ms) {
+        return event_time(*ctx, [&] { return solhip_render_batch(ctx, local.data(), num_jobs, cfg); }, ms);
+    };
+    return tune(*ctx, items, trial, out);
 }
 
 } // extern "C"

@@ -118,16 +118,30 @@ class Context:
         return GpuScene(self, scene, gpu_build=F.GpuBuildConfig(max_leaf_size, traversal_cost, intersection_cost,
                                                              search_radius))
 
-    def render_batch(self, jobs, config=None):
-        """jobs: list of (GpuScene, width, height, sample_index, sample_count, device_ptr)."""
+    @staticmethod
+    def _jobs(jobs):
         arr = (F.BatchJob * len(jobs))()
         for i, (gs, w, h, si, sc, ptr) in enumerate(jobs):
             arr[i].scene = gs.handle.value
             arr[i].width, arr[i].height = w, h
             arr[i].sample_index, arr[i].sample_count = si, sc
             arr[i].accum = ptr
+        return arr
+
+    def render_batch(self, jobs, config=None):
+        """jobs: list of (GpuScene, width, height, sample_index, sample_count, device_ptr)."""
         cfg = (config or PathTracerConfig()).to_c()
-        F.check_hip(F.hip.solhip_render_batch(self._h, arr, len(jobs), C.byref(cfg)), "solhip_render_batch")
+        F.check_hip(F.hip.solhip_render_batch(self._h, self._jobs(jobs), len(jobs), C.byref(cfg)),
+                    "solhip_render_batch")
+
+    def autotune_batch(self, jobs, config=None):
+        """solhip_autotune_batch over the jobs of render_batch; returns the chosen Tuning (kept in
+        the context)."""
+        cfg = (config or PathTracerConfig()).to_c()
+        t = F.Tuning()
+        F.check_hip(F.hip.solhip_autotune_batch(self._h, self._jobs(jobs), len(jobs), C.byref(cfg), C.byref(t)),
+                    "solhip_autotune_batch")
+        return t
 
 
 class GpuScene:
