@@ -223,6 +223,19 @@ int solhip_render(solhip_scene* scene, const sol_pt_config* cfg, int renderer,
                   uint32_t width, uint32_t height, uint64_t sample_index,
                   uint32_t sample_count, float* accum, int device_ptr);
 
+/* solhip_render with the megakernel into a DEVICE image (accum: 3 * width * height floats, planar
+   RGB), allocating nothing: no counterpart in sol; for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Its
+   scratch - counters, the stack-overflow flag and the traversal stack's spill area (64 entries per
+   lane beyond the LDS stack, for up to 196,608 lanes) - is one static array of 48 MiB + 256 B per
+   GPU, held until the process ends. Images equal solhip_render's bit for bit. Scenes whose BVH is
+   deeper than that stack (binary depth > lds_stack + 65, or 4-wide depth with 3 (depth - 1) >
+   lds_stack + 64) are refused with SOLHIP_E_TOO_DEEP. Calls that overlap in time on one GPU share
+   the array and race: serialise them, as every call on one context already is. */
+int solhip_render_noalloc(solhip_scene* scene, const sol_pt_config* cfg, uint32_t width,
+                          uint32_t height, uint64_t sample_index, uint32_t sample_count,
+                          float* accum);
+
 /* Many scenes and image tiles in one wavefront run fed by a global work queue. All scenes
    must belong to ctx. Accumulation semantics per job as solhip_render. Jobs may share an image
    (or overlapping memory), e.g. two sample ranges of one frame: their sums are added in job order,

@@ -213,6 +213,8 @@ _SIGS = {
         "solhip_autotune": (C.c_int, [_P, C.POINTER(PtConfig), C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.POINTER(Tuning)]),
         "solhip_autotune_batch": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(PtConfig), C.POINTER(Tuning)]),
+        "solhip_render_noalloc": (C.c_int, [_P, C.POINTER(PtConfig), C.c_uint32, C.c_uint32, C.c_uint64,
+                                            C.c_uint32, _P]),
     },
     host: {
         "sol_host_last_error": (C.c_char_p, []),

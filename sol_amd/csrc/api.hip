@@ -109,6 +109,7 @@ int solhip_create(int device, solhip_ctx** out) {
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(SOLHIP_E_NO_DEVICE, "no HIP device");
     if (device < 0 || device >= count) return fail(SOLHIP_E_NO_DEVICE, "device index out of range");
     HIP_TRY(hipSetDevice(device));
+    if (int st = noalloc_setup(device)) return st;
     auto ctx = std::make_unique<solhip_ctx>();
     ctx->device = device;
     hipDeviceProp_t prop;

@@ -90,6 +90,8 @@ struct Context {
 };
 
 int set_error(int code, const std::string& msg);
+// Looks up the device address of solhip_render_noalloc's scratch array on `device` (current).
+int noalloc_setup(int device);
 
 } // namespace solhip
 

@@ -235,6 +235,12 @@ class GpuScene:
         F.check_hip(F.hip.solhip_render(self._h, C.byref(cfg), renderer, image.shape[2], image.shape[1],
                                         sample_index, sample_count, image.ctypes.data, 0), "solhip_render")
 
+    def render_noalloc(self, ptr, width, height, sample_index, sample_count, config=None):
+        """solhip_render_noalloc: the megakernel into the device image at ptr, allocating nothing."""
+        cfg = (config or PathTracerConfig()).to_c()
+        F.check_hip(F.hip.solhip_render_noalloc(self._h, C.byref(cfg), width, height, sample_index, sample_count,
+                                                ptr), "solhip_render_noalloc")
+
     def render_device(self, ptr, width, height, sample_index, sample_count, config=None,
                       renderer=F.RENDERER_WAVEFRONT):
         cfg = (config or PathTracerConfig()).to_c()
