@@ -38,6 +38,24 @@ enum {
 
 typedef struct sol_scene sol_scene;
 
+/* A closest hit as sol::Geometry::intersect_closest returns it (sol::Hit, reference
+   include/sol/geometry.h:14-31), with the BSDF and emitter as indices into the scene's material
+   and light tables. */
+typedef struct sol_surface_hit {
+    int32_t hit;             /* 1 if the ray hit a triangle within (tmin, tmax)              */
+    int32_t front;           /* SurfaceInfo::is_front_side                                    */
+    int32_t material;        /* Hit::bsdf as a material index, -1 = none                      */
+    int32_t light;           /* Hit::light as a light index, -1 = none                        */
+    float t;                 /* the ray's tmax after the query: the hit distance on a hit     */
+    float point[3];          /* SurfaceInfo::point                                            */
+    float face_normal[3];    /* SurfaceInfo::face_normal, towards the ray (not normalised)    */
+    float tangent[3];        /* SurfaceInfo::local, column 0                                  */
+    float bitangent[3];      /* column 1                                                      */
+    float normal[3];         /* column 2: the shading normal, towards the ray                 */
+    float surf_coords[2];    /* SurfaceInfo::surf_coords: the triangle test's (u, v)          */
+    float tex_coords[2];     /* SurfaceInfo::tex_coords                                       */
+} sol_surface_hit;
+
 /* Options of the CPU BVH builder: sweep SAH (bvh v2 SweepSahBuilder semantics) followed by
    the reinsertion optimiser (bvh v2 SequentialReinsertionOptimizer semantics). */
 typedef struct sol_bvh_config {
@@ -149,6 +167,18 @@ int sol_scene_load_obj(const char* file_name, const sol_camera_params* camera, c
    meaning). */
 int sol_scene_from_arrays(const sol_mesh_view* mesh, const sol_bvh_config* cfg,
                           sol_scene** out);
+
+/* sol::Geometry of the scene's root (reference include/sol/geometry.h:33-43, answered by
+   TriangleMesh, src/triangle_mesh.cpp:64-117): attaches the root mesh to the GPU context
+   (TriangleMesh::attach_gpu), then answers queries through the C++ interface - one virtual
+   call per ray (batched == 0: intersect_closest(Rayf&) / intersect_any(const Rayf&)) or one
+   batched call for all rays (batched != 0). Host arrays; the context must outlive the scene's
+   attachment (sol_scene_destroy or another attach ends it). */
+int sol_scene_attach_gpu(sol_scene* scene, solhip_ctx* ctx);
+int sol_scene_intersect_closest(const sol_scene* scene, const sol_ray* rays, uint64_t n, int batched,
+                                sol_surface_hit* out);
+int sol_scene_intersect_any(const sol_scene* scene, const sol_ray* rays, uint64_t n, int batched,
+                            int32_t* out);
 
 void sol_scene_destroy(sol_scene* scene);
 

@@ -41,7 +41,8 @@ extern "C" {
 enum {
     SOLHIP_OK          = 0,
     SOLHIP_E_INVALID   = -1,  /* invalid argument (null, zero size, bad enum)          */
-    SOLHIP_E_NO_LIGHTS = -2,  /* scene has no light                                    */
+    SOLHIP_E_NO_LIGHTS = -2,  /* rendering a scene without light (one uploaded for
+                                 intersection queries only)                            */
     SOLHIP_E_HIP       = -3,  /* a HIP runtime call failed                             */
     SOLHIP_E_NO_DEVICE = -4,  /* no HIP device / device index out of range             */
     SOLHIP_E_TOO_DEEP  = -5,  /* BVH deeper than the traversal stack                   */
@@ -188,6 +189,9 @@ int solhip_set_profiling(solhip_ctx* ctx, int profiling);
 int solhip_get_stats(solhip_ctx* ctx, solhip_stats* out);
 int solhip_reset_stats(solhip_ctx* ctx);
 
+/* Copies a flat scene to the context's device. A scene without lights (num_lights 0) serves
+   intersection queries only (sol::Geometry through solhip_intersect); rendering it returns
+   SOLHIP_E_NO_LIGHTS, since sol's path tracer divides by the light count (path_tracer.cpp:27,36). */
 int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* desc, solhip_scene** out);
 void solhip_scene_destroy(solhip_scene* scene);
 

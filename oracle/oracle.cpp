@@ -1025,6 +1025,39 @@ int oracle_bsdf_sample(const sol_material* m, const float* n3, const float* out3
 
 float oracle_fresnel(float eta, float cos_i, float cos_t) { return fresnel_factor(eta, cos_i, cos_t); }
 
+int oracle_intersect_surface(const oracle_scene* s, const sol_ray* rays, uint64_t n, sol_surface_hit* out) {
+    if (!s || (n && (!rays || !out))) return -1;
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (long long i = 0; i < static_cast<long long>(n); ++i) {
+        const sol_ray& r = rays[i];
+        Ray ray{vec(r.org), vec(r.dir), r.tmin, r.tmax};
+        Hit hit;
+        Counters c;
+        sol_surface_hit& o = out[i];
+        o = sol_surface_hit{};
+        o.material = -1;
+        o.light = -1;
+        if (intersect_closest(*s, ray, hit, c)) {
+            const SurfaceInfo& si = hit.surf_info;
+            o.hit = 1;
+            o.front = si.is_front_side ? 1 : 0;
+            o.material = hit.bsdf;
+            o.light = hit.light;
+            const Vec3 cols[5] = {si.point, si.face_normal, si.local.c0, si.local.c1, si.local.c2};
+            float* dst[5] = {o.point, o.face_normal, o.tangent, o.bitangent, o.normal};
+            for (int k = 0; k < 5; ++k) {
+                dst[k][0] = cols[k].x;
+                dst[k][1] = cols[k].y;
+                dst[k][2] = cols[k].z;
+            }
+            o.surf_coords[0] = si.surf_u;
+            o.surf_coords[1] = si.surf_v;
+        }
+        o.t = ray.tmax;
+    }
+    return 0;
+}
+
 void oracle_sample_pixel(uint64_t sample_index, uint32_t x, uint32_t y, uint32_t w, uint32_t h, float* uv) {
     Pcg sampler(pixel_seed(sample_index, x, y));
     Uv p = sample_pixel(sampler, x, y, w, h);

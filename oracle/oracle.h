@@ -81,6 +81,10 @@ int oracle_ray_triangle(const float* org3, const float* dir3, const float* v0, c
 int oracle_bsdf_sample(const sol_material* m, const float* n3, const float* out3, int front, const float* randoms,
                        float* in3, float* pdf, float* cos, float* rgb);
 float oracle_fresnel(float eta, float cos_i, float cos_t);
+/* TriangleMesh::intersect_closest with its hit decoding (src/triangle_mesh.cpp:64-105): the
+   surface record of each ray's closest hit (tex_coords left 0: the oracle has none), t = the ray's
+   tmax after the query. */
+int oracle_intersect_surface(const oracle_scene* s, const sol_ray* rays, uint64_t n, sol_surface_hit* out);
 /* Renderer::sample_pixel (include/sol/renderer.h:50-54) of pixel (x, y) of a w x h image for the
    absolute sample index: the first two draws of the pixel's stream, as oracle_render makes them.
    uv[0] = u (left -1, right +1), uv[1] = v (top +1, bottom -1). */

@@ -146,6 +146,13 @@ class SceneInfo(C.Structure):
                                            "bvh_depth", "bvh4_depth", "top_nodes", "top_nodes4")]
 
 
+class SurfaceHit(C.Structure):
+    _fields_ = [("hit", C.c_int32), ("front", C.c_int32), ("material", C.c_int32), ("light", C.c_int32),
+                ("t", C.c_float), ("point", C.c_float * 3), ("face_normal", C.c_float * 3), ("tangent", C.c_float * 3),
+                ("bitangent", C.c_float * 3), ("normal", C.c_float * 3), ("surf_coords", C.c_float * 2),
+                ("tex_coords", C.c_float * 2)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("lds_stack", C.c_uint32), ("path_slots", C.c_uint32),
                 ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("graph_iters", C.c_uint32), ("bvh_width", C.c_uint32),
@@ -238,6 +245,9 @@ _SIGS = {
         "sol_scene_set_lights": (C.c_int, [_P, C.POINTER(Light), C.c_uint32]),
         "sol_camera_from_params": (None, [C.POINTER(CameraParams), C.POINTER(Camera)]),
         "sol_gen_rays": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, C.c_float, _P]),
+        "sol_scene_attach_gpu": (C.c_int, [_P, _P]),
+        "sol_scene_intersect_closest": (C.c_int, [_P, _P, C.c_uint64, C.c_int, _P]),
+        "sol_scene_intersect_any": (C.c_int, [_P, _P, C.c_uint64, C.c_int, _P]),
     },
 }
 

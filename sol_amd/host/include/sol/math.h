@@ -77,6 +77,46 @@ struct BBox {
 
 inline BBox merge(BBox a, const BBox& b) { return a.extend(b); }
 
+inline Vec2f operator+(const Vec2f& a, const Vec2f& b) { return Vec2f(a.x + b.x, a.y + b.y); }
+inline Vec2f operator*(const Vec2f& a, float s) { return Vec2f(a.x * s, a.y * s); }
+
+// Barycentric interpolation a (1 - u - v) + b u + c v: u weighs the second vertex, v the third
+// (the convention of the triangle test's (u, v) and of sol's hit decoding).
+inline Vec3f lerp(const Vec3f& a, const Vec3f& b, const Vec3f& c, float u, float v) {
+    return a * ((1.0f - u) - v) + b * u + c * v;
+}
+inline Vec2f lerp(const Vec2f& a, const Vec2f& b, const Vec2f& c, float u, float v) {
+    return a * ((1.0f - u) - v) + b * u + c * v;
+}
+
+// A ray and its valid interval (proto::Rayf): points org + t dir for tmin < t < tmax; tmax
+// defaults to the largest float. intersect_closest shrinks tmax to the hit's distance.
+struct Rayf {
+    Vec3f org, dir;
+    float tmin = 0.0f;
+    float tmax = std::numeric_limits<float>::max();
+    Rayf() = default;
+    Rayf(const Vec3f& org, const Vec3f& dir, float tmin = 0.0f, float tmax = std::numeric_limits<float>::max())
+        : org(org), dir(dir), tmin(tmin), tmax(tmax) {}
+    Vec3f point_at(float t) const { return org + dir * t; }
+};
+
+// 3x3 matrix by columns; a local frame has the tangent, bitangent and normal as its columns.
+struct Mat3x3f {
+    Vec3f c0, c1, c2;
+    const Vec3f& col(int i) const { return i == 0 ? c0 : (i == 1 ? c1 : c2); }
+    Vec3f operator*(const Vec3f& v) const { return c0 * v.x + c1 * v.y + c2 * v.z; }
+};
+
+// Orthonormal basis around the unit vector n, right-handed, n in the third column (Duff et al.,
+// "Building an Orthonormal Basis, Revisited", JCGT 2017).
+inline Mat3x3f ortho_basis(const Vec3f& n) {
+    const float sign = std::copysign(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    return Mat3x3f{Vec3f(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x), Vec3f(b, sign + n.y * n.y * a, -n.y), n};
+}
+
 struct Triangle {
     Vec3f v0, v1, v2;
     BBox bbox() const { BBox b = BBox::empty(); b.extend(v0); b.extend(v1); b.extend(v2); return b; }

@@ -61,6 +61,12 @@ struct FlatScene {
 // light, a BSDF or light outside the scene's tables).
 FlatScene flatten_scene(const Scene& scene, uint32_t top_levels = 6);
 
+class TriangleMesh;
+// The mesh part of flatten_scene: raw arrays, the bvh v2 layout and the GPU layouts (binary and
+// 4-wide nodes, leaf-order triangles and shading records). f.tri_material and f.tri_light must
+// already hold one entry per triangle (-1 for none), and f.materials the table they index.
+void flatten_geometry(const TriangleMesh& mesh, FlatScene& f, uint32_t top_levels = 6);
+
 } // namespace sol
 
 #endif

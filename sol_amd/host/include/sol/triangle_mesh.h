@@ -1,6 +1,10 @@
 // Triangle mesh with its BVH, as sol::TriangleMesh (reference include/sol/triangle_mesh.h:15-60,
 // src/triangle_mesh.cpp:44-60): the constructor takes the same arrays and builds the BVH on
-// the host. Flattening for the GPU happens in flatten_scene (scene.h).
+// the host. Flattening for the GPU happens in flatten_scene (scene.h). The Geometry queries
+// (src/triangle_mesh.cpp:64-117) run on the GPU: attach_gpu uploads the mesh's BVH and triangles
+// to a context, and every query is a solhip_intersect launch whose result is decoded on the host
+// as sol decodes its hits. A query on a mesh that is not attached throws std::logic_error: there
+// is no CPU traversal behind this interface.
 #ifndef SOL_TRIANGLE_MESH_H
 #define SOL_TRIANGLE_MESH_H
 
@@ -13,6 +17,7 @@
 #include "sol/geometry.h"
 #include "sol/math.h"
 #include "sol/bvh.h"
+#include "solhip.h"
 
 namespace sol {
 
@@ -31,6 +36,30 @@ public:
         std::vector<const Bsdf*>&& bsdfs,
         std::unordered_map<size_t, const Light*>&& lights,
         const bvh::BuildConfig& config = {});
+    TriangleMesh(const TriangleMesh&) = delete;
+    TriangleMesh& operator=(const TriangleMesh&) = delete;
+    ~TriangleMesh() override;
+
+    /// Uploads the mesh's BVH and triangles to the GPU context for intersection queries (no
+    /// materials or lights), replacing a previous attachment. The context must outlive the
+    /// attachment; queries on one context are serialised by the caller. Throws
+    /// std::runtime_error if the GPU library fails.
+    void attach_gpu(solhip_ctx* ctx);
+    void detach_gpu();
+    solhip_scene* gpu_geometry() const { return gpu_; }
+
+    /// Geometry queries (triangle_mesh.cpp:64-117), one ray per call.
+    std::optional<Hit> intersect_closest(Rayf& ray) const override;
+    bool intersect_any(const Rayf& ray) const override;
+    /// The same queries for n rays in one GPU launch: hits[i] (and rays[i].tmax on a hit) for
+    /// rays[i]; occluded[i] for rays[i].
+    void intersect_closest(Rayf* rays, size_t n, std::optional<Hit>* hits) const;
+    void intersect_any(const Rayf* rays, size_t n, bool* occluded) const;
+    /// The hit record of triangle t (original index) at barycentrics (u, v) for a ray whose tmax
+    /// is the hit distance, as triangle_mesh.cpp:78-104 builds it: shading normal and texture
+    /// coordinates interpolated, both normals turned towards the ray, the frame of the shading
+    /// normal, the triangle's emitter and BSDF.
+    Hit decode_hit(const Rayf& ray, size_t t, float u, float v) const;
 
     size_t triangle_count() const { return indices_.size() / 3; }
     std::tuple<size_t, size_t, size_t> triangle_indices(size_t t) const {
@@ -55,6 +84,7 @@ private:
     std::unordered_map<size_t, const Light*> lights_;
     bvh::Bvh bvh_;
     BuildTimes times_;
+    solhip_scene* gpu_ = nullptr;   // the attached GPU copy, owned
 };
 
 } // namespace sol

@@ -3,8 +3,11 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <new>
+#include <optional>
 #include <string>
+#include <vector>
 
 #include "sol_host.h"
 #include "sol/image.h"
@@ -292,6 +295,99 @@ int sol_scene_from_arrays(const sol_mesh_view* mv, const sol_bvh_config* cfg, so
 }
 
 void sol_scene_destroy(sol_scene* scene) { delete scene; }
+
+int sol_scene_attach_gpu(sol_scene* s, solhip_ctx* ctx) {
+    if (!s || !ctx) return fail(SOL_E_INVALID, "null argument");
+    auto* mesh = dynamic_cast<sol::TriangleMesh*>(s->scene.root.get());
+    if (!mesh) return fail(SOL_E_INVALID, "scene root is not a TriangleMesh");
+    try {
+        mesh->attach_gpu(ctx);
+    } catch (const std::exception& e) {
+        return fail(SOL_E_INTERNAL, e.what());
+    }
+    return SOL_OK;
+}
+
+namespace {
+
+sol::Rayf to_rayf(const sol_ray& r) {
+    return sol::Rayf(sol::Vec3f(r.org[0], r.org[1], r.org[2]), sol::Vec3f(r.dir[0], r.dir[1], r.dir[2]), r.tmin, r.tmax);
+}
+
+void put3(float* d, const sol::Vec3f& v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
+
+// The hit record with the BSDF and light as indices of the scene's tables (Scene::bsdfs and
+// Scene::lights order, the order of the flat material and light tables).
+sol_surface_hit to_surface_hit(const sol::Scene& scene, const sol::Rayf& ray, const std::optional<sol::Hit>& hit) {
+    sol_surface_hit o{};
+    o.material = -1;
+    o.light = -1;
+    o.t = ray.tmax;
+    if (!hit) return o;
+    const sol::SurfaceInfo& si = hit->surf_info;
+    o.hit = 1;
+    o.front = si.is_front_side ? 1 : 0;
+    for (size_t i = 0; i < scene.bsdfs.size(); ++i)
+        if (scene.bsdfs[i].get() == hit->bsdf) o.material = static_cast<int32_t>(i);
+    for (size_t i = 0; i < scene.lights.size(); ++i)
+        if (scene.lights[i].get() == hit->light) o.light = static_cast<int32_t>(i);
+    put3(o.point, si.point);
+    put3(o.face_normal, si.face_normal);
+    put3(o.tangent, si.local.col(0));
+    put3(o.bitangent, si.local.col(1));
+    put3(o.normal, si.local.col(2));
+    o.surf_coords[0] = si.surf_coords.x;
+    o.surf_coords[1] = si.surf_coords.y;
+    o.tex_coords[0] = si.tex_coords.x;
+    o.tex_coords[1] = si.tex_coords.y;
+    return o;
+}
+
+} // namespace
+
+int sol_scene_intersect_closest(const sol_scene* s, const sol_ray* rays, uint64_t n, int batched, sol_surface_hit* out) {
+    if (!s || (n && (!rays || !out))) return fail(SOL_E_INVALID, "null argument");
+    const sol::Geometry* geom = s->scene.root.get();
+    if (!geom) return fail(SOL_E_INVALID, "scene without root");
+    try {
+        std::vector<sol::Rayf> rs(n);
+        for (uint64_t i = 0; i < n; ++i) rs[i] = to_rayf(rays[i]);
+        std::vector<std::optional<sol::Hit>> hits(n);
+        if (batched) {
+            const auto* mesh = dynamic_cast<const sol::TriangleMesh*>(geom);
+            if (!mesh) return fail(SOL_E_INVALID, "scene root is not a TriangleMesh");
+            mesh->intersect_closest(rs.data(), n, hits.data());
+        } else {
+            for (uint64_t i = 0; i < n; ++i) hits[i] = geom->intersect_closest(rs[i]);   // virtual, one ray
+        }
+        for (uint64_t i = 0; i < n; ++i) out[i] = to_surface_hit(s->scene, rs[i], hits[i]);
+    } catch (const std::exception& e) {
+        return fail(SOL_E_INTERNAL, e.what());
+    }
+    return SOL_OK;
+}
+
+int sol_scene_intersect_any(const sol_scene* s, const sol_ray* rays, uint64_t n, int batched, int32_t* out) {
+    if (!s || (n && (!rays || !out))) return fail(SOL_E_INVALID, "null argument");
+    const sol::Geometry* geom = s->scene.root.get();
+    if (!geom) return fail(SOL_E_INVALID, "scene without root");
+    try {
+        std::vector<sol::Rayf> rs(n);
+        for (uint64_t i = 0; i < n; ++i) rs[i] = to_rayf(rays[i]);
+        if (batched) {
+            const auto* mesh = dynamic_cast<const sol::TriangleMesh*>(geom);
+            if (!mesh) return fail(SOL_E_INVALID, "scene root is not a TriangleMesh");
+            std::unique_ptr<bool[]> occ(new bool[n ? n : 1]);
+            mesh->intersect_any(rs.data(), n, occ.get());
+            for (uint64_t i = 0; i < n; ++i) out[i] = occ[i] ? 1 : 0;
+        } else {
+            for (uint64_t i = 0; i < n; ++i) out[i] = geom->intersect_any(rs[i]) ? 1 : 0;   // virtual, one ray
+        }
+    } catch (const std::exception& e) {
+        return fail(SOL_E_INTERNAL, e.what());
+    }
+    return SOL_OK;
+}
 
 int sol_scene_mesh(const sol_scene* s, sol_mesh_view* out) {
     if (!s || !out) return fail(SOL_E_INVALID, "null argument");

@@ -1,4 +1,5 @@
-// Image files: OpenEXR writer and format dispatch (reference src/image.cpp:26-37,
+// Image storage (planar channels: construction, clear, scale; reference src/image.cpp:13-23) and
+// image files: PFM, and the OpenEXR writer and format dispatch (reference src/image.cpp:26-37,
 // src/formats/exr.cpp:48-100). The EXR file follows the OpenEXR file layout: magic number, version
 // 2, a header of attributes, the scanline offset table, then one block per scanline.
 #include <algorithm>
@@ -39,6 +40,38 @@ struct Bytes {
 };
 
 } // namespace
+
+Image::Image(size_t width, size_t height, size_t channel_count) : width_(width), height_(height) {
+    channels_.resize(channel_count);
+    for (auto& c : channels_) {
+        c = std::make_unique<float[]>(width * height);
+        std::fill(c.get(), c.get() + width * height, 0.0f);
+    }
+}
+
+void Image::clear(float value) {
+    for (auto& c : channels_) std::fill(c.get(), c.get() + width_ * height_, value);
+}
+
+void Image::scale(float value) {
+    for (auto& c : channels_)
+        for (size_t i = 0; i < width_ * height_; ++i) c[i] *= value;
+}
+
+bool Image::save_pfm(std::string_view path) const {
+    if (channel_count() != 3) return false;
+    std::string p(path);
+    FILE* f = std::fopen(p.c_str(), "wb");
+    if (!f) return false;
+    std::fprintf(f, "PF\n%zu %zu\n-1.0\n", width_, height_);
+    std::vector<float> row(width_ * 3);
+    for (size_t y = height_; y-- > 0;) {
+        for (size_t x = 0; x < width_; ++x)
+            for (int c = 0; c < 3; ++c) row[x * 3 + c] = channels_[c][y * width_ + x];
+        std::fwrite(row.data(), sizeof(float), row.size(), f);
+    }
+    return std::fclose(f) == 0;
+}
 
 bool Image::save_exr(std::string_view path) const {
     const size_t n = channel_count();

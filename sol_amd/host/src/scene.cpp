@@ -168,40 +168,6 @@ TriangleMesh::TriangleMesh(
     times_.sah_final = bvh_.sah_cost(config.traversal_cost, config.intersection_cost);
 }
 
-// -- Image ------------------------------------------------------------------------------------------
-
-Image::Image(size_t width, size_t height, size_t channel_count) : width_(width), height_(height) {
-    channels_.resize(channel_count);
-    for (auto& c : channels_) {
-        c = std::make_unique<float[]>(width * height);
-        std::fill(c.get(), c.get() + width * height, 0.0f);
-    }
-}
-
-void Image::clear(float value) {
-    for (auto& c : channels_) std::fill(c.get(), c.get() + width_ * height_, value);
-}
-
-void Image::scale(float value) {
-    for (auto& c : channels_)
-        for (size_t i = 0; i < width_ * height_; ++i) c[i] *= value;
-}
-
-bool Image::save_pfm(std::string_view path) const {
-    if (channel_count() != 3) return false;
-    std::string p(path);
-    FILE* f = std::fopen(p.c_str(), "wb");
-    if (!f) return false;
-    std::fprintf(f, "PF\n%zu %zu\n-1.0\n", width_, height_);
-    std::vector<float> row(width_ * 3);
-    for (size_t y = height_; y-- > 0;) {
-        for (size_t x = 0; x < width_; ++x)
-            for (int c = 0; c < 3; ++c) row[x * 3 + c] = channels_[c][y * width_ + x];
-        std::fwrite(row.data(), sizeof(float), row.size(), f);
-    }
-    return std::fclose(f) == 0;
-}
-
 // -- Flattening -------------------------------------------------------------------------------------
 
 solhip_scene_desc FlatScene::gpu_desc() const {
@@ -249,15 +215,6 @@ FlatScene flatten_scene(const Scene& scene, uint32_t top_levels) {
     f.camera = scene.camera->basis();
 
     size_t nt = mesh->triangle_count();
-    size_t nv = mesh->vertices().size();
-    f.positions.resize(nv * 3);
-    f.normals.resize(nv * 3);
-    for (size_t i = 0; i < nv; ++i) {
-        copy3(&f.positions[i * 3], mesh->vertices()[i]);
-        copy3(&f.normals[i * 3], mesh->normals()[i]);
-    }
-    f.indices.resize(nt * 3);
-    for (size_t i = 0; i < nt * 3; ++i) f.indices[i] = static_cast<uint32_t>(mesh->indices()[i]);
     f.tri_material.assign(nt, -1);
     f.tri_light.assign(nt, -1);
     for (size_t t = 0; t < nt; ++t) {
@@ -273,6 +230,25 @@ FlatScene flatten_scene(const Scene& scene, uint32_t top_levels) {
         if (t >= nt || it == light_index.end()) throw std::invalid_argument("mesh light is not owned by the scene");
         f.tri_light[t] = it->second;
     }
+    flatten_geometry(*mesh, f, top_levels);
+    f.flatten_seconds = seconds_since(t0);
+    return f;
+}
+
+void flatten_geometry(const TriangleMesh& mesh_ref, FlatScene& f, uint32_t top_levels) {
+    const TriangleMesh* mesh = &mesh_ref;
+    size_t nt = mesh->triangle_count();
+    size_t nv = mesh->vertices().size();
+    if (f.tri_material.size() != nt || f.tri_light.size() != nt)
+        throw std::invalid_argument("per-triangle material and light tables must match the mesh");
+    f.positions.resize(nv * 3);
+    f.normals.resize(nv * 3);
+    for (size_t i = 0; i < nv; ++i) {
+        copy3(&f.positions[i * 3], mesh->vertices()[i]);
+        copy3(&f.normals[i * 3], mesh->normals()[i]);
+    }
+    f.indices.resize(nt * 3);
+    for (size_t i = 0; i < nt * 3; ++i) f.indices[i] = static_cast<uint32_t>(mesh->indices()[i]);
 
     const bvh::Bvh& b = mesh->bvh();
     f.bvh_nodes.resize(b.nodes.size());
@@ -321,8 +297,6 @@ FlatScene flatten_scene(const Scene& scene, uint32_t top_levels) {
         s.light = f.tri_light[t];
         s.bsdf = s.material >= 0 ? f.materials[s.material].type : SOL_BSDF_NONE;
     }
-    f.flatten_seconds = seconds_since(t0);
-    return f;
 }
 
 } // namespace sol

@@ -177,6 +177,29 @@ class Scene:
         arr = (F.Light * len(lights))(*lights)
         F.check_host(F.host.sol_scene_set_lights(self._h, arr, len(lights)), "sol_scene_set_lights")
 
+    # -- sol::Geometry of the root mesh ---------------------------------------------------------
+    def attach_gpu(self, ctx):
+        """TriangleMesh::attach_gpu: the mesh's BVH and triangles uploaded to the context for the
+        Geometry queries below (the context must outlive the attachment)."""
+        F.check_host(F.host.sol_scene_attach_gpu(self._h, ctx.handle), "sol_scene_attach_gpu")
+        self._attached_ctx = ctx
+
+    def geometry_intersect(self, rays, any_hit=False, batched=True):
+        """The root's Geometry::intersect_closest / intersect_any on rays (n, 8) float32 (org, tmin,
+        dir, tmax): one virtual call per ray (batched False) or one batched call. Closest: a
+        SurfaceHit structured array; any: int32 flags."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        if any_hit:
+            out = np.zeros(n, np.int32)
+            F.check_host(F.host.sol_scene_intersect_any(self._h, rays.ctypes.data, n, int(batched), out.ctypes.data),
+                         "sol_scene_intersect_any")
+            return out
+        out = (F.SurfaceHit * max(1, n))()
+        F.check_host(F.host.sol_scene_intersect_closest(self._h, rays.ctypes.data, n, int(batched), C.addressof(out)),
+                     "sol_scene_intersect_closest")
+        return surface_hits_to_numpy(out, n)
+
     def arrays(self):
         """Copies of the raw mesh arrays (original triangle order)."""
         mv = self.mesh_view()
@@ -195,6 +218,17 @@ class Scene:
         bounds = nodes[:, :6].view(np.float32)
         prims = _array(bv.prim_indices, bv.num_prims, np.uint32)
         return bounds, nodes[:, 6].copy(), nodes[:, 7].copy(), prims
+
+
+def surface_hits_to_numpy(arr, n):
+    """Dict of numpy arrays from a ctypes array of SurfaceHit records."""
+    raw = np.ctypeslib.as_array(C.cast(arr, C.POINTER(C.c_uint32)), shape=(max(1, n) * 24,)).copy()
+    raw = raw.reshape(-1, 24)[:n]
+    f = raw.view(np.float32)
+    return {"hit": raw[:, 0].view(np.int32), "front": raw[:, 1].view(np.int32), "material": raw[:, 2].view(np.int32),
+            "light": raw[:, 3].view(np.int32), "t": f[:, 4], "point": f[:, 5:8], "face_normal": f[:, 8:11],
+            "tangent": f[:, 11:14], "bitangent": f[:, 14:17], "normal": f[:, 17:20], "surf_coords": f[:, 20:22],
+            "tex_coords": f[:, 22:24]}
 
 
 def gen_rays(kind, n, seed=7, offset=1e-5):

@@ -47,6 +47,7 @@ lib.oracle_fresnel.argtypes = [C.c_float, C.c_float, C.c_float]
 lib.oracle_sample_pixel.restype = None
 lib.oracle_sample_pixel.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]
 lib.oracle_camera_ray.argtypes = [_P, C.c_float, C.c_float, _P, _P]
+lib.oracle_intersect_surface.argtypes = [_P, _P, C.c_uint64, _P]
 
 
 def f32(*v):
@@ -204,6 +205,15 @@ class OracleScene:
         org, d = np.zeros(3, np.float32), np.zeros(3, np.float32)
         assert lib.oracle_camera_ray(self._h, float(u), float(v), org.ctypes.data, d.ctypes.data) == 0
         return org, d
+
+    def intersect_surface(self, rays):
+        """Decoded closest hits (the dict of sol_amd.scene.surface_hits_to_numpy)."""
+        from sol_amd.scene import surface_hits_to_numpy
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        out = (F.SurfaceHit * max(1, n))()
+        assert lib.oracle_intersect_surface(self._h, rays.ctypes.data, n, C.addressof(out)) == 0
+        return surface_hits_to_numpy(out, n)
 
     def intersect(self, rays, any_hit=False, brute_force=False, stats=None):
         rays = np.ascontiguousarray(rays, np.float32)
