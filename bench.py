@@ -29,6 +29,9 @@ Also reported:
                 solhip_render_batch call, with the default launch parameters and after
                 solhip_autotune, against the same jobs rendered one by one with the megakernel
                 (all three timed alike: one warm-up, then the mean of the timed steps).
+  host_frames   frames of 1 and 8 samples of the 1080p frame rendered into a host image (the sums
+                return to host memory once per frame, as for sol::RenderJob over GpuPathTracer)
+                against the same frames into a device image.
   adjoint       solhip_backward on the sphere field with 256 material groups at 512x512, 32 spp
                 (Russian roulette off), with a central-difference check (h = 1e-3, common random
                 numbers) of the largest albedo, Phong log-exponent and emission gradients.
@@ -264,6 +267,36 @@ def bench_gpu_build(sol_amd, F, torch, ctx, scene, cfg, spp):
     return out
 
 
+def bench_host_frames(sol_amd, F, torch, g, cfg):
+    """Frames of a few samples rendered into a host image (what sol::RenderJob does through
+    GpuPathTracer: each frame's sums come back to host memory and are added there) against the
+    same frames into a device image, at 1920x1080 on the wavefront renderer."""
+    import numpy as np
+    out = {"width": WIDTH, "height": HEIGHT}
+    host = np.zeros((3, HEIGHT, WIDTH), np.float32)
+    dev = torch.zeros((3, HEIGHT, WIDTH), dtype=torch.float32, device="cuda")
+    for spf in (1, 8):
+        frames = 4
+        g.render(host, 0, spf, cfg)                       # warm-up
+        g.render_device(dev.data_ptr(), WIDTH, HEIGHT, 0, spf, cfg)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for f in range(frames):
+            g.render(host, f * spf, spf, cfg)
+        host_s = (time.perf_counter() - t) / frames
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for f in range(frames):
+            g.render_device(dev.data_ptr(), WIDTH, HEIGHT, f * spf, spf, cfg)
+        torch.cuda.synchronize()
+        dev_s = (time.perf_counter() - t) / frames
+        out[f"spf{spf}"] = {"host_image_ms": round(host_s * 1e3, 2), "device_image_ms": round(dev_s * 1e3, 2),
+                            "host_image_msamples_s": round(WIDTH * HEIGHT * spf / host_s / 1e6, 1),
+                            "device_image_msamples_s": round(WIDTH * HEIGHT * spf / dev_s / 1e6, 1)}
+    log(f"host frames: {out}")
+    return out
+
+
 def bench_adjoint(sol_amd, torch):
     import numpy as np
     from sol_amd.diff import DiffRenderer
@@ -427,6 +460,7 @@ def main():
         f"TA busy {ta_busy}, VALU busy {valu_busy}")
 
     gpu_build = None if args.no_small else bench_gpu_build(sol_amd, F, torch, ctx, scene, cfg, args.spp)
+    host_frames = None if args.no_small else bench_host_frames(sol_amd, F, torch, g, cfg)
     intersect = None if args.no_small else bench_intersect(sol_amd, torch, not args.no_cpu)
     cornell = None if args.no_small else bench_cornell(sol_amd, F, torch, not args.no_cpu)
     batched = None if args.no_batch else bench_batched(sol_amd, F, torch, args.steps)
@@ -505,6 +539,7 @@ def main():
                                             "shade": round(wst.shade_ms / args.steps, 2)}},
         "cpu_baseline": cpu,
         "gpu_build": gpu_build,
+        "host_frames": host_frames,
         "intersect": intersect,
         "cornell": cornell,
         "batched": batched,

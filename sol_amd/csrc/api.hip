@@ -78,6 +78,16 @@ LaunchShape Context::shape(int block, int lds) const {
     return s;
 }
 
+int Context::reserve_pinned(size_t bytes) {
+    if (pinned && bytes <= pinned_bytes) return SOLHIP_OK;
+    if (pinned) HIP_TRY(hipHostFree(pinned));
+    pinned = nullptr;
+    pinned_bytes = 0;
+    HIP_TRY(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
+    pinned_bytes = bytes;
+    return SOLHIP_OK;
+}
+
 int Context::check_overflow() {
     int flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, scratch.overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -142,6 +152,7 @@ void destroy_context(solhip_ctx* ctx) {
     if (ctx->scratch.spill) (void)hipFree(ctx->scratch.spill);
     if (ctx->scratch.overflow) (void)hipFree(ctx->scratch.overflow);
     if (ctx->counters) (void)hipFree(ctx->counters);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     (void)hipEventDestroy(ctx->ev_begin);
     (void)hipEventDestroy(ctx->ev_end);
     (void)hipStreamDestroy(ctx->stream);
@@ -226,7 +237,8 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     *out = nullptr;
     if (!d->num_nodes || !d->nodes || !d->num_tris || !d->tris || !d->shade)
         return fail(SOLHIP_E_INVALID, "scene without geometry");
-    if (!d->num_lights || !d->lights) return fail(SOLHIP_E_NO_LIGHTS, "scene without light");
+    // a scene without light serves intersection queries; rendering it is refused (SOLHIP_E_NO_LIGHTS)
+    if (d->num_lights && !d->lights) return fail(SOLHIP_E_INVALID, "null light table");
     if (d->num_materials && !d->materials) return fail(SOLHIP_E_INVALID, "null material table");
     if (const char* e = check_materials(d->materials, d->num_materials)) return fail(SOLHIP_E_INVALID, e);
     if (d->num_tris >= (1u << 28)) return fail(SOLHIP_E_INVALID, "too many triangles");
@@ -292,13 +304,14 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     HIP_TRY(hipMalloc(&s->tris, sizeof(sol_gpu_tri) * d->num_tris));
     HIP_TRY(hipMalloc(&s->shade, sizeof(sol_gpu_shade) * d->num_tris));
     HIP_TRY(hipMalloc(&s->materials, sizeof(sol_material) * std::max<uint32_t>(1, d->num_materials)));
-    HIP_TRY(hipMalloc(&s->lights, sizeof(sol_light) * d->num_lights));
+    HIP_TRY(hipMalloc(&s->lights, sizeof(sol_light) * std::max<uint32_t>(1, d->num_lights)));
     HIP_TRY(hipMemcpy(s->nodes, d->nodes, sizeof(sol_gpu_node) * d->num_nodes, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->tris, d->tris, sizeof(sol_gpu_tri) * d->num_tris, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->shade, d->shade, sizeof(sol_gpu_shade) * d->num_tris, hipMemcpyHostToDevice));
     if (d->num_materials)
         HIP_TRY(hipMemcpy(s->materials, d->materials, sizeof(sol_material) * d->num_materials, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->lights, d->lights, sizeof(sol_light) * d->num_lights, hipMemcpyHostToDevice));
+    if (d->num_lights)
+        HIP_TRY(hipMemcpy(s->lights, d->lights, sizeof(sol_light) * d->num_lights, hipMemcpyHostToDevice));
     SceneView& v = s->view;
     v.nodes = reinterpret_cast<const float4*>(s->nodes);
     v.nodes4 = reinterpret_cast<const float4*>(s->nodes4);
@@ -421,14 +434,15 @@ int solhip_intersect(solhip_scene* s, const sol_ray* rays, uint64_t n, int any_h
     HIP_TRY(hipSetDevice(ctx->device));
     const sol_ray* d_rays = rays;
     void* d_out = out;
-    DeviceBuffer tmp_rays, tmp_out;
     size_t out_bytes = any_hit ? sizeof(int32_t) * n : sizeof(sol_hit) * n;
     if (!device_ptrs) {
-        HIP_TRY(tmp_rays.alloc(sizeof(sol_ray) * n));
-        HIP_TRY(tmp_out.alloc(out_bytes));
-        HIP_TRY(hipMemcpyAsync(tmp_rays.ptr, rays, sizeof(sol_ray) * n, hipMemcpyHostToDevice, ctx->stream));
-        d_rays = static_cast<const sol_ray*>(tmp_rays.ptr);
-        d_out = tmp_out.ptr;
+        // staging buffers of the context, grown on demand: many small calls (one ray per call
+        // through sol::Geometry) allocate nothing
+        HIP_TRY(ctx->isect_rays.reserve(sizeof(sol_ray) * n));
+        HIP_TRY(ctx->isect_out.reserve(out_bytes));
+        HIP_TRY(hipMemcpyAsync(ctx->isect_rays.ptr, rays, sizeof(sol_ray) * n, hipMemcpyHostToDevice, ctx->stream));
+        d_rays = ctx->isect_rays.as<const sol_ray>();
+        d_out = ctx->isect_out.ptr;
     }
     LaunchShape shape = ctx->shape(ctx->tuning.block_size, ctx->tuning.lds_stack);
     int st = ctx->ensure_scratch(shape.grid * shape.block);
@@ -460,16 +474,19 @@ int solhip_render(solhip_scene* s, const sol_pt_config* cfg, int renderer, uint3
         return fail(SOLHIP_E_INVALID, "unknown renderer");
     if (cfg->min_survival_prob < 0 || cfg->max_survival_prob > 1 || cfg->min_survival_prob > cfg->max_survival_prob)
         return fail(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    if (!s->num_lights) return fail(SOLHIP_E_NO_LIGHTS, "scene without light cannot be rendered");
     if (sample_count == 0) return SOLHIP_OK;
     Context* ctx = s->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
-    size_t plane = static_cast<size_t>(width) * height;
-    DeviceBuffer tmp;
+    const size_t n = 3 * static_cast<size_t>(width) * height;
     float* d_accum = accum;
     if (!device_ptr) {
-        HIP_TRY(tmp.alloc(sizeof(float) * 3 * plane));
-        HIP_TRY(hipMemcpyAsync(tmp.ptr, accum, sizeof(float) * 3 * plane, hipMemcpyHostToDevice, ctx->stream));
-        d_accum = static_cast<float*>(tmp.ptr);
+        // A host image: the samples are summed into a cleared device staging image, which comes
+        // back once through pinned staging and is added on the host. The image is not sent to the
+        // device: accum + (0 + s) is accum + s bit for bit (a pixel's sum s is never -0).
+        HIP_TRY(ctx->host_image.reserve(sizeof(float) * n));
+        d_accum = ctx->host_image.as<float>();
+        HIP_TRY(launch_zero_f32(d_accum, n, ctx->stream));
     }
     int st;
     if (renderer == SOLHIP_RENDERER_MEGAKERNEL)
@@ -477,8 +494,13 @@ int solhip_render(solhip_scene* s, const sol_pt_config* cfg, int renderer, uint3
     else
         st = render_wavefront(*s, *cfg, width, height, sample_index, sample_count, d_accum);
     if (st) return st;
-    if (!device_ptr)
-        HIP_TRY(hipMemcpyAsync(accum, d_accum, sizeof(float) * 3 * plane, hipMemcpyDeviceToHost, ctx->stream));
+    if (!device_ptr) {
+        if ((st = ctx->reserve_pinned(sizeof(float) * n))) return st;
+        HIP_TRY(hipMemcpyAsync(ctx->pinned, d_accum, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        const float* sum = static_cast<const float*>(ctx->pinned);
+        for (size_t i = 0; i < n; ++i) accum[i] += sum[i];
+    }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return ctx->check_overflow();
 }
@@ -490,6 +512,7 @@ int solhip_backward(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
     if (width == 0 || height == 0) return fail(SOLHIP_E_INVALID, "empty image");
     if (cfg->min_survival_prob < 0 || cfg->max_survival_prob > 1 || cfg->min_survival_prob > cfg->max_survival_prob)
         return fail(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    if (!s->num_lights) return fail(SOLHIP_E_NO_LIGHTS, "scene without light cannot be rendered");
     if (sample_count == 0 || cfg->max_path_len == 0) return SOLHIP_OK;   // no path, no gradient
     Context* ctx = s->ctx;
     HIP_TRY(hipSetDevice(ctx->device));

@@ -2,6 +2,8 @@
 // of sol) and the plain GPU megakernel (PathTracer::render + trace_path,
 // src/algorithms/path_tracer.cpp:9-120), one thread per pixel looping over its samples.
 // Both are persistent: each wave fetches 64 work items at a time from an atomic counter.
+#include <algorithm>
+
 #include "launch.h"
 #include "megapath.h"
 
@@ -92,6 +94,11 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
     if (i < n) p[i] = 0;
 }
 
+__global__ __launch_bounds__(256) void zero_f32_kernel(float* p, size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256)
+        p[i] = 0.0f;
+}
+
 template <int BLOCK, int LDS>
 hipError_t intersect_impl(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
                           int32_t* occluded, unsigned long long* counter, const TraversalScratch& sc, int grid,
@@ -156,6 +163,14 @@ hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint3
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t stream) {
     (void)hipGetLastError();
     hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_f32(float* p, size_t n, hipStream_t stream) {
+    (void)hipGetLastError();
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, p, n);
     return hipGetLastError();
 }
 

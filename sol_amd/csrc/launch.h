@@ -45,6 +45,8 @@ hipError_t launch_adjoint(const SceneView& s, const sol_pt_config& cfg, uint32_t
                           const TraversalScratch& scratch, const LaunchShape& shape, hipStream_t stream);
 
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t stream);
+// Clears n floats (a kernel, not a memset, so that it can be captured in a graph).
+hipError_t launch_zero_f32(float* p, size_t n, hipStream_t stream);
 
 // Workgroups for a persistent launch: enough for every CU at full occupancy.
 int persistent_grid(int compute_units, int block);

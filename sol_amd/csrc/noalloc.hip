@@ -66,6 +66,7 @@ int solhip_render_noalloc(solhip_scene* s, const sol_pt_config* cfg, uint32_t wi
     if (width == 0 || height == 0) return set_error(SOLHIP_E_INVALID, "empty image");
     if (cfg->min_survival_prob < 0 || cfg->max_survival_prob > 1 || cfg->min_survival_prob > cfg->max_survival_prob)
         return set_error(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    if (!s->num_lights) return set_error(SOLHIP_E_NO_LIGHTS, "scene without light cannot be rendered");
     if (sample_count == 0) return SOLHIP_OK;
     Context* ctx = s->ctx;
     void* base = nullptr;

@@ -79,12 +79,17 @@ struct Context {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     WavefrontState* wave = nullptr;
     DeviceBuffer adj_radiance, adj_image;   // forward pass of solhip_backward (grow-only)
+    DeviceBuffer isect_rays, isect_out;     // host-pointer staging of solhip_intersect (grow-only)
+    DeviceBuffer host_image;                // device image of solhip_render for host images (grow-only)
+    void* pinned = nullptr;                 // pinned host staging (grow-only)
+    size_t pinned_bytes = 0;
     int live_scenes = 0;             // scenes uploaded and not yet destroyed
     bool destroy_requested = false;  // solhip_destroy was called while scenes were alive
 
     int ensure_scratch(int threads);
     LaunchShape shape(int block, int lds) const;
     int check_overflow();
+    int reserve_pinned(size_t bytes);
     int read_counters(unsigned long long* host, int n);
     void release_wavefront();
 };

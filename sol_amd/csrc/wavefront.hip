@@ -455,6 +455,7 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
         const solhip_batch_job& j = jobs[i];
         if (!j.scene || !j.accum || j.width == 0 || j.height == 0) return set_error(SOLHIP_E_INVALID, "invalid batch job");
         if (j.scene->ctx != ctx) return set_error(SOLHIP_E_CONTEXT, "scene belongs to another context");
+        if (!j.scene->num_lights) return set_error(SOLHIP_E_NO_LIGHTS, "scene without light cannot be rendered");
         if (j.sample_count == 0) continue;
         auto it = index.find(j.scene);
         uint32_t si;
@@ -571,6 +572,7 @@ int solhip_autotune(solhip_scene* s, const sol_pt_config* cfg, uint32_t width, u
     if (!s || !cfg) return set_error(SOLHIP_E_INVALID, "null argument");
     if (width == 0 || height == 0) return set_error(SOLHIP_E_INVALID, "empty image");
     if (!valid_survival(*cfg)) return set_error(SOLHIP_E_INVALID, "survival probabilities must satisfy 0 <= min <= max <= 1");
+    if (!s->num_lights) return set_error(SOLHIP_E_NO_LIGHTS, "scene without light cannot be rendered");
     if (budget_spp == 0) budget_spp = 1;
     Context& ctx = *s->ctx;
     HIP_OK(hipSetDevice(ctx.device));
