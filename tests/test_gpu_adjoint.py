@@ -53,6 +53,25 @@ def test_adjoint_matches_oracle_adjoint(ctx, scene_kind, rr):
     compare(dl, ol)
 
 
+def test_adjoint_matches_oracle_at_256_material_groups(ctx):
+    # The differentiable configuration's scene: the 400-sphere field (2,048,008 triangles) with 256
+    # material groups, depth 16, Russian roulette off; a loss gradient over a 256 x 256 image at 4 spp,
+    # the oracle traversing the same BVH.
+    scene = sol_amd.Scene.sphere_field(num_groups=256, aspect=1.0)
+    mats, lights = scene.materials(), scene.lights()
+    assert len(mats) >= 256
+    cfg = sol_amd.PathTracerConfig(max_path_len=16, disable_rr=True)
+    w = h = 256
+    d_image = np.random.default_rng(17).standard_normal((3, h, w)).astype(np.float32)
+    print(f"adjoint: {scene.num_tris} triangles, {len(mats)} materials, {w}x{h} 4 spp", flush=True)
+    dm, dl = ctx.upload(scene).backward(d_image, 0, 4, cfg)
+    om, ol = O.backward(O.OracleScene(scene, use_product_bvh=True), d_image, 0, 4, cfg, len(mats), len(lights))
+    # the diffuse groups (about a third) receive albedo gradients, the Phong ones exponent gradients
+    assert (np.abs(om[:, :3]).max(axis=1) > 0).sum() >= 64 and (np.abs(om[:, 3]) > 0).sum() >= 64
+    compare(dm, om)
+    compare(dl, ol)
+
+
 def test_adjoint_host_and_device_pointers_agree_and_accumulate(ctx):
     scene = sol_amd.Scene.cornell_box()
     cfg = sol_amd.PathTracerConfig(max_path_len=6)
