@@ -388,6 +388,60 @@ SOL_DEV void stack_push(const Stack& st, TraversalState& ts, int v) {
     ++ts.sp;
 }
 
+// The four child boxes of a 4-wide node against the ray: entry distances, hits (an empty slot never
+// hits) and the child references.
+SOL_DEV void test_node4(const RayPre& r, const Node4& n, float tmax, float& t0, float& t1, float& t2, float& t3,
+                        bool& h0, bool& h1, bool& h2, bool& h3, int& c0, int& c1, int& c2, int& c3) {
+    h0 = intersect_box_nf(r, n.near_x.x, n.far_x.x, n.near_y.x, n.far_y.x, n.near_z.x, n.far_z.x, tmax, t0);
+    h1 = intersect_box_nf(r, n.near_x.y, n.far_x.y, n.near_y.y, n.far_y.y, n.near_z.y, n.far_z.y, tmax, t1);
+    h2 = intersect_box_nf(r, n.near_x.z, n.far_x.z, n.near_y.z, n.far_y.z, n.near_z.z, n.far_z.z, tmax, t2);
+    h3 = intersect_box_nf(r, n.near_x.w, n.far_x.w, n.near_y.w, n.far_y.w, n.near_z.w, n.far_z.w, tmax, t3);
+    c0 = n.child.x;
+    c1 = n.child.y;
+    c2 = n.child.z;
+    c3 = n.child.w;
+    h0 = h0 && c0 != SOL_GPU_EMPTY;
+    h1 = h1 && c1 != SOL_GPU_EMPTY;
+    h2 = h2 && c2 != SOL_GPU_EMPTY;
+    h3 = h3 && c3 != SOL_GPU_EMPTY;
+}
+
+// Pushes the k (0..3) farther hit children of a 4-wide node, farthest first: c3 (k == 3), c2
+// (k >= 2), c1 (k >= 1), so that c1 ends on top. While the three fit in the LDS part of the stack
+// the writes are unconditional: child j goes to sp + k - j if j <= k, else to sp + k, a slot at or
+// above the new top that no pop reads before it is written again. Otherwise one push at a time.
+template <int BLOCK, int LDS_STACK>
+SOL_DEV void stack_push_children(const Stack& st, TraversalState& ts, int k, int c1, int c2, int c3) {
+    if (ts.sp + 3 <= LDS_STACK) {
+        const int top = ts.sp + k;
+        st.lds[(top - (k >= 3 ? 3 : 0)) * BLOCK] = c3;
+        st.lds[(top - (k >= 2 ? 2 : 0)) * BLOCK] = c2;
+        st.lds[(top - (k >= 1 ? 1 : 0)) * BLOCK] = c1;
+        ts.sp = top;
+    } else {
+        if (k > 2) stack_push<BLOCK, LDS_STACK>(st, ts, c3);
+        if (k > 1) stack_push<BLOCK, LDS_STACK>(st, ts, c2);
+        if (k > 0) stack_push<BLOCK, LDS_STACK>(st, ts, c1);
+    }
+}
+
+// Any-hit order: the first hit child (in slot order) is visited next and the other hit children
+// are pushed in slot order - an any-hit query stops at the first occluder it meets, so the order
+// does not change its answer. Each child's stack slot is its rank among the hit children after the
+// first (a prefix count); a child that is not pushed writes to the slot at the new top, which no
+// pop reads before it is written again. Needs sp + 3 <= LDS_STACK.
+template <int BLOCK>
+SOL_DEV int push_hit_children_any(const Stack& st, TraversalState& ts, int hits, bool h0, bool h1, bool h2, bool h3,
+                                  int c0, int c1, int c2, int c3) {
+    const int s1 = int(h0), s2 = s1 + int(h1), s3 = s2 + int(h2);
+    const int top = ts.sp + hits - 1;
+    st.lds[(h1 && s1 >= 1 ? ts.sp + s1 - 1 : top) * BLOCK] = c1;
+    st.lds[(h2 && s2 >= 1 ? ts.sp + s2 - 1 : top) * BLOCK] = c2;
+    st.lds[(h3 && s3 >= 1 ? ts.sp + s3 - 1 : top) * BLOCK] = c3;
+    ts.sp = top;
+    return h0 ? c0 : (h1 ? c1 : (h2 ? c2 : c3));
+}
+
 // Orders two (distance, child) pairs by distance.
 SOL_DEV void order_pair(float& ta, int& ca, float& tb, int& cb) {
     bool s = tb < ta;
@@ -410,24 +464,22 @@ template <int BLOCK, int LDS_STACK, typename Fetch>
 SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, float tmin, bool any, TraversalState& ts) {
     constexpr float kMiss = __builtin_huge_valf();
     int leaf = 0;          // postponed leaf (leaf references are negative), 0 = none
+    int leaf2 = 0;         // a second postponed leaf
+    int leaf3 = 0;         // a third
     bool more = true;      // ts.node still has to be visited
     while (__ballot(more && leaf == 0) != 0) {
         if (more && ts.node >= 0) {
             Node4 n;
             f.node(ts.node, r, n);
             float t0, t1, t2, t3;
-            bool h0 = intersect_box_nf(r, n.near_x.x, n.far_x.x, n.near_y.x, n.far_y.x, n.near_z.x, n.far_z.x, ts.tmax, t0);
-            bool h1 = intersect_box_nf(r, n.near_x.y, n.far_x.y, n.near_y.y, n.far_y.y, n.near_z.y, n.far_z.y, ts.tmax, t1);
-            bool h2 = intersect_box_nf(r, n.near_x.z, n.far_x.z, n.near_y.z, n.far_y.z, n.near_z.z, n.far_z.z, ts.tmax, t2);
-            bool h3 = intersect_box_nf(r, n.near_x.w, n.far_x.w, n.near_y.w, n.far_y.w, n.near_z.w, n.far_z.w, ts.tmax, t3);
-            int c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
-            h0 = h0 && c0 != SOL_GPU_EMPTY;
-            h1 = h1 && c1 != SOL_GPU_EMPTY;
-            h2 = h2 && c2 != SOL_GPU_EMPTY;
-            h3 = h3 && c3 != SOL_GPU_EMPTY;
+            bool h0, h1, h2, h3;
+            int c0, c1, c2, c3;
+            test_node4(r, n, ts.tmax, t0, t1, t2, t3, h0, h1, h2, h3, c0, c1, c2, c3);
             const int hits = int(h0) + int(h1) + int(h2) + int(h3);
             if (hits == 0) {
                 more = stack_pop<BLOCK, LDS_STACK>(st, ts);
+            } else if (any && ts.sp + 3 <= LDS_STACK) {
+                ts.node = push_hit_children_any<BLOCK>(st, ts, hits, h0, h1, h2, h3, c0, c1, c2, c3);
             } else {
                 float k0 = h0 ? t0 : kMiss, k1 = h1 ? t1 : kMiss, k2 = h2 ? t2 : kMiss, k3 = h3 ? t3 : kMiss;
                 order_pair(k0, c0, k1, c1);
@@ -435,23 +487,27 @@ SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, flo
                 order_pair(k0, c0, k2, c2);
                 order_pair(k1, c1, k3, c3);
                 order_pair(k1, c1, k2, c2);
-                if (hits > 3) stack_push<BLOCK, LDS_STACK>(st, ts, c3);
-                if (hits > 2) stack_push<BLOCK, LDS_STACK>(st, ts, c2);
-                if (hits > 1) stack_push<BLOCK, LDS_STACK>(st, ts, c1);
+                stack_push_children<BLOCK, LDS_STACK>(st, ts, hits - 1, c1, c2, c3);
                 ts.node = c0;
             }
         }
-        if (more && ts.node < 0 && leaf == 0) {
-            leaf = ts.node;
+        if (more && ts.node < 0 && leaf3 == 0) {
+            if (leaf == 0) leaf = ts.node;
+            else if (leaf2 == 0) leaf2 = ts.node;
+            else leaf3 = ts.node;
             more = stack_pop<BLOCK, LDS_STACK>(st, ts);
         }
     }
     if (leaf != 0) {
-        uint32_t first, count;
+        uint32_t first, count, first2 = 0, count2 = 0, first3 = 0, count3 = 0;
         decode_leaf(leaf, first, count);
-        for (uint32_t i = 0; i < count; ++i) {
+        if (leaf2 != 0) decode_leaf(leaf2, first2, count2);
+        if (leaf3 != 0) decode_leaf(leaf3, first3, count3);
+        const uint32_t total = count + count2 + count3;
+        for (uint32_t j = 0; j < total; ++j) {
+            const uint32_t i = j < count ? first + j : (j < count + count2 ? first2 + (j - count) : first3 + (j - count - count2));
             float4 p0, p1, p2;
-            f.tri(first + i, p0, p1, p2);
+            f.tri(i, p0, p1, p2);
             float t, u, v;
             if (intersect_triangle(r, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, t, u, v)) {
                 int prim = __float_as_int(p0.w);
@@ -459,7 +515,7 @@ SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, flo
                     if (t > tmin && t < ts.tmax) return kStepOccluded;
                 } else if (t > tmin && (t < ts.tmax || (t == ts.tmax && ts.hit.prim >= 0 && prim < ts.hit.prim))) {
                     ts.tmax = t;
-                    ts.hit.slot = static_cast<int>(first + i);
+                    ts.hit.slot = static_cast<int>(i);
                     ts.hit.prim = prim;
                     ts.hit.t = t;
                     ts.hit.u = u;
