@@ -96,6 +96,14 @@ typedef struct solhip_tuning {
     uint32_t chunk_items;   /* wavefront: work items (path samples) per chunk of work, the
                                capacity of the per-sample radiance buffer (16 B an item);
                                65536..2^28, default 192 Mi (3 GiB). Whole tiles make a chunk. */
+    uint32_t queue_segments;/* wavefront queue split: segments per queue, 1 (default), 2, 4 or
+                               8. Each segment has its own length counter, so the appends of
+                               a pass spread their device atomics over that many addresses;
+                               one atomic per queue and round of 512 entries is rarely
+                               contended, and one segment keeps each queue contiguous.      */
+    uint32_t shade_queues;  /* wavefront queue split: 3 (default) sorts the hits into diffuse,
+                               glossy and specular queues shaded by one kernel each; 1 keeps one
+                               queue shaded by a kernel for every BSDF. Images are the same.  */
 } solhip_tuning;
 
 /* Counters accumulated by the context since the last solhip_reset_stats. */

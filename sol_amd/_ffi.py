@@ -156,7 +156,7 @@ class SurfaceHit(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("lds_stack", C.c_uint32), ("path_slots", C.c_uint32),
                 ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("graph_iters", C.c_uint32), ("bvh_width", C.c_uint32),
-                ("chunk_items", C.c_uint32)]
+                ("chunk_items", C.c_uint32), ("queue_segments", C.c_uint32), ("shade_queues", C.c_uint32)]
 
 
 class Stats(C.Structure):

@@ -189,7 +189,12 @@ int solhip_set_tuning(solhip_ctx* ctx, const solhip_tuning* t) {
     if (!n.graph_iters) n.graph_iters = d.graph_iters;
     if (!n.bvh_width) n.bvh_width = d.bvh_width;
     if (!n.chunk_items) n.chunk_items = d.chunk_items;
+    if (!n.queue_segments) n.queue_segments = d.queue_segments;
+    if (!n.shade_queues) n.shade_queues = d.shade_queues;
     if (n.chunk_items < (1u << 16) || n.chunk_items > (1u << 28)) return fail(SOLHIP_E_INVALID, "chunk_items out of range");
+    if (n.queue_segments > 8 || (n.queue_segments & (n.queue_segments - 1)))
+        return fail(SOLHIP_E_INVALID, "queue_segments must be 1, 2, 4 or 8");
+    if (n.shade_queues != 1 && n.shade_queues != 3) return fail(SOLHIP_E_INVALID, "shade_queues must be 1 or 3");
     if (n.bvh_width != 2 && n.bvh_width != 4) return fail(SOLHIP_E_INVALID, "bvh_width must be 2 or 4");
     if (!valid_shape_values(n.block_size, n.lds_stack)) return fail(SOLHIP_E_INVALID, "unsupported block size or LDS stack");
     if (n.path_slots < 4096 || n.path_slots > (1u << 27)) return fail(SOLHIP_E_INVALID, "path_slots out of range");

@@ -64,6 +64,8 @@ inline solhip_tuning default_tuning(unsigned long long device_memory) {
     t.graph_iters = 16;
     t.bvh_width = 4;
     t.chunk_items = kDefaultChunkItems;
+    t.queue_segments = 1;
+    t.shade_queues = 3;
     return t;
 }
 

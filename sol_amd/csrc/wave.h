@@ -20,13 +20,19 @@ enum : int {
     kQueueDiffuse = 0,
     kQueuePhong = 1,
     kQueueSpecular = 2,
-    kNumShadeQueues = 3
+    kNumShadeQueues = 3,
+    kQueueAll = 3          // the merged queue's kernel (its entries are stored as queue 0)
 };
 
-// Queues are split into kNumSegments segments, each with its own length counter: a workgroup
-// appends to segment blockIdx % kNumSegments only, so the device atomics of the appends spread
-// over kNumSegments addresses. Readers see a queue as the concatenation of its segments.
-constexpr int kNumSegments = 8;
+// Queues are split into nseg segments (WaveBuf::nseg: 1, 2, 4 or 8, a launch parameter), each with
+// its own length counter: a workgroup appends to segment blockIdx % nseg only, so the device
+// atomics of the appends spread over nseg addresses. Readers see a queue as the concatenation of
+// its segments. Counters and item ranges are laid out for kMaxSegments.
+constexpr int kMaxSegments = 8;
+
+// The shading queues are either material-sorted (kNumShadeQueues: diffuse, glossy, specular, one
+// kernel each) or merged into queue 0 with one kernel for every BSDF (kQueueAll), a launch
+// parameter (WaveBuf::merged).
 
 enum : int {
     kQExtend0 = 0,      // extension queues (two: one read, one written per bounce)
@@ -36,17 +42,17 @@ enum : int {
     kNumQueues = 6
 };
 
-// Device counters: the length of segment s of queue q is counter q * kNumSegments + s, then the
+// Device counters: the length of segment s of queue q is counter q * kMaxSegments + s, then the
 // work-fetch counter of the traversal kernel (these kNumWaveCounters are cleared for every pass),
 // then the start flag: nonzero while extension queue 0 holds the start fill of a chunk, whose
 // entries leave out their first word (the ray origin is the camera's eye, tmin 0). Counters sit
 // kCounterStride words (256 bytes) apart: device atomics on one cache line serialise, so each
 // counter gets a line of its own.
 constexpr int kCounterStride = 64;
-SOL_HD constexpr int qcount(int queue, int seg) { return (queue * kNumSegments + seg) * kCounterStride; }
+SOL_HD constexpr int qcount(int queue, int seg) { return (queue * kMaxSegments + seg) * kCounterStride; }
 enum : int {
-    kNumWaveCounters = kNumQueues * kNumSegments + 1,
-    kCntTraceWork = kNumQueues * kNumSegments * kCounterStride,
+    kNumWaveCounters = kNumQueues * kMaxSegments + 1,
+    kCntTraceWork = kNumQueues * kMaxSegments * kCounterStride,
     kCntStartFill = kNumWaveCounters * kCounterStride,
     kCounterWords = (kNumWaveCounters + 1) * kCounterStride
 };
@@ -57,9 +63,9 @@ enum : int {
 // items[kItemChunkBegin] is the first item of the chunk (index 0 of the sample buffer).
 constexpr int kItemStride = 32;
 SOL_HD constexpr int item_next(int seg) { return seg * kItemStride; }
-SOL_HD constexpr int item_end(int seg) { return (kNumSegments + seg) * kItemStride; }
+SOL_HD constexpr int item_end(int seg) { return (kMaxSegments + seg) * kItemStride; }
 enum : int {
-    kItemChunkBegin = 2 * kNumSegments * kItemStride,
+    kItemChunkBegin = 2 * kMaxSegments * kItemStride,
     kItemWords = kItemChunkBegin + 1
 };
 
@@ -92,21 +98,23 @@ enum : int {
 //                             contribution then goes to the sample entry of the item)
 //
 // Segment capacities (entries), for a launch of n input entries handed out in rounds of R =
-// kWaveBlock by static round robin over a grid that is a multiple of kNumSegments: segment s gets
-// the rounds r with r % kNumSegments == s, at most n / kNumSegments + 2 R entries per launch and
+// kWaveBlock by static round robin over a grid that is a multiple of kMaxSegments: segment s gets
+// the rounds r with r % kMaxSegments == s, at most n / kMaxSegments + 2 R entries per launch and
 // output queue. Shade queues and the shadow queue receive from launches whose inputs sum to at
 // most `slots`; an extension queue receives from sort and the three shade launches, whose inputs
 // sum to at most 2 slots.
 struct WaveBuf {
     uint32_t slots;
+    uint32_t nseg;                    // segments per queue (1, 2, 4 or 8)
+    uint32_t merged;                  // 1: one shading queue (queue 0) for every BSDF
     uint32_t cap_e, cap_m, cap_s;     // segment capacities: extension, shade, shadow queues
-    uint32_t stride_e, stride_m, stride_s;   // kNumSegments * cap_e, cap_m, cap_s (the queues' field strides)
+    uint32_t stride_e, stride_m, stride_s;   // nseg * cap_e, cap_m, cap_s (the queues' field strides)
     float4* path;                     // [slots]: slot records
     float4* hits;                     // [slots]: closest hit of extension entry i (in queue order):
                                       // leaf-order triangle (int bits, -1 = miss), t, u, v
-    float4* eq[2];                    // [kNumSegments * cap_e * 4] each
-    float4* mq[kNumShadeQueues];      // [kNumSegments * cap_m * 4] each
-    float4* sq;                       // [kNumSegments * cap_s * 3]
+    float4* eq[2];                    // [nseg * cap_e * 4] each
+    float4* mq[kNumShadeQueues];      // [nseg * cap_m * 4] each (only mq[0] when merged)
+    float4* sq;                       // [nseg * cap_s * 3]
     unsigned* counts;                 // kCounterWords
     unsigned long long* items;        // kItemWords
     unsigned long long* stats;        // [0] extension rays [1] shadow rays [2] samples

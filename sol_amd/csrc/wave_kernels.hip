@@ -67,8 +67,8 @@ struct RoundLds {
     unsigned ext_base;
     unsigned n_cont;
     unsigned n_pieces;              // work items granted: n_pieces ranges [piece_base, + piece_count)
-    unsigned piece_count[kNumSegments];
-    unsigned long long piece_base[kNumSegments];
+    unsigned piece_count[kMaxSegments];
+    unsigned long long piece_base[kMaxSegments];
 };
 
 struct RoundPlan {
@@ -79,13 +79,13 @@ struct RoundPlan {
 };
 
 // Reserves up to `need` work items, own segment first (items[item_next(s)] counts up to
-// items[item_end(s)]). Counters only grow, so a relaxed read at or past the end means the
-// segment is exhausted and it is skipped without an atomic.
+// items[item_end(s)]), then the others of the nseg segments. Counters only grow, so a relaxed
+// read at or past the end means the segment is exhausted and it is skipped without an atomic.
 template <int NQ>
-SOL_DEV void grant_items(RoundLds<NQ>& lds, unsigned long long* items, int seg, unsigned need) {
+SOL_DEV void grant_items(RoundLds<NQ>& lds, unsigned long long* items, int seg, int nseg, unsigned need) {
     unsigned n = 0;
-    for (int k = 0; k < kNumSegments && need > 0; ++k) {
-        int sk = (seg + k) % kNumSegments;
+    for (int k = 0; k < nseg && need > 0; ++k) {
+        int sk = (seg + k) & (nseg - 1);
         const unsigned long long end = items[item_end(sk)];
         if (k > 0 && __hip_atomic_load(&items[item_next(sk)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= end)
             continue;
@@ -103,7 +103,7 @@ SOL_DEV void grant_items(RoundLds<NQ>& lds, unsigned long long* items, int seg, 
 
 template <int NQ>
 SOL_DEV RoundPlan plan_round(RoundLds<NQ>& lds, unsigned* const* qcounters, unsigned* ext_counter,
-                             unsigned long long* items, int seg, int dest, bool cont, bool retire) {
+                             unsigned long long* items, int seg, int nseg, int dest, bool cont, bool retire) {
     const int wave = threadIdx.x >> 6;
     const bool lane0 = __lane_id() == 0;
     const unsigned long long below = lanes_below();
@@ -139,7 +139,7 @@ SOL_DEV RoundPlan plan_round(RoundLds<NQ>& lds, unsigned* const* qcounters, unsi
             lds.ext_base = n ? atomicAdd(ext_counter, n) : 0u;
             lds.n_cont = run;
         } else {
-            grant_items(lds, items, seg, run);
+            grant_items(lds, items, seg, nseg, run);
         }
     }
     __syncthreads();
@@ -171,25 +171,26 @@ SOL_DEV void write_hole(const WaveBuf& b, int nxt, unsigned entry) {
     *b.ext(nxt, entry, 1) = make_float4(0.0f, 0.0f, 0.0f, as_f(kHole));
 }
 
-// The segment this workgroup appends to (grids are multiples of kNumSegments).
-SOL_DEV int my_segment() { return static_cast<int>(blockIdx.x % kNumSegments); }
+// The segment this workgroup appends to (grids are multiples of kMaxSegments, so of nseg).
+SOL_DEV int my_segment(const WaveBuf& b) { return static_cast<int>(blockIdx.x & (b.nseg - 1)); }
 
-// A queue read as the concatenation of its segments: entry(i) is the storage index of the i-th
-// entry.
+// A queue read as the concatenation of its nseg segments: entry(i) is the storage index of the
+// i-th entry. Segments past nseg count as empty.
 struct SegQueue {
-    unsigned pre[kNumSegments + 1];
+    unsigned pre[kMaxSegments + 1];
     unsigned cap;
 
-    SOL_DEV SegQueue(const unsigned* counts, int queue, unsigned capacity) : cap(capacity) {
+    SOL_DEV SegQueue(const WaveBuf& b, int queue, unsigned capacity) : cap(capacity) {
         pre[0] = 0;
 #pragma unroll
-        for (int k = 0; k < kNumSegments; ++k) pre[k + 1] = pre[k] + counts[qcount(queue, k)];
+        for (int k = 0; k < kMaxSegments; ++k)
+            pre[k + 1] = pre[k] + (k < static_cast<int>(b.nseg) ? b.counts[qcount(queue, k)] : 0u);
     }
-    SOL_DEV unsigned size() const { return pre[kNumSegments]; }
+    SOL_DEV unsigned size() const { return pre[kMaxSegments]; }
     SOL_DEV unsigned entry(unsigned i) const {
         unsigned seg = 0, start = 0;
 #pragma unroll
-        for (int k = 1; k < kNumSegments; ++k) {
+        for (int k = 1; k < kMaxSegments; ++k) {
             bool ge = i >= pre[k];
             seg = ge ? static_cast<unsigned>(k) : seg;
             start = ge ? pre[k] : start;
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                                                  int cur, TraversalScratch sc) {
     __shared__ float4 lds_nodes[kTopLdsFloat4<WIDE>];
     __shared__ int lds_stack[LDS * BLOCK];
-    const SegQueue eqv(b.counts, kQExtend0 + cur, b.cap_e), sqv(b.counts, kQShadow, b.cap_s);
+    const SegQueue eqv(b, kQExtend0 + cur, b.cap_e), sqv(b, kQShadow, b.cap_s);
     const unsigned n_ext = eqv.size();
     const unsigned n_shadow = sqv.size();
     const unsigned total = n_ext + n_shadow;
@@ -497,10 +498,10 @@ template <bool MULTI>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sort(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                  int cur) {
     __shared__ RoundLds<kNumShadeQueues> lds[2];
-    const SegQueue eqv(b.counts, kQExtend0 + cur, b.cap_e);
+    const SegQueue eqv(b, kQExtend0 + cur, b.cap_e);
     const unsigned count = eqv.size();
     const int nxt = 1 - cur;
-    const int seg = my_segment();
+    const int seg = my_segment(b), nseg = static_cast<int>(b.nseg);
     unsigned* qcnt[kNumShadeQueues] = {&b.counts[qcount(kQShade0, seg)], &b.counts[qcount(kQShade0 + 1, seg)],
                                        &b.counts[qcount(kQShade0 + 2, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
@@ -553,12 +554,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                     rec1 = f4(sf.n, as_f(slot));
                     rec2 = f4(sf.face_n, e0.w);   // depth
                     rec3 = f4(dir, as_f(ea));     // where the path state is
-                    dest = queue_of(sf.bsdf);
+                    dest = b.merged ? 0 : queue_of(sf.bsdf);
                 }
             }
             if (dead) retire(b, *b.rec(slot), item_of(*b.ext(cur, ea, 3)));
         }
-        RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, dest, false, dead);
+        RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, nseg, dest, false, dead);
         if (plan.pos >= 0) {
             const unsigned a = seg * b.cap_m + static_cast<unsigned>(plan.pos);
             *b.shade_entry(dest, a, 0) = rec0;
@@ -581,15 +582,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 // One kernel per queue; the material type of the queue is a compile-time constant (glass and
 // mirror share the specular queue), so each kernel carries only its BSDF's code. The glossy
 // kernel (Phong and the diffuse-Phong mix) is held to three waves per SIMD (at most 168 VGPRs).
+// QUEUE == kQueueAll: the merged queue (stored as queue 0), every BSDF chosen at run time, held
+// to three waves per SIMD like the glossy kernel.
 template <int QUEUE, bool MULTI>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE == kQueuePhong ? 3 : 1))) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE == kQueuePhong || QUEUE == kQueueAll ? 3 : 1))) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                   int nxt) {
+    constexpr int SQ = QUEUE == kQueueAll ? 0 : QUEUE;   // storage queue
     __shared__ RoundLds<1> lds[2];
     // after the first sort of a chunk, extension queue 0 no longer holds its start fill
-    if (QUEUE == 0 && blockIdx.x == 0 && threadIdx.x == 0) b.counts[kCntStartFill] = 0;
-    const SegQueue mqv(b.counts, kQShade0 + QUEUE, b.cap_m);
+    if (SQ == 0 && blockIdx.x == 0 && threadIdx.x == 0) b.counts[kCntStartFill] = 0;
+    const SegQueue mqv(b, kQShade0 + SQ, b.cap_m);
     const unsigned count = mqv.size();
-    const int seg = my_segment();
+    const int seg = my_segment(b), nseg = static_cast<int>(b.nseg);
     unsigned* scnt[1] = {&b.counts[qcount(kQShadow, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
     unsigned started = 0, parity = 0;
@@ -604,8 +608,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
         unsigned long long item = 0, rng_next = 0;
         if (valid) {
             const unsigned a = mqv.entry(i);
-            float4 r0 = *b.shade_entry(QUEUE, a, 0), r1 = *b.shade_entry(QUEUE, a, 1), r2 = *b.shade_entry(QUEUE, a, 2),
-                   r3 = *b.shade_entry(QUEUE, a, 3);
+            float4 r0 = *b.shade_entry(SQ, a, 0), r1 = *b.shade_entry(SQ, a, 1), r2 = *b.shade_entry(SQ, a, 2),
+                   r3 = *b.shade_entry(SQ, a, 3);
             // the path state stays in the extension entry the sort read (queue 1 - nxt)
             const uint32_t depth0 = as_u(r2.w), at = as_u(r3.w);
             const float4 r4 = ext_throughput(b, 1 - nxt, at, depth0), r5 = *b.ext(1 - nxt, at, 3);
@@ -631,7 +635,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
             Pcg rng;
             rng.state = rng_of(r5);
             uint32_t depth = depth0;
-            bool skip_nee = QUEUE == kQueueSpecular || (cfg.flags & SOL_PT_DISABLE_NEE);
+            bool skip_nee = QUEUE == kQueueSpecular || (QUEUE == kQueueAll && is_specular(m.type)) ||
+                            (cfg.flags & SOL_PT_DISABLE_NEE);
             if (!skip_nee) {
                 NeeSample ns;
                 shoot = nee_sample(s, cfg.flags, sf, m, out, T, rng, 1.0f / static_cast<float>(s.num_lights), ns);
@@ -653,7 +658,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
             if (!cont) retire(b, *b.rec(slot), item);
         }
         bool retired = valid && !cont;
-        RoundPlan plan = plan_round<1>(lds[parity], scnt, ecnt, b.items, seg, shoot ? 0 : -1, cont, retired);
+        RoundPlan plan = plan_round<1>(lds[parity], scnt, ecnt, b.items, seg, nseg, shoot ? 0 : -1, cont, retired);
         if (plan.pos >= 0) {
             const unsigned a = seg * b.cap_s + static_cast<unsigned>(plan.pos);
             *b.shadow(a, 0) = f4(point, as_f(cont ? slot : kSampleTarget));
@@ -678,16 +683,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
 // ---- start / reset / reduce ------------------------------------------------------------------------
 
 // The start fill lays the slots out as the rounds of the other passes would: slot s is in round
-// s / kBlock, round r in segment r % kNumSegments, and a slot's place in its segment follows the
-// rounds in order. Every slot gets one extension entry (a path, or a hole if no item is left).
-SOL_HD unsigned start_segment(unsigned slot) { return (slot / kBlock) % kNumSegments; }
-SOL_HD unsigned start_position(unsigned slot) {
-    return (slot / kBlock / kNumSegments) * kBlock + slot % kBlock;
+// s / kBlock, round r in segment r % nseg, and a slot's place in its segment follows the rounds in
+// order. Every slot gets one extension entry (a path, or a hole if no item is left).
+SOL_HD unsigned start_segment(unsigned slot, unsigned nseg) { return (slot / kBlock) % nseg; }
+SOL_HD unsigned start_position(unsigned slot, unsigned nseg) {
+    return (slot / kBlock / nseg) * kBlock + slot % kBlock;
 }
-SOL_HD unsigned start_slots(unsigned slots, unsigned seg) {   // slots of segment seg
+SOL_HD unsigned start_slots(unsigned slots, unsigned seg, unsigned nseg) {   // slots of segment seg
     const unsigned full = slots / kBlock, rem = slots % kBlock;
-    unsigned c = full > seg ? ((full - 1 - seg) / kNumSegments + 1) * kBlock : 0u;
-    if (rem && full % kNumSegments == seg) c += rem;
+    unsigned c = full > seg ? ((full - 1 - seg) / nseg + 1) * kBlock : 0u;
+    if (rem && full % nseg == seg) c += rem;
     return c;
 }
 
@@ -703,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
     for (unsigned base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < b.slots; base += wave_step) {
         const unsigned slot = base + __lane_id();
         if (slot < b.slots) {
-            const unsigned seg = start_segment(slot), pos = start_position(slot);
+            const unsigned seg = start_segment(slot, b.nseg), pos = start_position(slot, b.nseg);
             const unsigned long long item = b.items[item_next(static_cast<int>(seg)) + 1] + pos;
             const unsigned entry = seg * b.cap_e + pos;
             if (item < b.items[item_next(static_cast<int>(seg))]) {
@@ -724,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
 // shade(k) append to (kernel, not a memset: it is captured in graphs).
 __global__ void k_reset(WaveBuf b, int cur) {
     int i = threadIdx.x;
-    if (i < kNumWaveCounters && i / kNumSegments != kQExtend0 + cur) b.counts[i * kCounterStride] = 0;
+    if (i < kNumWaveCounters && i / kMaxSegments != kQExtend0 + cur) b.counts[i * kCounterStride] = 0;
 }
 
 // Work items of a chunk [begin, end) by segment, consecutive ranges in segment order: segment g
@@ -735,18 +740,19 @@ __global__ void k_set_chunk(WaveBuf b, unsigned long long begin, unsigned long l
     if (threadIdx.x < kNumWaveCounters) b.counts[threadIdx.x * kCounterStride] = 0;
     __syncthreads();
     if (threadIdx.x != 0) return;
-    unsigned long long left = end - begin, filled[kNumSegments];
-    for (int g = 0; g < kNumSegments; ++g) {
-        filled[g] = min(static_cast<unsigned long long>(start_slots(b.slots, g)), left);
+    const unsigned nseg = b.nseg;
+    unsigned long long left = end - begin, filled[kMaxSegments];
+    for (unsigned g = 0; g < nseg; ++g) {
+        filled[g] = min(static_cast<unsigned long long>(start_slots(b.slots, g, nseg)), left);
         left -= filled[g];
     }
     unsigned long long base = begin;
-    for (int g = 0; g < kNumSegments; ++g) {
-        const unsigned long long extra = left * (g + 1) / kNumSegments - left * g / kNumSegments;
+    for (unsigned g = 0; g < nseg; ++g) {
+        const unsigned long long extra = left * (g + 1) / nseg - left * g / nseg;
         b.items[item_next(g) + 1] = base;
         b.items[item_next(g)] = base + filled[g];
         b.items[item_end(g)] = base + filled[g] + extra;
-        b.counts[qcount(kQExtend0 + nxt, g)] = start_slots(b.slots, g);
+        b.counts[qcount(kQExtend0 + nxt, g)] = start_slots(b.slots, g, nseg);
         base += filled[g] + extra;
     }
     b.items[kItemChunkBegin] = begin;
@@ -818,9 +824,9 @@ hipError_t resident_grid(K kernel, int compute_units, int cap, int& grid) {
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel),
                                                                kBlock, 0);
     if (e != hipSuccess) return e;
-    // a multiple of kNumSegments: round r goes to workgroup r % grid, whose segment is then r % kNumSegments
-    grid = std::min(cap, compute_units * std::max(1, per_cu)) / kNumSegments * kNumSegments;
-    grid = std::max(grid, kNumSegments);
+    // a multiple of kMaxSegments: round r goes to workgroup r % grid, whose segment is then r % nseg
+    grid = std::min(cap, compute_units * std::max(1, per_cu)) / kMaxSegments * kMaxSegments;
+    grid = std::max(grid, kMaxSegments);
     return hipSuccess;
 }
 } // namespace
@@ -833,11 +839,13 @@ hipError_t wave_set_grids(WaveLaunch& L, int compute_units) {
         if ((e = resident_grid(k_shade<0, true>, compute_units, cap, L.grid_kshade[0])) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<1, true>, compute_units, cap, L.grid_kshade[1])) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<2, true>, compute_units, cap, L.grid_kshade[2])) != hipSuccess) return e;
+        if ((e = resident_grid(k_shade<kQueueAll, true>, compute_units, cap, L.grid_kshade[3])) != hipSuccess) return e;
     } else {
         if ((e = resident_grid(k_sort<false>, compute_units, cap, L.grid_sort)) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<0, false>, compute_units, cap, L.grid_kshade[0])) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<1, false>, compute_units, cap, L.grid_kshade[1])) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<2, false>, compute_units, cap, L.grid_kshade[2])) != hipSuccess) return e;
+        if ((e = resident_grid(k_shade<kQueueAll, false>, compute_units, cap, L.grid_kshade[3])) != hipSuccess) return e;
     }
     return resident_grid(k_start, compute_units, cap, L.grid_start);
 }
@@ -865,10 +873,14 @@ hipError_t wave_launch_shade(const WaveLaunch& L, int queue, int nxt) {
         if (queue == 0) hipLaunchKernelGGL((k_shade<0, true>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
         if (queue == 1) hipLaunchKernelGGL((k_shade<1, true>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
         if (queue == 2) hipLaunchKernelGGL((k_shade<2, true>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
+        if (queue == kQueueAll)
+            hipLaunchKernelGGL((k_shade<kQueueAll, true>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
     } else {
         if (queue == 0) hipLaunchKernelGGL((k_shade<0, false>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
         if (queue == 1) hipLaunchKernelGGL((k_shade<1, false>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
         if (queue == 2) hipLaunchKernelGGL((k_shade<2, false>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
+        if (queue == kQueueAll)
+            hipLaunchKernelGGL((k_shade<kQueueAll, false>), g, bs, 0, L.stream, L.single, L.tabs, L.buf, L.cfg, nxt);
     }
     return hipGetLastError();
 }

@@ -17,7 +17,7 @@ struct WaveLaunch {
     int grid_traverse;
     int grid_shade;           // workgroups of k_sort, k_shade, k_start before wave_set_grids
     int grid_sort;            // resident workgroups of each kernel (set by wave_set_grids): these
-    int grid_kshade[3];       // kernels hand out rounds by workgroup index, so every workgroup
+    int grid_kshade[4];       // kernels hand out rounds by workgroup index, so every workgroup
     int grid_start;           // of the grid must be resident at once
     bool multi;
     bool wide;                // every scene has the 4-wide layout: k_trace traverses it
@@ -30,6 +30,7 @@ hipError_t wave_set_grids(WaveLaunch& L, int compute_units);
 
 hipError_t wave_launch_trace(const WaveLaunch& L, int cur);
 hipError_t wave_launch_sort(const WaveLaunch& L, int cur);
+// queue: kQueueDiffuse, kQueuePhong, kQueueSpecular, or kQueueAll (the merged queue)
 hipError_t wave_launch_shade(const WaveLaunch& L, int queue, int nxt);
 hipError_t wave_launch_start(const WaveLaunch& L, int nxt);
 hipError_t wave_launch_reset(const WaveLaunch& L, int cur);

@@ -56,41 +56,59 @@ struct WavefrontState {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
     }
 
-    // Grows the per-slot arena to at least n slots (never shrinks it).
-    int ensure_slots(uint32_t n) {
-        if (arena.ptr && n <= slots) return SOLHIP_OK;
-        // Segment capacities (see WaveBuf): extension queues slots / 4 + 8 R, shade queues
-        // slots / 8 + 2 R, shadow queue slots / 8 + 6 R, R = kWaveBlock, rounded up to 64 entries.
+    // Segment capacities (see WaveBuf) of n slots in nseg segments: extension queues
+    // 2 n / nseg + 8 R, shade queues n / nseg + 2 R, shadow queue n / nseg + 6 R, R = kWaveBlock,
+    // rounded up to 64 entries. The bounds hold for the material-split queues (three shade
+    // launches) and so for the merged queue (one).
+    struct Layout {
+        size_t cap_e, cap_m, cap_s;
+        // slot record and hit 16 B each per slot; queue entries 64 B (extension), 64 B (shade),
+        // 48 B (shadow) per segment entry
+        size_t float4s(size_t n, size_t nseg) const {
+            return n * (1 + 1) + nseg * (2 * cap_e * 4 + kNumShadeQueues * cap_m * 4 + cap_s * 3);
+        }
+    };
+    static Layout layout(size_t n, size_t nseg) {
         const size_t R = kWaveBlock;
         auto round64 = [](size_t v) { return (v + 63) & ~size_t(63); };
-        const size_t cap_e = round64(n / 4 + 8 * R), cap_m = round64(n / 8 + 2 * R), cap_s = round64(n / 8 + 6 * R);
-        // slot record 32 B and hit 16 B per slot; queue entries 64 B (extension), 96 B (shade),
-        // 48 B (shadow) per segment entry
-        size_t f4_total = size_t(n) * (1 + 1) + kNumSegments * (2 * cap_e * 4 + kNumShadeQueues * cap_m * 4 + cap_s * 3);
-        size_t bytes = sizeof(float4) * f4_total + 16 * 4096;
-        HIP_OK(arena.alloc(bytes));
-        HIP_OK(counters.alloc(sizeof(unsigned) * kCounterWords + align_up(sizeof(unsigned long long) * kItemWords) + 256));
+        return {round64(2 * n / nseg + 8 * R), round64(n / nseg + 2 * R), round64(n / nseg + 6 * R)};
+    }
+
+    // Grows the per-slot arena to at least n slots (never shrinks it; sized for every segment
+    // count) and lays the queues out in it for nseg segments and the shading queue split.
+    int ensure_slots(uint32_t n, uint32_t nseg, bool merged) {
+        if (!arena.ptr || n > slots) {
+            size_t f4_total = 0;
+            for (size_t g = 1; g <= kMaxSegments; g *= 2) f4_total = std::max(f4_total, layout(n, g).float4s(n, g));
+            HIP_OK(arena.alloc(sizeof(float4) * f4_total + 16 * 4096));
+            HIP_OK(counters.alloc(sizeof(unsigned) * kCounterWords + align_up(sizeof(unsigned long long) * kItemWords) + 256));
+            slots = n;
+            buf.nseg = 0;   // lay out again
+        }
+        WaveBuf& b = buf;
+        if (b.nseg == nseg && (b.merged != 0) == merged) return SOLHIP_OK;
+        const Layout lay = layout(slots, nseg);
         char* p = arena.as<char>();
         auto f4s = [&](size_t count) { float4* r = reinterpret_cast<float4*>(p); p += align_up(sizeof(float4) * count); return r; };
-        WaveBuf& b = buf;
-        b.slots = n;
-        b.cap_e = static_cast<uint32_t>(cap_e);
-        b.cap_m = static_cast<uint32_t>(cap_m);
-        b.cap_s = static_cast<uint32_t>(cap_s);
-        b.stride_e = static_cast<uint32_t>(kNumSegments * cap_e);
-        b.stride_m = static_cast<uint32_t>(kNumSegments * cap_m);
-        b.stride_s = static_cast<uint32_t>(kNumSegments * cap_s);
-        b.path = f4s(size_t(n));
-        b.hits = f4s(n);
-        for (int q = 0; q < 2; ++q) b.eq[q] = f4s(kNumSegments * cap_e * 4);
-        for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = f4s(kNumSegments * cap_m * 4);
-        b.sq = f4s(kNumSegments * cap_s * 3);
+        b.slots = slots;
+        b.nseg = nseg;
+        b.merged = merged ? 1u : 0u;
+        b.cap_e = static_cast<uint32_t>(lay.cap_e);
+        b.cap_m = static_cast<uint32_t>(lay.cap_m);
+        b.cap_s = static_cast<uint32_t>(lay.cap_s);
+        b.stride_e = static_cast<uint32_t>(nseg * lay.cap_e);
+        b.stride_m = static_cast<uint32_t>(nseg * lay.cap_m);
+        b.stride_s = static_cast<uint32_t>(nseg * lay.cap_s);
+        b.path = f4s(slots);
+        b.hits = f4s(slots);
+        for (int q = 0; q < 2; ++q) b.eq[q] = f4s(nseg * lay.cap_e * 4);
+        for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = f4s(nseg * lay.cap_m * 4);
+        b.sq = f4s(nseg * lay.cap_s * 3);
         char* c = counters.as<char>();
         b.counts = reinterpret_cast<unsigned*>(c);
         b.items = reinterpret_cast<unsigned long long*>(c + sizeof(unsigned) * kCounterWords);
         b.stats = reinterpret_cast<unsigned long long*>(c + sizeof(unsigned) * kCounterWords +
                                                         align_up(sizeof(unsigned long long) * kItemWords));
-        slots = n;
         ++generation;
         return SOLHIP_OK;
     }
@@ -134,8 +152,8 @@ void Context::release_wavefront() {
 namespace {
 
 // One bounce of every path in flight: trace (closest hits of queue `cur`, any hits of the shadow
-// queue), reset, sort, shade (three queues). Paths that end in sort or shade start their slot's
-// next work item there.
+// queue), reset, sort, shade (three material queues, or the merged one). Paths that end in sort or
+// shade start their slot's next work item there.
 hipError_t enqueue_iteration(const WaveLaunch& L, int cur, hipEvent_t* ev) {
     int nxt = 1 - cur;
     hipError_t e;
@@ -145,8 +163,12 @@ hipError_t enqueue_iteration(const WaveLaunch& L, int cur, hipEvent_t* ev) {
     if ((e = wave_launch_reset(L, cur)) != hipSuccess) return e;
     if ((e = wave_launch_sort(L, cur)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[2], L.stream)) != hipSuccess) return e;
-    for (int q = 0; q < kNumShadeQueues; ++q)
-        if ((e = wave_launch_shade(L, q, nxt)) != hipSuccess) return e;
+    if (L.buf.merged) {
+        if ((e = wave_launch_shade(L, kQueueAll, nxt)) != hipSuccess) return e;
+    } else {
+        for (int q = 0; q < kNumShadeQueues; ++q)
+            if ((e = wave_launch_shade(L, q, nxt)) != hipSuccess) return e;
+    }
     if (ev && (e = hipEventRecord(ev[3], L.stream)) != hipSuccess) return e;
     return hipSuccess;
 }
@@ -156,9 +178,10 @@ std::string graph_key(const WaveLaunch& L, const WavefrontState& st, int iters) 
     k.append(reinterpret_cast<const char*>(&L.single), sizeof(L.single));
     k.append(reinterpret_cast<const char*>(&L.tabs), sizeof(L.tabs));
     k.append(reinterpret_cast<const char*>(&L.buf), sizeof(L.buf));
-    int meta[12] = {L.block,          L.lds_stack,      L.grid_traverse,  L.grid_shade,
+    int meta[13] = {L.block,          L.lds_stack,      L.grid_traverse,  L.grid_shade,
                     L.grid_sort,      L.grid_kshade[0], L.grid_kshade[1], L.grid_kshade[2],
-                    L.grid_start,     L.multi ? 1 : 0,  L.wide ? 1 : 0,   iters};
+                    L.grid_kshade[3], L.grid_start,     L.multi ? 1 : 0,  L.wide ? 1 : 0,
+                    iters};
     k.append(reinterpret_cast<const char*>(meta), sizeof(meta));
     k.append(reinterpret_cast<const char*>(&st.generation), sizeof(st.generation));
     k.append(reinterpret_cast<const char*>(&L.scratch), sizeof(L.scratch));
@@ -296,7 +319,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     // no more slots than the largest chunk has items: a small render does not walk idle slots
     const uint32_t want_slots =
         static_cast<uint32_t>(std::min<unsigned long long>(tu.path_slots, std::max<unsigned long long>(256, (biggest + 255) & ~255ull)));
-    if ((rc = st.ensure_slots(want_slots))) return rc;
+    if ((rc = st.ensure_slots(want_slots, tu.queue_segments, tu.shade_queues == 1))) return rc;
 
     const bool multi = views.size() > 1;
     WaveLaunch L{};
@@ -404,7 +427,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
                 }
             }
             unsigned pending = 0;
-            for (int k = 0; k < kNumSegments; ++k)
+            for (int k = 0; k < kMaxSegments; ++k)
                 pending += counts[qcount(kQExtend0, k)] + counts[qcount(kQShadow, k)];
             if (pending == 0) break;
         }
@@ -484,8 +507,8 @@ namespace {
 
 // Coordinate descent over the wavefront launch parameters from the context's current ones: paths in
 // flight (candidates above `items`, the work of one trial, only add idle slots and are left out),
-// traversal workgroup size, LDS stack depth, bounce iterations per graph, BVH node width and tile
-// shape. `trial` renders with ctx.tuning and returns its duration in ms (the first call of a
+// traversal workgroup size, LDS stack depth, bounce iterations per graph, BVH node width, the queue
+// split (segments per queue, material-split or merged shading queues) and tile shape. `trial` renders with ctx.tuning and returns its duration in ms (the first call of a
 // candidate builds the slot arena and captures the graph, so each candidate runs twice and the
 // second run counts). A candidate replaces the current best if it is at least 1 % faster. The
 // statistics and the profiling switch of the context are restored; ctx.tuning ends as the best.
@@ -511,7 +534,7 @@ int tune(Context& ctx, unsigned long long items, Trial trial, solhip_tuning* out
     solhip_tuning best = start;
     float best_ms = 0.0f;
     int rc = timed(best, best_ms);
-    for (int dim = 0; rc == SOLHIP_OK && dim < 6; ++dim) {
+    for (int dim = 0; rc == SOLHIP_OK && dim < 8; ++dim) {
         std::vector<solhip_tuning> cands;
         solhip_tuning t = best;
         switch (dim) {
@@ -520,6 +543,8 @@ int tune(Context& ctx, unsigned long long items, Trial trial, solhip_tuning* out
             case 2: for (uint32_t v : {8u, 16u}) { t.lds_stack = v; cands.push_back(t); } break;
             case 3: for (uint32_t v : {8u, 16u, 32u}) { t.graph_iters = v; cands.push_back(t); } break;
             case 4: for (uint32_t v : {2u, 4u}) { t.bvh_width = v; cands.push_back(t); } break;
+            case 5: for (uint32_t v : {1u, 2u, 4u, 8u}) { t.queue_segments = v; cands.push_back(t); } break;
+            case 6: for (uint32_t v : {1u, 3u}) { t.shade_queues = v; cands.push_back(t); } break;
             default:
                 for (auto wh : {std::pair<uint32_t, uint32_t>{16, 16}, {32, 32}, {64, 16}, {128, 8}}) {
                     t.tile_w = wh.first;

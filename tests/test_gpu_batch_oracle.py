@@ -4,7 +4,8 @@ A batch runs many scenes and images in one wavefront run: tiles of every job fee
 queue, paths of different scenes share the queues and the sample buffer, and per-pixel sums are
 reduced chunk by chunk. None of that may change a pixel: each job must equal the oracle's
 PathTracer::render of its own scene, image and sample range (src/algorithms/path_tracer.cpp:9-23),
-whatever the tile shape, the paths in flight or the chunking. The full 64-scene batched workload
+whatever the tile shape, the paths in flight, the chunking or the queue split (segments per queue,
+material-split or merged shading queues). The full 64-scene batched workload
 is run as well, with properties that hold at any size.
 """
 import os
@@ -50,8 +51,13 @@ JOBS = [
 SHARED = {5: 0}
 
 
-@pytest.mark.parametrize("slots,chunk,tiles", [(5000, 65536, (16, 8)), (1 << 20, 1 << 20, (32, 32))])
-def test_batch_matches_oracle_per_job(ctx, slots, chunk, tiles):
+@pytest.mark.parametrize("slots,chunk,tiles,segments,shade_queues", [
+    (5000, 65536, (16, 8), 8, 3),
+    (1 << 20, 1 << 20, (32, 32), 8, 3),
+    (5000, 65536, (16, 8), 1, 1),
+    (1 << 20, 1 << 20, (32, 32), 2, 1),
+])
+def test_batch_matches_oracle_per_job(ctx, slots, chunk, tiles, segments, shade_queues):
     scenes = _scenes()
     gs = {k: ctx.upload(s) for k, s in scenes.items()}
     cfg = sol_amd.PathTracerConfig(max_path_len=BATCH_DEPTH)
@@ -66,14 +72,17 @@ def test_batch_matches_oracle_per_job(ctx, slots, chunk, tiles):
         jobs.append((gs[name], w, h, si, sc, buf.data_ptr()))
     torch.cuda.synchronize()
     total = sum(w * h * sc for _, w, h, _, sc in JOBS)
-    print(f"batch of {len(JOBS)} jobs, {total} samples, slots {slots}, chunk {chunk}, tiles {tiles}", flush=True)
-    ctx.set_tuning(path_slots=slots, chunk_items=chunk, tile_w=tiles[0], tile_h=tiles[1], graph_iters=4)
+    print(f"batch of {len(JOBS)} jobs, {total} samples, slots {slots}, chunk {chunk}, tiles {tiles}, "
+          f"{segments} segments, {shade_queues} shade queues", flush=True)
+    ctx.set_tuning(path_slots=slots, chunk_items=chunk, tile_w=tiles[0], tile_h=tiles[1], graph_iters=4,
+                   queue_segments=segments, shade_queues=shade_queues)
     try:
         ctx.reset_stats()
         ctx.render_batch(jobs, cfg)
         st = ctx.stats()
     finally:
-        ctx.set_tuning(path_slots=0, chunk_items=0, tile_w=0, tile_h=0, graph_iters=0)
+        ctx.set_tuning(path_slots=0, chunk_items=0, tile_w=0, tile_h=0, graph_iters=0, queue_segments=0,
+                       shade_queues=0)
     assert st.samples == total
     if chunk == 65536:
         assert total > 2 * chunk                 # at least three chunks of work
