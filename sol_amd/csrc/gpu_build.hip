@@ -2,32 +2,42 @@
 // (src/triangle_mesh.cpp:119-146); this builder produces the same kind of tree — binary, leaves of
 // up to 8 triangles, SAH-driven — in the layouts the traversal kernels read, without leaving HBM:
 //
-//   1. triangle boxes, centroid bounds, 63-bit Morton codes of the centroids, radix sort;
+//   1. triangle boxes, centroid bounds, 63-bit Morton codes of the centroids, stable radix sort
+//      (build_prims.h);
 //   2. PLOC (Meister and Bittner, "Parallel Locally-Ordered Clustering for Bounding Volume
 //      Hierarchy Construction", TVCG 2018): every cluster finds the cluster within `radius`
 //      positions of the Morton order whose merged box has the smallest surface area; mutual
 //      nearest neighbours merge; survivors are compacted in order; repeat until one cluster.
 //      Ties go to the smaller index, so the globally closest pair is always mutual and every pass
 //      merges at least once; the result is deterministic;
-//   3. bottom-up SAH pass (one thread per triangle climbing to the root, the second thread to
-//      reach a node finishes it): a subtree of at most max_leaf_size triangles becomes a leaf when
-//      its leaf cost (intersection cost x triangles x area) is no higher than its split cost;
+//   3. the SAH collapse, computed as each node is made (both its children were made by earlier
+//      passes): a subtree of at most max_leaf_size triangles becomes a leaf when its leaf cost
+//      (intersection cost x triangles x area) is no higher than its split cost, and the costs of
+//      the 4-wide collapse are prepared;
 //   4. breadth-first layouts, one pass per level with a prefix sum: the binary nodes
 //      (sol_gpu_node), the leaf ranges of the triangles (leaf order), then the 4-wide nodes
 //      (sol_gpu_node4: a binary node's children, opening the largest-area inner child until four,
 //      as the host layout does); the top levels come first, as the LDS staging of the traversal
 //      kernels needs;
 //   5. triangles and shading records gathered into leaf order.
+//
+// The cluster count of a clustering pass and the node count of a layout level stay in device
+// memory: each launch reads them there, with a grid sized for a bound the host knows (the last
+// count it read; a level has at most two, or four, times the nodes of the one above it). The host
+// reads the counts back every kPassesPerSync passes and kLevelsPerSync levels; launches past the
+// end find nothing to do. Temporaries live in one workspace held by the context (grown, never
+// shrunk), so a build allocates only the scene's own arrays.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
+#include "build_prims.h"
 #include "solhip.h"
 #include "state.h"
 #include "traverse.h"
@@ -35,7 +45,6 @@
 namespace solhip {
 
 namespace {
-
 #define BUILD_TRY(expr)                                                                          \
     do {                                                                                         \
         hipError_t e__ = (expr);                                                                 \
@@ -49,6 +58,8 @@ constexpr int kBB = 256;          // threads per block of the build kernels
 constexpr int kMaxRadius = 32;
 constexpr uint32_t kTopLevels = 6;    // binary levels laid out first (63 nodes at most)
 constexpr uint32_t kTopLevels4 = 4;   // 4-wide levels laid out first (85 nodes at most)
+constexpr int kPassesPerSync = 8;     // clustering passes launched between count read-backs
+constexpr int kLevelsPerSync = 4;     // layout levels launched between count read-backs
 
 unsigned grid_for(size_t n) { return static_cast<unsigned>((n + kBB - 1) / kBB); }
 
@@ -72,8 +83,11 @@ SOL_HD float half_area(float4 lo, float4 hi) {
 SOL_DEV V3 vertex(const float* P, uint32_t i) { return V3{P[3 * i], P[3 * i + 1], P[3 * i + 2]}; }
 
 // Node boxes: nodes [0, n) are the triangles (original index), [n, 2n - 1) the clusters made by
-// the merges, in the order they are made.
-__global__ void k_tri_boxes(const float* P, const uint32_t* I, uint32_t n, float4* lo, float4* hi, unsigned* cb) {
+// the merges, in the order they are made. A triangle is a leaf of one triangle for the SAH
+// collapse (count 1, cost C_i x area, no 4-wide cost).
+__global__ void k_tri_boxes(const float* P, const uint32_t* I, uint32_t n, float4* lo, float4* hi, unsigned* cb,
+                            float ci, unsigned* count, float* cost, unsigned* collapse, float* wcost,
+                            unsigned* wsplit) {
     __shared__ unsigned s[6];
     if (threadIdx.x < 6) s[threadIdx.x] = (threadIdx.x < 3) ? 0xFFFFFFFFu : 0u;
     __syncthreads();
@@ -84,6 +98,11 @@ __global__ void k_tri_boxes(const float* P, const uint32_t* I, uint32_t n, float
         float4 h = make_float4(fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y), fmaxf(fmaxf(a.z, b.z), c.z), 0.0f);
         lo[t] = l;
         hi[t] = h;
+        count[t] = 1;
+        cost[t] = ci * half_area(l, h);
+        collapse[t] = 1;
+        reinterpret_cast<float4*>(wcost)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        wsplit[t] = 0;
         float cx = (l.x + h.x) * 0.5f, cy = (l.y + h.y) * 0.5f, cz = (l.z + h.z) * 0.5f;
         atomicMin(&s[0], f2o(cx));
         atomicMin(&s[1], f2o(cy));
@@ -130,11 +149,26 @@ __global__ void k_morton(const float4* lo, const float4* hi, uint32_t n, const u
     ids[t] = t;
 }
 
+// State of the clustering, double-buffered by pass parity: a pass reads one copy and its last
+// cluster writes the other.
+struct PlocState {
+    uint32_t m;           // clusters
+    uint32_t next_node;   // index of the next node to make
+    uint32_t passes;      // passes that merged
+    uint32_t stalled;     // a pass merged nothing (cannot happen with a consistent order)
+};
+
+__global__ void k_ploc_init(PlocState* st, uint32_t n) {
+    if (threadIdx.x == 0) st[0] = PlocState{n, n, 0u, 0u};
+}
+
 // Nearest neighbour of every cluster within `r` positions (merged half area; ties to the smaller
 // index). The boxes of the block's window are staged in LDS.
-__global__ __launch_bounds__(kBB) void k_nearest(const int* C, uint32_t m, const float4* nlo, const float4* nhi, int r,
-                                                 int* nn) {
+__global__ __launch_bounds__(kBB) void k_nearest(const int* C, const PlocState* st, uint32_t bound, const float4* nlo,
+                                                 const float4* nhi, int r, int* nn) {
     __shared__ float4 slo[kBB + 2 * kMaxRadius], shi[kBB + 2 * kMaxRadius];
+    const uint32_t m = min(st->m, bound);
+    if (m <= 1 || blockIdx.x * kBB >= m) return;   // uniform over the workgroup
     const int base = static_cast<int>(blockIdx.x) * kBB - r;
     for (int k = threadIdx.x; k < kBB + 2 * r; k += kBB) {
         int j = base + k;
@@ -168,9 +202,10 @@ __global__ __launch_bounds__(kBB) void k_nearest(const int* C, uint32_t m, const
 
 // Flags packed for one 64-bit prefix sum: high word 1 if cluster i makes a node (lower index of a
 // mutual pair), low word 1 if cluster i survives (it is not the higher index of a mutual pair).
-__global__ void k_merge_flags(const int* nn, uint32_t m, unsigned long long* flags) {
+__global__ void k_merge_flags(const int* nn, const PlocState* st, uint32_t bound, unsigned long long* flags) {
+    const uint32_t m = min(st->m, bound);
     uint32_t i = blockIdx.x * kBB + threadIdx.x;
-    if (i >= m) return;
+    if (m <= 1 || i >= m) return;
     int j = nn[i];
     bool mutual = j >= 0 && nn[j] == static_cast<int>(i);
     bool make = mutual && static_cast<int>(i) < j;
@@ -178,104 +213,94 @@ __global__ void k_merge_flags(const int* nn, uint32_t m, unsigned long long* fla
     flags[i] = (static_cast<unsigned long long>(make) << 32) | static_cast<unsigned long long>(keep);
 }
 
+// SAH collapse of a node made from children a and b (both final): count and cost of the subtree,
+// collapse[node] = 1 if it becomes a leaf.
+// The same step prepares the 4-wide collapse (the host layout's rule): each inner node becomes a
+// 4-wide node or is opened into its parent's so that the summed half area of the 4-wide nodes is
+// least. wcost[4 x + k - 1] is the least such sum for subtree x in at most k slots of its parent
+// (0 for a leaf; for k = 1 x is a 4-wide node, area(x) + fill(x, 4)); byte k - 2 of wsplit[x] is
+// how many of k slots the left subtree takes in the best fill(x, k).
+SOL_DEV void finish_node(int node, int a, int b, float area, unsigned* count, float* cost, unsigned* collapse,
+                         float* wcost, unsigned* wsplit, float ct, float ci, uint32_t max_leaf) {
+    const unsigned cnt = count[a] + count[b];
+    const float split = ct * area + cost[a] + cost[b];
+    const float as_leaf = ci * static_cast<float>(cnt) * area;
+    const bool leaf = cnt <= max_leaf && as_leaf <= split;
+    count[node] = cnt;
+    cost[node] = leaf ? as_leaf : split;
+    collapse[node] = leaf ? 1u : 0u;
+    float4* wc4 = reinterpret_cast<float4*>(wcost);
+    if (leaf) {
+        wc4[node] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        wsplit[node] = 0;
+        return;
+    }
+    const float4 wa = wc4[a], wb = wc4[b];
+    const float A[4] = {wa.x, wa.y, wa.z, wa.w}, B[4] = {wb.x, wb.y, wb.z, wb.w};
+    float fill[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    unsigned sp = 0;
+    for (int k = 2; k <= 4; ++k) {
+        float best = HUGE_VALF;
+        int bj = 1;
+        for (int j = 1; j < k; ++j) {
+            const float c = A[j - 1] + B[k - j - 1];
+            if (c < best) {
+                best = c;
+                bj = j;
+            }
+        }
+        fill[k] = best;
+        sp |= static_cast<unsigned>(bj) << (8 * (k - 2));
+    }
+    const float wide = area + fill[4];
+    wc4[node] = make_float4(wide, fminf(wide, fill[2]), fminf(wide, fill[3]), fminf(wide, fill[4]));
+    wsplit[node] = sp;
+}
+
+// Makes the nodes of the mutual pairs, finishes their SAH collapse and compacts the surviving
+// clusters in order into C_out. The last cluster writes the next state; a pass that finds one
+// cluster or none passes the state and the cluster array on unchanged.
 __global__ void k_merge(const int* C, const int* nn, const unsigned long long* flags, const unsigned long long* offs,
-                        uint32_t m, uint32_t next_node, float4* nlo, float4* nhi, int* left, int* right, int* parent,
-                        int* C_out, unsigned* totals) {
+                        const PlocState* st, PlocState* st_out, uint32_t bound, float4* nlo, float4* nhi, int* left,
+                        int* right, int* C_out, unsigned* count, float* cost, unsigned* collapse,
+                        float* wcost, unsigned* wsplit, float ct, float ci, uint32_t max_leaf) {
+    const PlocState s = *st;
+    const uint32_t m = min(s.m, bound);
     uint32_t i = blockIdx.x * kBB + threadIdx.x;
+    if (m <= 1) {
+        if (i == 0) {
+            *st_out = s;
+            C_out[0] = C[0];
+        }
+        return;
+    }
     if (i >= m) return;
     const unsigned long long f = flags[i], o = offs[i];
     const bool make = (f >> 32) != 0, keep = (f & 0xFFFFFFFFull) != 0;
     const uint32_t make_off = static_cast<uint32_t>(o >> 32), keep_off = static_cast<uint32_t>(o);
     if (i == m - 1) {
-        totals[0] = make_off + (make ? 1u : 0u);
-        totals[1] = keep_off + (keep ? 1u : 0u);
+        const uint32_t made = make_off + (make ? 1u : 0u), kept = keep_off + (keep ? 1u : 0u);
+        st_out->m = kept;
+        st_out->next_node = s.next_node + made;
+        st_out->passes = s.passes + 1;
+        st_out->stalled = s.stalled | ((made == 0 || kept != m - made) ? 1u : 0u);
     }
     if (!keep) return;
     int out = C[i];
     if (make) {
         const int a = C[i], b = C[nn[i]];
-        const int node = static_cast<int>(next_node + make_off);
+        const int node = static_cast<int>(s.next_node + make_off);
         float4 al = nlo[a], ah = nhi[a], bl = nlo[b], bh = nhi[b];
-        nlo[node] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
-        nhi[node] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+        const float4 lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+        const float4 hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+        nlo[node] = lo;
+        nhi[node] = hi;
         left[node] = a;
         right[node] = b;
-        parent[a] = node;
-        parent[b] = node;
+        finish_node(node, a, b, half_area(lo, hi), count, cost, collapse, wcost, wsplit, ct, ci, max_leaf);
         out = node;
     }
     C_out[keep_off] = out;
-}
-
-// Bottom-up SAH pass: count and cost of every node; collapse[node] = 1 if the subtree becomes a
-// leaf. Triangles (nodes < n) are leaves of one triangle.
-// The same pass prepares the 4-wide collapse (the host layout's rule): each inner node becomes a
-// 4-wide node or is opened into its parent's so that the summed half area of the 4-wide nodes is
-// least. wcost[4 x + k - 1] is the least such sum for subtree x in at most k slots of its parent
-// (0 for a leaf; for k = 1 x is a 4-wide node, area(x) + fill(x, 4)); byte k - 2 of wsplit[x] is
-// how many of k slots the left subtree takes in the best fill(x, k).
-__global__ void k_collapse(uint32_t n, const float4* nlo, const float4* nhi, const int* left, const int* right,
-                           const int* parent, unsigned* visits, unsigned* count, float* cost, unsigned* collapse,
-                           float* wcost, unsigned* wsplit, float ct, float ci, uint32_t max_leaf) {
-    uint32_t t = blockIdx.x * kBB + threadIdx.x;
-    if (t >= n) return;
-    count[t] = 1;
-    cost[t] = ci * half_area(nlo[t], nhi[t]);
-    collapse[t] = 1;
-    for (int k = 0; k < 4; ++k) wcost[4 * size_t(t) + k] = 0.0f;
-    wsplit[t] = 0;
-    __threadfence();
-    int node = parent[t];
-    while (node >= 0) {
-        if (atomicAdd(&visits[node], 1u) == 0) return;   // the other child finishes this node
-        __threadfence();
-        const int a = left[node], b = right[node];
-        const unsigned cnt = __hip_atomic_load(&count[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
-                             __hip_atomic_load(&count[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const float ca = __hip_atomic_load(&cost[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const float cb = __hip_atomic_load(&cost[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const float area = half_area(nlo[node], nhi[node]);
-        const float split = ct * area + ca + cb;
-        const float as_leaf = ci * static_cast<float>(cnt) * area;
-        const bool leaf = cnt <= max_leaf && as_leaf <= split;
-        count[node] = cnt;
-        cost[node] = leaf ? as_leaf : split;
-        collapse[node] = leaf ? 1u : 0u;
-        float* wc = wcost + 4 * size_t(node);
-        if (leaf) {
-            wc[0] = wc[1] = wc[2] = wc[3] = 0.0f;
-            wsplit[node] = 0;
-        } else {
-            float A[4], B[4];
-            for (int k = 0; k < 4; ++k) {
-                A[k] = __hip_atomic_load(&wcost[4 * size_t(a) + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                B[k] = __hip_atomic_load(&wcost[4 * size_t(b) + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            float fill[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            unsigned sp = 0;
-            for (int k = 2; k <= 4; ++k) {
-                float best = HUGE_VALF;
-                int bj = 1;
-                for (int j = 1; j < k; ++j) {
-                    const float c = A[j - 1] + B[k - j - 1];
-                    if (c < best) {
-                        best = c;
-                        bj = j;
-                    }
-                }
-                fill[k] = best;
-                sp |= static_cast<unsigned>(bj) << (8 * (k - 2));
-            }
-            const float wide = area + fill[4];
-            wc[0] = wide;
-            wc[1] = fminf(wide, fill[2]);
-            wc[2] = fminf(wide, fill[3]);
-            wc[3] = fminf(wide, fill[4]);
-            wsplit[node] = sp;
-        }
-        __threadfence();
-        node = parent[node];
-    }
 }
 
 // Leaf reference of a leaf-like node and the triangles of its subtree written at `first`.
@@ -302,9 +327,29 @@ SOL_DEV bool inner_node(int node, uint32_t n, const unsigned* collapse) {
     return node >= static_cast<int>(n) && collapse[node] == 0;
 }
 
+// State of a breadth-first layout, double-buffered by level parity like PlocState.
+struct LevelState {
+    uint32_t size;         // nodes of the current level (0: the layout is done)
+    uint32_t level_base;   // index of its first node
+    uint32_t tri_base;     // binary layout: triangles placed in leaves so far
+    uint32_t level;        // levels laid out
+    uint32_t top_nodes;    // nodes of the first top levels
+    uint32_t pad[3];
+};
+
+// Level 0 is the root (frontier F[0]).
+__global__ void k_level_init(LevelState* st, int* F, int root) {
+    if (threadIdx.x == 0) {
+        st[0] = LevelState{1u, 0u, 0u, 0u, 0u, {0u, 0u, 0u}};
+        F[0] = root;
+    }
+}
+
 // Per node of a binary level: inner children (high word) and leaf triangles (low word).
-__global__ void k_level_count(const int* F, uint32_t size, uint32_t n, const int* left, const int* right,
-                              const unsigned* count, const unsigned* collapse, unsigned long long* packed) {
+__global__ void k_level_count(const int* F, const LevelState* st, uint32_t bound, uint32_t n, const int* left,
+                              const int* right, const unsigned* count, const unsigned* collapse,
+                              unsigned long long* packed) {
+    const uint32_t size = min(st->size, bound);
     uint32_t k = blockIdx.x * kBB + threadIdx.x;
     if (k >= size) return;
     const int node = F[k];
@@ -317,18 +362,29 @@ __global__ void k_level_count(const int* F, uint32_t size, uint32_t n, const int
     packed[k] = (inner << 32) | tris;
 }
 
-__global__ void k_level_write(const int* F, uint32_t size, uint32_t n, uint32_t level_base, uint32_t next_base,
-                              uint32_t tri_base, const unsigned long long* offs, const float4* nlo, const float4* nhi,
-                              const int* left, const int* right, const unsigned* count, const unsigned* collapse,
-                              sol_gpu_node* out, int* F_next, int32_t* leafref, unsigned* order,
-                              unsigned* totals, const unsigned long long* packed) {
+__global__ void k_level_write(const int* F, const LevelState* st, LevelState* st_out, uint32_t bound, uint32_t n,
+                              const unsigned long long* offs, const float4* nlo, const float4* nhi, const int* left,
+                              const int* right, const unsigned* count, const unsigned* collapse, sol_gpu_node* out,
+                              int* F_next, int32_t* leafref, unsigned* order, const unsigned long long* packed) {
+    const LevelState s = *st;
+    const uint32_t size = min(s.size, bound);
     uint32_t k = blockIdx.x * kBB + threadIdx.x;
+    if (size == 0) {
+        if (k == 0) *st_out = s;
+        return;
+    }
     if (k >= size) return;
+    const uint32_t next_base = s.level_base + size;
     const unsigned long long o = offs[k];
     uint32_t inner_off = static_cast<uint32_t>(o >> 32), tri_off = static_cast<uint32_t>(o);
     if (k == size - 1) {
-        totals[0] = inner_off + static_cast<uint32_t>(packed[k] >> 32);
-        totals[1] = tri_off + static_cast<uint32_t>(packed[k]);
+        LevelState t = s;
+        t.size = inner_off + static_cast<uint32_t>(packed[k] >> 32);
+        t.level_base = next_base;
+        t.tri_base = s.tri_base + tri_off + static_cast<uint32_t>(packed[k]);
+        t.level = s.level + 1;
+        t.top_nodes = s.level < kTopLevels ? next_base : s.top_nodes;
+        *st_out = t;
     }
     const int node = F[k];
     sol_gpu_node g;
@@ -340,7 +396,7 @@ __global__ void k_level_write(const int* F, uint32_t size, uint32_t n, uint32_t 
             F_next[inner_off] = ch;
             ++inner_off;
         } else {
-            ref = emit_leaf(ch, tri_base + tri_off, n, left, right, count, order);
+            ref = emit_leaf(ch, s.tri_base + tri_off, n, left, right, count, order);
             leafref[ch] = ref;
             tri_off += count[ch];
         }
@@ -352,11 +408,11 @@ __global__ void k_level_write(const int* F, uint32_t size, uint32_t n, uint32_t 
         g.child[c] = ref;
     }
     g.pad[0] = g.pad[1] = 0;
-    out[level_base + k] = g;
+    out[s.level_base + k] = g;
 }
 
 // The up-to-four children of the 4-wide node made from binary inner node `node`, left to right,
-// as k_collapse's costs choose them: subtree x placed in k slots is opened into its two subtrees
+// as the collapse costs choose them: subtree x placed in k slots is opened into its two subtrees
 // (split by wsplit) when that costs less than making x a 4-wide node. The slots of the pending
 // subtrees and the children placed always sum to four, so at most four are pending.
 SOL_DEV int wide_children(int node, uint32_t n, const int* left, const int* right, const unsigned* collapse,
@@ -384,8 +440,10 @@ SOL_DEV int wide_children(int node, uint32_t n, const int* left, const int* righ
     return cnt;
 }
 
-__global__ void k_wide_count(const int* F, uint32_t size, uint32_t n, const int* left, const int* right,
-                             const unsigned* collapse, const float* wcost, const unsigned* wsplit, unsigned* cnt) {
+__global__ void k_wide_count(const int* F, const LevelState* st, uint32_t bound, uint32_t n, const int* left,
+                             const int* right, const unsigned* collapse, const float* wcost, const unsigned* wsplit,
+                             unsigned* cnt) {
+    const uint32_t size = min(st->size, bound);
     uint32_t k = blockIdx.x * kBB + threadIdx.x;
     if (k >= size) return;
     int kids[4];
@@ -395,15 +453,28 @@ __global__ void k_wide_count(const int* F, uint32_t size, uint32_t n, const int*
     cnt[k] = inner;
 }
 
-__global__ void k_wide_write(const int* F, uint32_t size, uint32_t n, uint32_t level_base, uint32_t next_base,
+__global__ void k_wide_write(const int* F, const LevelState* st, LevelState* st_out, uint32_t bound, uint32_t n,
                              const unsigned* offs, const unsigned* cnt, const float4* nlo, const float4* nhi,
                              const int* left, const int* right, const unsigned* collapse, const float* wcost,
-                             const unsigned* wsplit, const int32_t* leafref, sol_gpu_node4* out, int* F_next,
-                             unsigned* totals) {
+                             const unsigned* wsplit, const int32_t* leafref, sol_gpu_node4* out, int* F_next) {
+    const LevelState s = *st;
+    const uint32_t size = min(s.size, bound);
     uint32_t k = blockIdx.x * kBB + threadIdx.x;
+    if (size == 0) {
+        if (k == 0) *st_out = s;
+        return;
+    }
     if (k >= size) return;
+    const uint32_t next_base = s.level_base + size;
     uint32_t off = offs[k];
-    if (k == size - 1) totals[0] = off + cnt[k];
+    if (k == size - 1) {
+        LevelState t = s;
+        t.size = off + cnt[k];
+        t.level_base = next_base;
+        t.level = s.level + 1;
+        t.top_nodes = s.level < kTopLevels4 ? next_base : s.top_nodes;
+        *st_out = t;
+    }
     int kids[4];
     const int c = wide_children(F[k], n, left, right, collapse, wcost, wsplit, kids);
     sol_gpu_node4 g;
@@ -428,7 +499,7 @@ __global__ void k_wide_write(const int* F, uint32_t size, uint32_t n, uint32_t l
             g.child[i] = leafref[ch];
         }
     }
-    out[level_base + k] = g;
+    out[s.level_base + k] = g;
 }
 
 // A tree whose root is a leaf: one binary node with that leaf twice, one 4-wide node with it once.
@@ -525,6 +596,72 @@ struct Buf {
     }
 };
 
+// Temporaries of a build, carved from the context's build workspace.
+struct Workspace {
+    float *P, *N;
+    uint32_t* I;
+    int32_t *tmat, *tlight;
+    float4 *nlo, *nhi;
+    int *left, *right, *C0, *C1, *nn, *F0, *F1;
+    unsigned *cb, *count, *collapse, *wsplit, *order;
+    float *cost, *wcost;
+    int32_t* leafref;
+    unsigned long long *keys0, *keys1, *flags, *offs, *sums64;
+    uint32_t *sort_work, *sums32;
+    PlocState* ploc;
+    LevelState* lev;
+    sol_gpu_node* nodes;
+    sol_gpu_node4* nodes4;
+    double* acc;
+
+    // Lays the arrays out from `base` (null: only measures); returns the bytes taken.
+    size_t carve(char* base, uint32_t n, uint32_t nv) {
+        size_t off = 0;
+        auto take = [&](auto*& p, size_t count) {
+            using T = std::remove_reference_t<decltype(*p)>;
+            off = (off + 255) & ~size_t(255);
+            p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+            off += sizeof(T) * std::max<size_t>(count, 1);
+        };
+        const size_t nodes_cap = 2 * size_t(n);
+        take(P, 3 * size_t(nv));
+        take(N, 3 * size_t(nv));
+        take(I, 3 * size_t(n));
+        take(tmat, n);
+        take(tlight, n);
+        take(nlo, nodes_cap);
+        take(nhi, nodes_cap);
+        take(left, nodes_cap);
+        take(right, nodes_cap);
+        take(C0, n);
+        take(C1, n);
+        take(nn, n);
+        take(F0, n);
+        take(F1, n);
+        take(cb, 6);
+        take(count, nodes_cap);
+        take(collapse, nodes_cap);
+        take(wsplit, nodes_cap);
+        take(order, n);
+        take(cost, nodes_cap);
+        take(wcost, 4 * nodes_cap);
+        take(leafref, nodes_cap);
+        take(keys0, n);
+        take(keys1, n);
+        take(flags, n);
+        take(offs, n);
+        take(sums64, prims::scan_sums_words(n));
+        take(sort_work, prims::sort_workspace_words(n));
+        take(sums32, prims::scan_sums_words(n));
+        take(ploc, 2);
+        take(lev, 2);
+        take(nodes, n);
+        take(nodes4, n);
+        take(acc, 1);
+        return off;
+    }
+};
+
 } // namespace
 
 int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_config& cfg, solhip_scene** out,
@@ -534,195 +671,161 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     auto t0 = std::chrono::steady_clock::now();
     BUILD_TRY(hipSetDevice(ctx.device));
 
+    Workspace w{};
+    BUILD_TRY(ctx.build_ws.reserve(w.carve(nullptr, n, nv)));
+    w.carve(ctx.build_ws.as<char>(), n, nv);
+
     // inputs
-    Buf<float> P, N;
-    Buf<uint32_t> I;
-    Buf<int32_t> tmat, tlight;
-    BUILD_TRY(P.alloc(3 * size_t(nv)));
-    BUILD_TRY(N.alloc(3 * size_t(nv)));
-    BUILD_TRY(I.alloc(3 * size_t(n)));
-    BUILD_TRY(tmat.alloc(n));
-    BUILD_TRY(tlight.alloc(n));
-    BUILD_TRY(hipMemcpyAsync(P.p, m.positions, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, st));
-    BUILD_TRY(hipMemcpyAsync(N.p, m.normals, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, st));
-    BUILD_TRY(hipMemcpyAsync(I.p, m.indices, sizeof(uint32_t) * 3 * n, hipMemcpyHostToDevice, st));
-    BUILD_TRY(hipMemcpyAsync(tmat.p, m.tri_material, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
-    BUILD_TRY(hipMemcpyAsync(tlight.p, m.tri_light, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    BUILD_TRY(hipMemcpyAsync(w.P, m.positions, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, st));
+    BUILD_TRY(hipMemcpyAsync(w.N, m.normals, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, st));
+    BUILD_TRY(hipMemcpyAsync(w.I, m.indices, sizeof(uint32_t) * 3 * n, hipMemcpyHostToDevice, st));
+    BUILD_TRY(hipMemcpyAsync(w.tmat, m.tri_material, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    BUILD_TRY(hipMemcpyAsync(w.tlight, m.tri_light, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
 
-    // node arrays: [0, n) triangles, [n, 2n - 1) clusters
-    const size_t nodes_cap = 2 * size_t(n);
-    Buf<float4> nlo, nhi;
-    Buf<int> left, right, parent, C0, C1, nn;
-    Buf<unsigned> cb, totals, visits, count, collapse, wsplit, ids0, ids1;
-    Buf<float> cost, wcost;
-    Buf<unsigned long long> keys0, keys1, flags, offs;
-    BUILD_TRY(nlo.alloc(nodes_cap));
-    BUILD_TRY(nhi.alloc(nodes_cap));
-    BUILD_TRY(left.alloc(nodes_cap));
-    BUILD_TRY(right.alloc(nodes_cap));
-    BUILD_TRY(parent.alloc(nodes_cap));
-    BUILD_TRY(C0.alloc(n));
-    BUILD_TRY(C1.alloc(n));
-    BUILD_TRY(nn.alloc(n));
-    BUILD_TRY(cb.alloc(6));
-    BUILD_TRY(totals.alloc(4));
-    BUILD_TRY(visits.alloc(nodes_cap));
-    BUILD_TRY(count.alloc(nodes_cap));
-    BUILD_TRY(collapse.alloc(nodes_cap));
-    BUILD_TRY(cost.alloc(nodes_cap));
-    BUILD_TRY(wcost.alloc(4 * nodes_cap));
-    BUILD_TRY(wsplit.alloc(nodes_cap));
-    BUILD_TRY(ids0.alloc(n));
-    BUILD_TRY(ids1.alloc(n));
-    BUILD_TRY(keys0.alloc(n));
-    BUILD_TRY(keys1.alloc(n));
-    BUILD_TRY(flags.alloc(n));
-    BUILD_TRY(offs.alloc(n));
-
-    // 1. boxes, Morton codes, sort
-    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, cb.p);
-    hipLaunchKernelGGL(k_tri_boxes, dim3(grid_for(n)), dim3(kBB), 0, st, P.p, I.p, n, nlo.p, nhi.p, cb.p);
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(kBB), 0, st, nlo.p, nhi.p, n, cb.p, keys0.p, ids0.p);
-    BUILD_TRY(hipGetLastError());
-    size_t sort_bytes = 0, scan_bytes = 0, scan32_bytes = 0;
-    BUILD_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys0.p, keys1.p, ids0.p, ids1.p,
-                                                 static_cast<int>(n), 0, 63, st));
-    BUILD_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags.p, offs.p, static_cast<int>(n), st));
-    BUILD_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan32_bytes, ids0.p, ids1.p, static_cast<int>(n), st));
-    Buf<unsigned char> tmp;
-    BUILD_TRY(tmp.alloc(std::max(sort_bytes, std::max(scan_bytes, scan32_bytes))));
-    BUILD_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.p, sort_bytes, keys0.p, keys1.p, ids0.p, ids1.p,
-                                                 static_cast<int>(n), 0, 63, st));
-    // clusters start as the triangles in Morton order
-    BUILD_TRY(hipMemcpyAsync(C0.p, ids1.p, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-    BUILD_TRY(hipMemsetAsync(parent.p, 0xFF, sizeof(int) * nodes_cap, st));
-
-    // 2. PLOC
-    const int r = static_cast<int>(cfg.search_radius);
-    uint32_t mcount = n, next_node = n, iterations = 0;
-    int* Ccur = C0.p;
-    int* Cnext = C1.p;
-    while (mcount > 1) {
-        hipLaunchKernelGGL(k_nearest, dim3(grid_for(mcount)), dim3(kBB), 0, st, Ccur, mcount, nlo.p, nhi.p, r, nn.p);
-        hipLaunchKernelGGL(k_merge_flags, dim3(grid_for(mcount)), dim3(kBB), 0, st, nn.p, mcount, flags.p);
-        BUILD_TRY(hipGetLastError());
-        BUILD_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan_bytes, flags.p, offs.p, static_cast<int>(mcount), st));
-        hipLaunchKernelGGL(k_merge, dim3(grid_for(mcount)), dim3(kBB), 0, st, Ccur, nn.p, flags.p, offs.p, mcount,
-                           next_node, nlo.p, nhi.p, left.p, right.p, parent.p, Cnext, totals.p);
-        BUILD_TRY(hipGetLastError());
-        unsigned tot[2];
-        BUILD_TRY(hipMemcpyAsync(tot, totals.p, sizeof(tot), hipMemcpyDeviceToHost, st));
-        BUILD_TRY(hipStreamSynchronize(st));
-        if (tot[0] == 0 || tot[1] != mcount - tot[0]) return set_error(SOLHIP_E_HIP, "GPU build: clustering made no progress");
-        next_node += tot[0];
-        mcount = tot[1];
-        std::swap(Ccur, Cnext);
-        ++iterations;
-    }
-    int root = 0;
-    BUILD_TRY(hipMemcpyAsync(&root, Ccur, sizeof(int), hipMemcpyDeviceToHost, st));
-    BUILD_TRY(hipStreamSynchronize(st));
-
-    // 3. SAH collapse
+    // 1. boxes (and the triangles' SAH leaf values), Morton codes, sort
+    const float ct = cfg.traversal_cost, ci = cfg.intersection_cost;
     const uint32_t max_leaf = cfg.max_leaf_size;
-    BUILD_TRY(hipMemsetAsync(visits.p, 0, sizeof(unsigned) * nodes_cap, st));
-    hipLaunchKernelGGL(k_collapse, dim3(grid_for(n)), dim3(kBB), 0, st, n, nlo.p, nhi.p, left.p, right.p, parent.p,
-                       visits.p, count.p, cost.p, collapse.p, wcost.p, wsplit.p, cfg.traversal_cost,
-                       cfg.intersection_cost, max_leaf);
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, w.cb);
+    hipLaunchKernelGGL(k_tri_boxes, dim3(grid_for(n)), dim3(kBB), 0, st, w.P, w.I, n, w.nlo, w.nhi, w.cb, ci, w.count,
+                       w.cost, w.collapse, w.wcost, w.wsplit);
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(kBB), 0, st, w.nlo, w.nhi, n, w.cb, w.keys0,
+                       reinterpret_cast<unsigned*>(w.C0));
     BUILD_TRY(hipGetLastError());
+    bool in_tmp = false;
+    BUILD_TRY(prims::radix_sort_pairs(w.keys0, reinterpret_cast<uint32_t*>(w.C0), w.keys1,
+                                      reinterpret_cast<uint32_t*>(w.C1), n, 63, w.sort_work, st, in_tmp));
+    // clusters start as the triangles in Morton order
+    int* Cbuf[2] = {in_tmp ? w.C1 : w.C0, in_tmp ? w.C0 : w.C1};
+
+    // 2. PLOC, with the SAH collapse of each node as it is made
+    const int r = static_cast<int>(cfg.search_radius);
+    uint32_t iterations = 0;
+    int root = 0;
+    if (n > 1) {
+        hipLaunchKernelGGL(k_ploc_init, dim3(1), dim3(64), 0, st, w.ploc, n);
+        uint32_t bound = n;
+        int p = 0;
+        while (true) {
+            const unsigned g = grid_for(bound);
+            for (int k = 0; k < kPassesPerSync; ++k, ++p) {
+                const PlocState* si = w.ploc + (p & 1);
+                PlocState* so = w.ploc + ((p + 1) & 1);
+                int* Cc = Cbuf[p & 1];
+                int* Cn = Cbuf[(p + 1) & 1];
+                hipLaunchKernelGGL(k_nearest, dim3(g), dim3(kBB), 0, st, Cc, si, bound, w.nlo, w.nhi, r, w.nn);
+                hipLaunchKernelGGL(k_merge_flags, dim3(g), dim3(kBB), 0, st, w.nn, si, bound, w.flags);
+                BUILD_TRY(hipGetLastError());
+                BUILD_TRY(prims::exclusive_scan<unsigned long long>(w.flags, w.offs, &si->m, bound, w.sums64, st));
+                hipLaunchKernelGGL(k_merge, dim3(g), dim3(kBB), 0, st, Cc, w.nn, w.flags, w.offs, si, so, bound, w.nlo,
+                                   w.nhi, w.left, w.right, Cn, w.count, w.cost, w.collapse, w.wcost,
+                                   w.wsplit, ct, ci, max_leaf);
+                BUILD_TRY(hipGetLastError());
+            }
+            PlocState hs{};
+            BUILD_TRY(hipMemcpyAsync(&hs, w.ploc + (p & 1), sizeof(hs), hipMemcpyDeviceToHost, st));
+            BUILD_TRY(hipMemcpyAsync(&root, Cbuf[p & 1], sizeof(int), hipMemcpyDeviceToHost, st));
+            BUILD_TRY(hipStreamSynchronize(st));
+            if (hs.stalled || hs.m == 0 || hs.m > bound)
+                return set_error(SOLHIP_E_HIP, "GPU build: clustering made no progress");
+            iterations = hs.passes;
+            if (hs.m == 1) break;
+            bound = hs.m;
+        }
+    }
     unsigned root_collapsed = 0;
-    BUILD_TRY(hipMemcpyAsync(&root_collapsed, collapse.p + root, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    BUILD_TRY(hipStreamSynchronize(st));
+    if (root >= static_cast<int>(n)) {
+        BUILD_TRY(hipMemcpyAsync(&root_collapsed, w.collapse + root, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        BUILD_TRY(hipStreamSynchronize(st));
+    }
     // a root of more triangles than a leaf holds is never collapsed; a smaller one may be
     const bool single_leaf = root < static_cast<int>(n) || root_collapsed;
 
-    // 4. layouts
+    // 3. layouts
     Buf<sol_gpu_node> gnodes;
     Buf<sol_gpu_node4> gnodes4;
-    Buf<int32_t> leafref;
-    Buf<unsigned> order;
-    Buf<int> F0, F1;
-    BUILD_TRY(order.alloc(n));
-    BUILD_TRY(leafref.alloc(nodes_cap));
     uint32_t num_nodes = 0, num_nodes4 = 0, depth = 0, depth4 = 0, top_nodes = 0, top_nodes4 = 0, leaves = 0;
     if (single_leaf) {
         BUILD_TRY(gnodes.alloc(1));
         BUILD_TRY(gnodes4.alloc(1));
-        hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(64), 0, st, root, n, nlo.p, nhi.p, left.p, right.p, count.p,
-                           order.p, gnodes.p, gnodes4.p);
+        hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(64), 0, st, root, n, w.nlo, w.nhi, w.left, w.right, w.count,
+                           w.order, gnodes.p, gnodes4.p);
         BUILD_TRY(hipGetLastError());
         num_nodes = num_nodes4 = 1;
         depth = depth4 = 2;
         top_nodes = top_nodes4 = 1;
         leaves = 1;
     } else {
-        // inner binary nodes: at most n - 1
-        BUILD_TRY(gnodes.alloc(n));
-        BUILD_TRY(F0.alloc(n));
-        BUILD_TRY(F1.alloc(n));
-        BUILD_TRY(hipMemcpyAsync(F0.p, &root, sizeof(int), hipMemcpyHostToDevice, st));
-        uint32_t size = 1, level_base = 0, tri_base = 0, level = 0;
-        int* Fc = F0.p;
-        int* Fn = F1.p;
-        while (size > 0) {
-            hipLaunchKernelGGL(k_level_count, dim3(grid_for(size)), dim3(kBB), 0, st, Fc, size, n, left.p, right.p,
-                               count.p, collapse.p, flags.p);
-            BUILD_TRY(hipGetLastError());
-            BUILD_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan_bytes, flags.p, offs.p, static_cast<int>(size), st));
-            hipLaunchKernelGGL(k_level_write, dim3(grid_for(size)), dim3(kBB), 0, st, Fc, size, n, level_base,
-                               level_base + size, tri_base, offs.p, nlo.p, nhi.p, left.p, right.p, count.p,
-                               collapse.p, gnodes.p, Fn, leafref.p, order.p, totals.p, flags.p);
-            BUILD_TRY(hipGetLastError());
-            unsigned tot[2];
-            BUILD_TRY(hipMemcpyAsync(tot, totals.p, sizeof(tot), hipMemcpyDeviceToHost, st));
+        int* Fb[2] = {w.F0, w.F1};
+        // binary levels: a level has at most twice the inner nodes of the one above it
+        hipLaunchKernelGGL(k_level_init, dim3(1), dim3(64), 0, st, w.lev, w.F0, root);
+        LevelState hs{};
+        uint32_t bound = 1;
+        int p = 0;
+        while (true) {
+            for (int k = 0; k < kLevelsPerSync; ++k, ++p) {
+                const LevelState* si = w.lev + (p & 1);
+                LevelState* so = w.lev + ((p + 1) & 1);
+                const unsigned g = grid_for(bound);
+                hipLaunchKernelGGL(k_level_count, dim3(g), dim3(kBB), 0, st, Fb[p & 1], si, bound, n, w.left, w.right,
+                                   w.count, w.collapse, w.flags);
+                BUILD_TRY(hipGetLastError());
+                BUILD_TRY(prims::exclusive_scan<unsigned long long>(w.flags, w.offs, &si->size, bound, w.sums64, st));
+                hipLaunchKernelGGL(k_level_write, dim3(g), dim3(kBB), 0, st, Fb[p & 1], si, so, bound, n, w.offs, w.nlo,
+                                   w.nhi, w.left, w.right, w.count, w.collapse, w.nodes, Fb[(p + 1) & 1], w.leafref,
+                                   w.order, w.flags);
+                BUILD_TRY(hipGetLastError());
+                bound = std::min<uint32_t>(2 * bound, n);
+            }
+            BUILD_TRY(hipMemcpyAsync(&hs, w.lev + (p & 1), sizeof(hs), hipMemcpyDeviceToHost, st));
             BUILD_TRY(hipStreamSynchronize(st));
-            level_base += size;
-            if (level < kTopLevels) top_nodes = level_base;
-            tri_base += tot[1];
-            size = tot[0];
-            std::swap(Fc, Fn);
-            ++level;
+            if (hs.size == 0) break;
+            bound = hs.size;
         }
-        if (tri_base != n) return set_error(SOLHIP_E_HIP, "GPU build: leaves do not cover every triangle");
-        num_nodes = level_base;
-        depth = level + 1;
+        if (hs.tri_base != n) return set_error(SOLHIP_E_HIP, "GPU build: leaves do not cover every triangle");
+        num_nodes = hs.level_base;
+        depth = hs.level + 1;
+        top_nodes = hs.top_nodes;
         leaves = num_nodes + 1;
-        // 4-wide: at most as many nodes as binary inner nodes
-        BUILD_TRY(gnodes4.alloc(num_nodes));
-        BUILD_TRY(hipMemcpyAsync(F0.p, &root, sizeof(int), hipMemcpyHostToDevice, st));
-        Fc = F0.p;
-        Fn = F1.p;
-        size = 1;
-        level_base = 0;
-        level = 0;
-        unsigned* cnt = reinterpret_cast<unsigned*>(flags.p);
-        unsigned* cnt_off = reinterpret_cast<unsigned*>(offs.p);
-        while (size > 0) {
-            hipLaunchKernelGGL(k_wide_count, dim3(grid_for(size)), dim3(kBB), 0, st, Fc, size, n, left.p, right.p,
-                               collapse.p, wcost.p, wsplit.p, cnt);
-            BUILD_TRY(hipGetLastError());
-            BUILD_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan32_bytes, cnt, cnt_off, static_cast<int>(size), st));
-            hipLaunchKernelGGL(k_wide_write, dim3(grid_for(size)), dim3(kBB), 0, st, Fc, size, n, level_base,
-                               level_base + size, cnt_off, cnt, nlo.p, nhi.p, left.p, right.p, collapse.p, wcost.p,
-                               wsplit.p, leafref.p, gnodes4.p, Fn, totals.p);
-            BUILD_TRY(hipGetLastError());
-            unsigned tot = 0;
-            BUILD_TRY(hipMemcpyAsync(&tot, totals.p, sizeof(tot), hipMemcpyDeviceToHost, st));
+        // 4-wide levels: at most four times the nodes of the level above
+        hipLaunchKernelGGL(k_level_init, dim3(1), dim3(64), 0, st, w.lev, w.F0, root);
+        bound = 1;
+        p = 0;
+        unsigned* cnt = reinterpret_cast<unsigned*>(w.flags);
+        unsigned* cnt_off = reinterpret_cast<unsigned*>(w.offs);
+        while (true) {
+            for (int k = 0; k < kLevelsPerSync; ++k, ++p) {
+                const LevelState* si = w.lev + (p & 1);
+                LevelState* so = w.lev + ((p + 1) & 1);
+                const unsigned g = grid_for(bound);
+                hipLaunchKernelGGL(k_wide_count, dim3(g), dim3(kBB), 0, st, Fb[p & 1], si, bound, n, w.left, w.right,
+                                   w.collapse, w.wcost, w.wsplit, cnt);
+                BUILD_TRY(hipGetLastError());
+                BUILD_TRY(prims::exclusive_scan<uint32_t>(cnt, cnt_off, &si->size, bound, w.sums32, st));
+                hipLaunchKernelGGL(k_wide_write, dim3(g), dim3(kBB), 0, st, Fb[p & 1], si, so, bound, n, cnt_off, cnt,
+                                   w.nlo, w.nhi, w.left, w.right, w.collapse, w.wcost, w.wsplit, w.leafref, w.nodes4,
+                                   Fb[(p + 1) & 1]);
+                BUILD_TRY(hipGetLastError());
+                bound = std::min<uint32_t>(4 * bound, n);
+            }
+            BUILD_TRY(hipMemcpyAsync(&hs, w.lev + (p & 1), sizeof(hs), hipMemcpyDeviceToHost, st));
             BUILD_TRY(hipStreamSynchronize(st));
-            level_base += size;
-            if (level < kTopLevels4) top_nodes4 = level_base;
-            size = tot;
-            std::swap(Fc, Fn);
-            ++level;
+            if (hs.size == 0) break;
+            bound = hs.size;
         }
-        num_nodes4 = level_base;
-        depth4 = level + 1;
+        num_nodes4 = hs.level_base;
+        depth4 = hs.level + 1;
+        top_nodes4 = hs.top_nodes;
+        // the scene keeps arrays of the exact sizes
+        BUILD_TRY(gnodes.alloc(num_nodes));
+        BUILD_TRY(gnodes4.alloc(num_nodes4));
+        BUILD_TRY(hipMemcpyAsync(gnodes.p, w.nodes, sizeof(sol_gpu_node) * num_nodes, hipMemcpyDeviceToDevice, st));
+        BUILD_TRY(hipMemcpyAsync(gnodes4.p, w.nodes4, sizeof(sol_gpu_node4) * num_nodes4, hipMemcpyDeviceToDevice, st));
     }
     if (depth > static_cast<uint32_t>(8 + kSpillEntries) || 3 * (depth4 - 1) > static_cast<uint32_t>(8 + kSpillEntries))
         return set_error(SOLHIP_E_TOO_DEEP, "GPU-built BVH depth " + std::to_string(depth) +
                                                 " exceeds the traversal stack");
 
-    // 5. leaf-order triangles and shading records
+    // 4. leaf-order triangles and shading records
     Buf<sol_gpu_tri> tris;
     Buf<sol_gpu_shade> shade;
     Buf<sol_material> mats;
@@ -734,22 +837,19 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     if (m.num_materials)
         BUILD_TRY(hipMemcpyAsync(mats.p, m.materials, sizeof(sol_material) * m.num_materials, hipMemcpyHostToDevice, st));
     BUILD_TRY(hipMemcpyAsync(lights.p, m.lights, sizeof(sol_light) * m.num_lights, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kBB), 0, st, order.p, n, P.p, N.p, I.p, tmat.p, tlight.p,
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kBB), 0, st, w.order, n, w.P, w.N, w.I, w.tmat, w.tlight,
                        mats.p, tris.p, shade.p);
     BUILD_TRY(hipGetLastError());
     float sah = 0.0f;
     if (stats) {
-        Buf<double> acc;
-        BUILD_TRY(acc.alloc(1));
-        BUILD_TRY(hipMemsetAsync(acc.p, 0, sizeof(double), st));
-        hipLaunchKernelGGL(k_sah, dim3(grid_for(num_nodes)), dim3(kBB), 0, st, gnodes.p, num_nodes,
-                           cfg.traversal_cost, cfg.intersection_cost, acc.p);
+        BUILD_TRY(hipMemsetAsync(w.acc, 0, sizeof(double), st));
+        hipLaunchKernelGGL(k_sah, dim3(grid_for(num_nodes)), dim3(kBB), 0, st, gnodes.p, num_nodes, ct, ci, w.acc);
         BUILD_TRY(hipGetLastError());
         double total = 0.0;
         float4 rl, rh;
-        BUILD_TRY(hipMemcpyAsync(&total, acc.p, sizeof(double), hipMemcpyDeviceToHost, st));
-        BUILD_TRY(hipMemcpyAsync(&rl, nlo.p + root, sizeof(float4), hipMemcpyDeviceToHost, st));
-        BUILD_TRY(hipMemcpyAsync(&rh, nhi.p + root, sizeof(float4), hipMemcpyDeviceToHost, st));
+        BUILD_TRY(hipMemcpyAsync(&total, w.acc, sizeof(double), hipMemcpyDeviceToHost, st));
+        BUILD_TRY(hipMemcpyAsync(&rl, w.nlo + root, sizeof(float4), hipMemcpyDeviceToHost, st));
+        BUILD_TRY(hipMemcpyAsync(&rh, w.nhi + root, sizeof(float4), hipMemcpyDeviceToHost, st));
         BUILD_TRY(hipStreamSynchronize(st));
         float ra = half_area(rl, rh);
         sah = ra > 0.0f ? static_cast<float>(total / ra) : 0.0f;

@@ -83,6 +83,7 @@ struct Context {
     DeviceBuffer adj_radiance, adj_image;   // forward pass of solhip_backward (grow-only)
     DeviceBuffer isect_rays, isect_out;     // host-pointer staging of solhip_intersect (grow-only)
     DeviceBuffer host_image;                // device image of solhip_render for host images (grow-only)
+    DeviceBuffer build_ws;                  // temporaries of solhip_scene_build (grow-only)
     void* pinned = nullptr;                 // pinned host staging (grow-only)
     size_t pinned_bytes = 0;
     int live_scenes = 0;             // scenes uploaded and not yet destroyed
