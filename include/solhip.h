@@ -84,7 +84,7 @@ typedef struct solhip_tuning {
     uint32_t block_size;    /* threads per workgroup of the traversal kernels: 64, 128, 256 */
     uint32_t lds_stack;     /* short-stack entries per lane kept in LDS: 8, 16, 32           */
     uint32_t path_slots;    /* wavefront: paths in flight, 4096..2^27 (default: the largest
-                               power of two whose 528 B a slot take at most a quarter of the
+                               power of two whose 348 B a slot take at most a quarter of the
                                device memory, 2^27 on a 288 GB MI355X)                      */
     uint32_t tile_w;        /* batch: tile shape of the global work queue                    */
     uint32_t tile_h;

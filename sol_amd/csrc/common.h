@@ -160,6 +160,7 @@ struct SceneView {
     uint32_t top_nodes4;
     const float4* tris;    // 3 x float4 per triangle (sol_gpu_tri), leaf order
     const float4* shade;   // 4 x float4 per triangle (sol_gpu_shade), leaf order
+    const uint8_t* kinds;  // 1 byte per triangle, leaf order: tri_kind(bsdf, light)
     const sol_material* materials;
     const sol_light* lights;
     uint32_t num_lights;
@@ -168,6 +169,15 @@ struct SceneView {
     uint32_t num_tris;
     sol_camera camera;
 };
+
+// What the hit sorting needs to know of a triangle, in one byte so that the per-triangle table
+// (leaf order, 1 B a triangle) stays in L2: the BSDF kind of its material + 1 (0: no BSDF) in bits
+// 0-2, bit 3 set if the triangle is a light.
+SOL_HD uint8_t tri_kind(int bsdf, int light) {
+    return static_cast<uint8_t>((bsdf + 1) | (light >= 0 ? 8 : 0));
+}
+SOL_HD int kind_bsdf(uint32_t kind) { return static_cast<int>(kind & 7u) - 1; }
+SOL_HD bool kind_emissive(uint32_t kind) { return (kind & 8u) != 0; }
 
 // ---- ray / triangle ------------------------------------------------------------------------------
 

@@ -534,7 +534,7 @@ __global__ void k_single_leaf(int root, uint32_t n, const float4* nlo, const flo
 // cross(v1 - v0, v2 - v0) in fp32 without contraction).
 __global__ void k_gather(const unsigned* order, uint32_t n, const float* P, const float* N, const uint32_t* I,
                          const int32_t* tmat, const int32_t* tlight, const sol_material* mats, sol_gpu_tri* tris,
-                         sol_gpu_shade* shade) {
+                         sol_gpu_shade* shade, uint8_t* kinds) {
     uint32_t s = blockIdx.x * kBB + threadIdx.x;
     if (s >= n) return;
     const uint32_t t = order[s];
@@ -559,6 +559,7 @@ __global__ void k_gather(const unsigned* order, uint32_t n, const float* P, cons
     h.bsdf = h.material >= 0 ? mats[h.material].type : SOL_BSDF_NONE;
     h.pad = 0;
     shade[s] = h;
+    kinds[s] = tri_kind(h.bsdf, h.light);
 }
 
 // SAH cost of the binary layout: C_t x area of every inner node + C_i x triangles x area of every
@@ -828,17 +829,19 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     // 4. leaf-order triangles and shading records
     Buf<sol_gpu_tri> tris;
     Buf<sol_gpu_shade> shade;
+    Buf<uint8_t> kinds;
     Buf<sol_material> mats;
     Buf<sol_light> lights;
     BUILD_TRY(tris.alloc(n));
     BUILD_TRY(shade.alloc(n));
+    BUILD_TRY(kinds.alloc(n));
     BUILD_TRY(mats.alloc(m.num_materials));
     BUILD_TRY(lights.alloc(m.num_lights));
     if (m.num_materials)
         BUILD_TRY(hipMemcpyAsync(mats.p, m.materials, sizeof(sol_material) * m.num_materials, hipMemcpyHostToDevice, st));
     BUILD_TRY(hipMemcpyAsync(lights.p, m.lights, sizeof(sol_light) * m.num_lights, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kBB), 0, st, w.order, n, w.P, w.N, w.I, w.tmat, w.tlight,
-                       mats.p, tris.p, shade.p);
+                       mats.p, tris.p, shade.p, kinds.p);
     BUILD_TRY(hipGetLastError());
     float sah = 0.0f;
     if (stats) {
@@ -870,6 +873,7 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     s->nodes4 = gnodes4.release();
     s->tris = tris.release();
     s->shade = shade.release();
+    s->kinds = kinds.release();
     s->materials = mats.release();
     s->lights = lights.release();
     SceneView& v = s->view;
@@ -879,6 +883,7 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     v.top_nodes4 = top_nodes4;
     v.tris = reinterpret_cast<const float4*>(s->tris);
     v.shade = reinterpret_cast<const float4*>(s->shade);
+    v.kinds = s->kinds;
     v.materials = s->materials;
     v.lights = s->lights;
     v.num_lights = m.num_lights;

@@ -38,11 +38,11 @@ struct DeviceBuffer {
 struct WavefrontState;  // defined in wavefront.hip
 
 // Device memory of the wavefront state per path slot: the slot record and the hit (32 B) and the
-// queue segments (two extension queues 256 B, three shade queues 192 B, the shadow queue 48 B).
-constexpr unsigned long long kPathSlotBytes = 528;
+// queue segments (two extension queues 256 B, three shade queues 12 B, the shadow queue 48 B).
+constexpr unsigned long long kPathSlotBytes = 348;
 
 // Paths in flight by default: the largest power of two up to 2^27 whose state takes at most a
-// quarter of the device's memory (2^27 slots, 71 GB, on a 288 GB MI355X: a 1080p 64-spp frame is
+// quarter of the device's memory (2^27 slots, 47 GB, on a 288 GB MI355X: a 1080p 64-spp frame is
 // then in flight at once; 2^24 slots where the memory size is not known).
 inline uint32_t default_path_slots(unsigned long long device_memory) {
     if (device_memory == 0) return 1u << 24;
@@ -111,6 +111,7 @@ struct solhip_scene {
     sol_gpu_node4* nodes4 = nullptr;   // may be null (the wavefront traversal then uses nodes)
     sol_gpu_tri* tris = nullptr;
     sol_gpu_shade* shade = nullptr;
+    uint8_t* kinds = nullptr;          // tri_kind of every triangle (leaf order)
     sol_material* materials = nullptr;
     sol_light* lights = nullptr;
     uint32_t num_nodes = 0, num_tris = 0, num_materials = 0, num_lights = 0, bvh_depth = 0, bvh4_depth = 0;

@@ -70,16 +70,18 @@ enum : int {
 };
 
 // One bounce of all paths in flight is four launches: trace (closest hit of the extension rays
-// and any hit of the previous bounce's shadow rays), reset, sort (hit decoding, emission, material
-// queues) and shade (one kernel per material queue: NEE, Russian roulette, BSDF sampling).
+// and any hit of the previous bounce's shadow rays), reset, sort (material queues from the hit
+// triangle's kind byte; paths that missed end there) and shade (one kernel per material queue:
+// hit decoding, emission, NEE, Russian roulette, BSDF sampling).
 // A path's state travels with it in its queue entries, written at compacted positions
 // (coalesced, field-major): extension rays (64 B: the ray and the path state), hits (16 B), shade
-// entries (the hit record and where the path state is, 64 B), shadow rays (48 B). A path's
-// throughput and PCG state stay in its extension entry until the shading kernel has read them
-// (nothing writes that queue during the bounce), so the sort does not copy them. Only what other kernels add to
-// out of order stays in a 16-byte record per slot (random access): the radiance (emission added by
-// the sort, unoccluded light samples by the trace) and the scene. Appends are
-// compacted per workgroup (one atomic per round of kWaveBlock entries, queue and segment).
+// entries (4 B: the index of the extension entry and its hit), shadow rays (48 B). The ray, hit,
+// throughput and PCG state of a path stay where the trace read and wrote them until the shading
+// kernel has read them (nothing writes that extension queue or the hits during the bounce), so the
+// sort copies nothing. Only what other kernels add to out of order stays in a 16-byte record per
+// slot (random access): the radiance (emission added by the shading, unoccluded light samples by
+// the trace) and the scene. Appends are compacted per workgroup (one atomic per round of
+// kWaveBlock entries, queue and segment).
 // A path that ends writes its radiance to its sample's entry of the sample buffer and the slot
 // starts the next work item at once (in the kernel that ended it); a shadow ray still in flight
 // for the ended path adds its contribution to that sample entry instead of the slot record.
@@ -90,9 +92,7 @@ enum : int {
 //                             [2] throughput xyz, pdf of the previous bounce (not stored at depth
 //                             0: throughput 1, pdf 0)
 //                             [3] PCG state lo, hi, work item lo, hi (uint bits)
-// shade entry (4 x float4):   [0] point xyz, material | front bit  [1] shading normal xyz, slot
-//                             [2] geometric normal xyz, depth
-//                             [3] ray direction xyz, storage index of the extension entry
+// shade entry (uint):         position i of the extension entry in its queue (its hit is hits[i])
 // shadow entry (3 x float4):  [0] org xyz, target  [1] dir xyz, item hi  [2] contribution xyz, item lo
 //                             target = slot, or kSampleTarget | 0 when the path has ended (the
 //                             contribution then goes to the sample entry of the item)
@@ -108,12 +108,12 @@ struct WaveBuf {
     uint32_t nseg;                    // segments per queue (1, 2, 4 or 8)
     uint32_t merged;                  // 1: one shading queue (queue 0) for every BSDF
     uint32_t cap_e, cap_m, cap_s;     // segment capacities: extension, shade, shadow queues
-    uint32_t stride_e, stride_m, stride_s;   // nseg * cap_e, cap_m, cap_s (the queues' field strides)
+    uint32_t stride_e, stride_s;      // nseg * cap_e, cap_s (the queues' field strides)
     float4* path;                     // [slots]: slot records
     float4* hits;                     // [slots]: closest hit of extension entry i (in queue order):
                                       // leaf-order triangle (int bits, -1 = miss), t, u, v
     float4* eq[2];                    // [nseg * cap_e * 4] each
-    float4* mq[kNumShadeQueues];      // [nseg * cap_m * 4] each (only mq[0] when merged)
+    uint32_t* mq[kNumShadeQueues];    // [nseg * cap_m] each (only mq[0] when merged)
     float4* sq;                       // [nseg * cap_s * 3]
     unsigned* counts;                 // kCounterWords
     unsigned long long* items;        // kItemWords
@@ -129,7 +129,7 @@ struct WaveBuf {
     // indices fit 32 bits: at 2^27 slots an extension queue has 4 words of 2^28 entries.
     SOL_DEV float4* rec(uint32_t slot) const { return path + slot; }
     SOL_DEV float4* ext(int q, uint32_t a, uint32_t k) const { return eq[q] + (k * stride_e + a); }
-    SOL_DEV float4* shade_entry(int q, uint32_t a, uint32_t k) const { return mq[q] + (k * stride_m + a); }
+    SOL_DEV uint32_t* shade_entry(int q, uint32_t a) const { return mq[q] + a; }
     SOL_DEV float4* shadow(uint32_t a, uint32_t k) const { return sq + (k * stride_s + a); }
 };
 

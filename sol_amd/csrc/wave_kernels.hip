@@ -494,6 +494,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
 
 // ---- sort ------------------------------------------------------------------------------------------
 
+// Diffuse / glossy (Phong, and the diffuse-Phong mix) / specular: the queues follow the BSDF's
+// shading work; a light without BSDF goes to the diffuse queue, whose kernel adds its emission and
+// ends the path.
+SOL_DEV int queue_of_kind(uint32_t kind) {
+    const int bsdf = kind_bsdf(kind);
+    return bsdf < 0 ? kQueueDiffuse : queue_of(bsdf);
+}
+
+// Material queues of the extension queue's hits: each hit's entry index goes to the queue of its
+// triangle's kind (a byte per triangle, held in L2). A path that missed, or hit a triangle that
+// neither scatters nor emits, ends here; its slot starts the next work item.
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sort(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                  int cur) {
@@ -505,7 +516,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     unsigned* qcnt[kNumShadeQueues] = {&b.counts[qcount(kQShade0, seg)], &b.counts[qcount(kQShade0 + 1, seg)],
                                        &b.counts[qcount(kQShade0 + 2, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
-    const bool start_fill = cur == 0 && b.counts[kCntStartFill] != 0;   // see k_trace
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
         unsigned i = round * kBlock + threadIdx.x;
@@ -513,60 +523,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         int dest = -1;
         bool dead = false;
         uint32_t slot = 0;
-        float4 rec0, rec1, rec2, rec3, e0, e1, h;
-        unsigned ea = 0;   // the entry's storage index: the shading kernel reads the path state there
+        unsigned ea = 0;
         if (valid) {
-            // the ray and its hit at once (a hole's are in bounds and unused): one memory round
-            // trip fewer before the shading record, which depends on the hit
             ea = eqv.entry(i);
-            if (!start_fill) e0 = *b.ext(cur, ea, 0);
-            e1 = *b.ext(cur, ea, 1);
-            h = b.hits[i];
-            slot = as_u(e1.w);
+            const float4 h = b.hits[i];           // a hole's is in bounds and unused
+            slot = as_u(b.ext(cur, ea, 1)->w);
             valid = slot != kHole;
-            if (start_fill && valid) {   // a new path: the camera's eye, depth 0
-                const SceneView& sv = MULTI ? tabs.scenes[as_u(b.rec(slot)->w)] : single;
-                e0 = make_float4(sv.camera.eye[0], sv.camera.eye[1], sv.camera.eye[2], as_f(0u));
-            }
-        }
-        if (valid) {
-            int tri = static_cast<int>(as_u(h.x));
-            if (tri < 0) {
-                dead = true;
-            } else {
-                SceneView tmp;
-                const SceneView& s = scene_of<MULTI>(single, tabs, b, slot, tmp);
-                V3 org = xyz(e0), dir = xyz(e1);
-                Surface sf = decode_hit(s, org, dir, tri, h.y, h.z, h.w);
-                if (sf.light >= 0 && sf.front) {
-                    const float4 e2 = ext_throughput(b, cur, ea, as_u(e0.w));
-                    float4 rad = *b.rec(slot);
-                    V3 Le;
-                    float w;
-                    V3 C = xyz(rad) + hit_emission(s, cfg.flags, sf, dir, h.y, e2.w,
-                                                   1.0f / static_cast<float>(s.num_lights), xyz(e2), Le, w);
-                    *b.rec(slot) = f4(C, rad.w);
+            if (valid) {
+                const int tri = static_cast<int>(as_u(h.x));
+                uint32_t kind = 0;
+                if (tri >= 0) {
+                    SceneView tmp;
+                    kind = scene_of<MULTI>(single, tabs, b, slot, tmp).kinds[tri];
                 }
-                if (sf.mat < 0) {
-                    dead = true;
-                } else {
-                    rec0 = f4(sf.point, as_f(static_cast<uint32_t>(sf.mat) | (sf.front ? kFrontBit : 0u)));
-                    rec1 = f4(sf.n, as_f(slot));
-                    rec2 = f4(sf.face_n, e0.w);   // depth
-                    rec3 = f4(dir, as_f(ea));     // where the path state is
-                    dest = b.merged ? 0 : queue_of(sf.bsdf);
-                }
+                if (kind == 0) dead = true;   // a miss, or no BSDF and no light
+                else dest = b.merged ? 0 : queue_of_kind(kind);
+                if (dead) retire(b, *b.rec(slot), item_of(*b.ext(cur, ea, 3)));
             }
-            if (dead) retire(b, *b.rec(slot), item_of(*b.ext(cur, ea, 3)));
         }
         RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, nseg, dest, false, dead);
-        if (plan.pos >= 0) {
-            const unsigned a = seg * b.cap_m + static_cast<unsigned>(plan.pos);
-            *b.shade_entry(dest, a, 0) = rec0;
-            *b.shade_entry(dest, a, 1) = rec1;
-            *b.shade_entry(dest, a, 2) = rec2;
-            *b.shade_entry(dest, a, 3) = rec3;
-        }
+        if (plan.pos >= 0) *b.shade_entry(dest, seg * b.cap_m + static_cast<unsigned>(plan.pos)) = i;
         if (plan.start) {
             ++started;
             init_path(tabs, b, plan.item, slot, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
@@ -584,22 +560,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 // kernel (Phong and the diffuse-Phong mix) is held to three waves per SIMD (at most 168 VGPRs).
 // QUEUE == kQueueAll: the merged queue (stored as queue 0), every BSDF chosen at run time, held
 // to three waves per SIMD like the glossy kernel.
+// Per entry: the ray and path state from the extension entry of queue 1 - nxt, the hit decoded
+// (triangle_mesh.cpp:78-104), the emission of a front-side light hit added to the path's radiance
+// (path_tracer.cpp:49-60), then - if the surface has a BSDF - NEE, Russian roulette and the BSDF
+// sample of the next ray (path_tracer.cpp:62-118).
 template <int QUEUE, bool MULTI>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE == kQueuePhong || QUEUE == kQueueAll ? 3 : 1))) void k_shade(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                   int nxt) {
     constexpr int SQ = QUEUE == kQueueAll ? 0 : QUEUE;   // storage queue
     __shared__ RoundLds<1> lds[2];
-    // after the first sort of a chunk, extension queue 0 no longer holds its start fill
-    if (SQ == 0 && blockIdx.x == 0 && threadIdx.x == 0) b.counts[kCntStartFill] = 0;
-    const SegQueue mqv(b, kQShade0 + SQ, b.cap_m);
+    const int cur = 1 - nxt;
+    const SegQueue mqv(b, kQShade0 + SQ, b.cap_m), eqv(b, kQExtend0 + cur, b.cap_e);
     const unsigned count = mqv.size();
+    // extension queue 0 holding a chunk's start fill: origins are the camera's eye (see k_trace)
+    const bool start_fill = cur == 0 && b.counts[kCntStartFill] != 0;
     const int seg = my_segment(b), nseg = static_cast<int>(b.nseg);
     unsigned* scnt[1] = {&b.counts[qcount(kQShadow, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
-        unsigned i = round * kBlock + threadIdx.x;
-        bool valid = i < count;
+        unsigned j = round * kBlock + threadIdx.x;
+        bool valid = j < count;
         uint32_t slot = 0;
         bool shoot = false, cont = false;
         V3 point{0, 0, 0}, sdir{0, 0, 0}, contrib{0, 0, 0}, ndir{0, 0, 0}, T_next{0, 0, 0};
@@ -607,53 +588,58 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
         uint32_t depth_next = 0;
         unsigned long long item = 0, rng_next = 0;
         if (valid) {
-            const unsigned a = mqv.entry(i);
-            float4 r0 = *b.shade_entry(SQ, a, 0), r1 = *b.shade_entry(SQ, a, 1), r2 = *b.shade_entry(SQ, a, 2),
-                   r3 = *b.shade_entry(SQ, a, 3);
-            // the path state stays in the extension entry the sort read (queue 1 - nxt)
-            const uint32_t depth0 = as_u(r2.w), at = as_u(r3.w);
-            const float4 r4 = ext_throughput(b, 1 - nxt, at, depth0), r5 = *b.ext(1 - nxt, at, 3);
-            slot = as_u(r1.w);
+            const unsigned i = *b.shade_entry(SQ, mqv.entry(j));
+            const unsigned ea = eqv.entry(i);
+            const float4 h = b.hits[i], e1 = *b.ext(cur, ea, 1), e3 = *b.ext(cur, ea, 3);
+            slot = as_u(e1.w);
             SceneView tmp;
             const SceneView& s = scene_of<MULTI>(single, tabs, b, slot, tmp);
-            Surface sf;
-            uint32_t hm = as_u(r0.w);
-            sf.point = xyz(r0);
-            sf.n = xyz(r1);
-            sf.face_n = xyz(r2);
-            sf.front = (hm & kFrontBit) != 0;
-            sf.mat = static_cast<int>(hm & ~kFrontBit);
-            sf.light = -1;
-            point = sf.point;
-            V3 out = -xyz(r3);
-            sol_material m = s.materials[sf.mat];
-            if (QUEUE == kQueueDiffuse) m.type = SOL_BSDF_DIFFUSE;
-            if (QUEUE == kQueuePhong && m.type != SOL_BSDF_INTERP) m.type = SOL_BSDF_PHONG;
-            if (QUEUE == kQueueSpecular && m.type != SOL_BSDF_GLASS) m.type = SOL_BSDF_MIRROR;
-            item = item_of(r5);
-            V3 T = xyz(r4);
-            Pcg rng;
-            rng.state = rng_of(r5);
-            uint32_t depth = depth0;
-            bool skip_nee = QUEUE == kQueueSpecular || (QUEUE == kQueueAll && is_specular(m.type)) ||
-                            (cfg.flags & SOL_PT_DISABLE_NEE);
-            if (!skip_nee) {
-                NeeSample ns;
-                shoot = nee_sample(s, cfg.flags, sf, m, out, T, rng, 1.0f / static_cast<float>(s.num_lights), ns);
-                sdir = ns.shadow_dir;
-                contrib = ns.contrib;
+            const float4 e0 = start_fill ? make_float4(s.camera.eye[0], s.camera.eye[1], s.camera.eye[2], as_f(0u))
+                                         : *b.ext(cur, ea, 0);
+            const uint32_t depth0 = as_u(e0.w);
+            const float4 e2 = ext_throughput(b, cur, ea, depth0);
+            const V3 dir = xyz(e1);
+            const Surface sf = decode_hit(s, xyz(e0), dir, static_cast<int>(as_u(h.x)), h.y, h.z, h.w);
+            const float pick = 1.0f / static_cast<float>(s.num_lights);
+            item = item_of(e3);
+            V3 T = xyz(e2);
+            if (sf.light >= 0 && sf.front) {
+                const float4 rad = *b.rec(slot);
+                V3 Le;
+                float w;
+                const V3 C = xyz(rad) + hit_emission(s, cfg.flags, sf, dir, h.y, e2.w, pick, T, Le, w);
+                *b.rec(slot) = f4(C, rad.w);
             }
-            float q;
-            BsdfSample bs;
-            if (russian_roulette(cfg, depth, T, rng, q) && bsdf_sample(m, sf, ortho_basis(sf.n), out, rng, bs)) {
-                T = T * (bs.color * (bs.cos / (bs.pdf * q)));
-                depth += 1;
-                cont = depth < cfg.max_path_len;
-                ndir = bs.in;
-                T_next = T;
-                pdf_next = skip_nee ? 0.0f : bs.pdf;
-                depth_next = depth;
-                rng_next = rng.state;
+            if (sf.mat >= 0) {
+                point = sf.point;
+                const V3 out = -dir;
+                sol_material m = s.materials[sf.mat];
+                if (QUEUE == kQueueDiffuse) m.type = SOL_BSDF_DIFFUSE;
+                if (QUEUE == kQueuePhong && m.type != SOL_BSDF_INTERP) m.type = SOL_BSDF_PHONG;
+                if (QUEUE == kQueueSpecular && m.type != SOL_BSDF_GLASS) m.type = SOL_BSDF_MIRROR;
+                Pcg rng;
+                rng.state = rng_of(e3);
+                uint32_t depth = depth0;
+                bool skip_nee = QUEUE == kQueueSpecular || (QUEUE == kQueueAll && is_specular(m.type)) ||
+                                (cfg.flags & SOL_PT_DISABLE_NEE);
+                if (!skip_nee) {
+                    NeeSample ns;
+                    shoot = nee_sample(s, cfg.flags, sf, m, out, T, rng, pick, ns);
+                    sdir = ns.shadow_dir;
+                    contrib = ns.contrib;
+                }
+                float q;
+                BsdfSample bs;
+                if (russian_roulette(cfg, depth, T, rng, q) && bsdf_sample(m, sf, ortho_basis(sf.n), out, rng, bs)) {
+                    T = T * (bs.color * (bs.cos / (bs.pdf * q)));
+                    depth += 1;
+                    cont = depth < cfg.max_path_len;
+                    ndir = bs.in;
+                    T_next = T;
+                    pdf_next = skip_nee ? 0.0f : bs.pdf;
+                    depth_next = depth;
+                    rng_next = rng.state;
+                }
             }
             if (!cont) retire(b, *b.rec(slot), item);
         }
@@ -726,10 +712,13 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
 }
 
 // After trace(k) has consumed extension queue `cur` and the shadow queue: clears what sort(k) and
-// shade(k) append to (kernel, not a memset: it is captured in graphs).
+// shade(k) append to (kernel, not a memset: it is captured in graphs). When cur is 1, extension
+// queue 0 has been traced, sorted and shaded since the last chunk started, so it no longer holds a
+// start fill.
 __global__ void k_reset(WaveBuf b, int cur) {
     int i = threadIdx.x;
     if (i < kNumWaveCounters && i / kMaxSegments != kQExtend0 + cur) b.counts[i * kCounterStride] = 0;
+    if (i == 0 && cur == 1) b.counts[kCntStartFill] = 0;
 }
 
 // Work items of a chunk [begin, end) by segment, consecutive ranges in segment order: segment g

@@ -62,10 +62,11 @@ struct WavefrontState {
     // launches) and so for the merged queue (one).
     struct Layout {
         size_t cap_e, cap_m, cap_s;
-        // slot record and hit 16 B each per slot; queue entries 64 B (extension), 64 B (shade),
-        // 48 B (shadow) per segment entry
-        size_t float4s(size_t n, size_t nseg) const {
-            return n * (1 + 1) + nseg * (2 * cap_e * 4 + kNumShadeQueues * cap_m * 4 + cap_s * 3);
+        // slot record and hit 16 B each per slot; queue entries 64 B (extension), 4 B (shade),
+        // 48 B (shadow) per segment entry; each array 256-byte aligned
+        size_t bytes(size_t n, size_t nseg) const {
+            return 2 * align_up(sizeof(float4) * n) + 2 * align_up(sizeof(float4) * 4 * nseg * cap_e) +
+                   kNumShadeQueues * align_up(sizeof(uint32_t) * nseg * cap_m) + align_up(sizeof(float4) * 3 * nseg * cap_s);
         }
     };
     static Layout layout(size_t n, size_t nseg) {
@@ -78,9 +79,9 @@ struct WavefrontState {
     // count) and lays the queues out in it for nseg segments and the shading queue split.
     int ensure_slots(uint32_t n, uint32_t nseg, bool merged) {
         if (!arena.ptr || n > slots) {
-            size_t f4_total = 0;
-            for (size_t g = 1; g <= kMaxSegments; g *= 2) f4_total = std::max(f4_total, layout(n, g).float4s(n, g));
-            HIP_OK(arena.alloc(sizeof(float4) * f4_total + 16 * 4096));
+            size_t total = 0;
+            for (size_t g = 1; g <= kMaxSegments; g *= 2) total = std::max(total, layout(n, g).bytes(n, g));
+            HIP_OK(arena.alloc(total));
             HIP_OK(counters.alloc(sizeof(unsigned) * kCounterWords + align_up(sizeof(unsigned long long) * kItemWords) + 256));
             slots = n;
             buf.nseg = 0;   // lay out again
@@ -90,6 +91,7 @@ struct WavefrontState {
         const Layout lay = layout(slots, nseg);
         char* p = arena.as<char>();
         auto f4s = [&](size_t count) { float4* r = reinterpret_cast<float4*>(p); p += align_up(sizeof(float4) * count); return r; };
+        auto u32s = [&](size_t count) { uint32_t* r = reinterpret_cast<uint32_t*>(p); p += align_up(sizeof(uint32_t) * count); return r; };
         b.slots = slots;
         b.nseg = nseg;
         b.merged = merged ? 1u : 0u;
@@ -97,12 +99,11 @@ struct WavefrontState {
         b.cap_m = static_cast<uint32_t>(lay.cap_m);
         b.cap_s = static_cast<uint32_t>(lay.cap_s);
         b.stride_e = static_cast<uint32_t>(nseg * lay.cap_e);
-        b.stride_m = static_cast<uint32_t>(nseg * lay.cap_m);
         b.stride_s = static_cast<uint32_t>(nseg * lay.cap_s);
         b.path = f4s(slots);
         b.hits = f4s(slots);
         for (int q = 0; q < 2; ++q) b.eq[q] = f4s(nseg * lay.cap_e * 4);
-        for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = f4s(nseg * lay.cap_m * 4);
+        for (int q = 0; q < kNumShadeQueues; ++q) b.mq[q] = u32s(nseg * lay.cap_m);
         b.sq = f4s(nseg * lay.cap_s * 3);
         char* c = counters.as<char>();
         b.counts = reinterpret_cast<unsigned*>(c);

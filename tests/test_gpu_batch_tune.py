@@ -92,14 +92,14 @@ def test_queue_split_keeps_the_image(ctx, name):
 
 
 def test_default_paths_in_flight_follow_device_memory(ctx):
-    # the largest power of two up to 2^27 whose 528 bytes a slot fit in a quarter of the memory
+    # the largest power of two up to 2^27 whose 348 bytes a slot fit in a quarter of the memory
     mem = ctx.device_info().total_memory
     ctx.set_tuning(path_slots=0)
     want = 1 << 27
-    while want > 4096 and want * 528 > mem // 4:
+    while want > 4096 and want * 348 > mem // 4:
         want //= 2
     assert ctx.tuning().path_slots == want
-    if mem >= 4 * 528 * (1 << 27):          # an MI355X (288 GB): a 1080p 64-spp frame in flight at once
+    if mem >= 4 * 348 * (1 << 27):          # an MI355X (288 GB): a 1080p 64-spp frame in flight at once
         assert want == 1 << 27
 
 
