@@ -16,8 +16,9 @@ Also reported:
                 per ray (64 B per inner-node visit and 48 B per triangle test of the binary tree,
                 counted on the same BVH and rays in tests/golden/traversal_counts.json): a request
                 rate that caches serve mostly, so it can exceed HBM peak. "issue" names the binding
-                limit: the texture-address unit's busy fraction (memory-instruction issue) from the
-                committed counter summary. "copy_gbs" is a device-to-device copy timed here.
+                limit: the busier of the VALU (vector ALU instruction issue) and the texture-address
+                unit (vector memory instruction issue), with both busy fractions, from the committed
+                counter summary. "copy_gbs" is a device-to-device copy timed here.
   cpu_baseline  the CPU oracle (C++ restatement of sol's path tracer, OpenMP) on the host's cores,
                 on a bounded sample of the same frame (all pixels; by default all 64 samples, about
                 11 s on 16 threads, and then the oracle's image is also compared with the wavefront
@@ -83,6 +84,15 @@ def read_key(path, key):
             if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     return None
+
+
+def issue_limit(ta_busy, valu_busy):
+    """The instruction-issue limit that binds k_trace: the busier of the two units."""
+    if ta_busy is None or valu_busy is None:
+        return None
+    if valu_busy >= ta_busy:
+        return "VALU (vector ALU instruction issue)"
+    return "texture-address unit (vector memory instruction issue)"
 
 
 def read_traffic():
@@ -529,7 +539,7 @@ def main():
                                  "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4),
                                  "model": "survey bytes per ray: 32 ray + 16 / 4 result + 64 per binary inner-node "
                                           "visit + 48 per triangle test; served mostly by LDS, L2 and MALL"},
-                     "issue": {"limit": "texture-address unit (vector memory instruction issue)",
+                     "issue": {"limit": issue_limit(ta_busy, valu_busy),
                                "ta_busy_frac": ta_busy, "valu_busy_frac": valu_busy,
                                "source": os.path.relpath(COUNTERS, ROOT)},
                      "avg_launch_ms": round(launch_ms, 4),

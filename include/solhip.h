@@ -88,7 +88,10 @@ typedef struct solhip_tuning {
                                device memory, 2^27 on a 288 GB MI355X)                      */
     uint32_t tile_w;        /* batch: tile shape of the global work queue                    */
     uint32_t tile_h;
-    uint32_t graph_iters;   /* bounce-loop iterations per captured graph                     */
+    uint32_t graph_iters;   /* bounce-loop iterations per captured graph (a render whose
+                               chunks all fit in the paths in flight captures max_path_len + 1
+                               iterations, rounded up to even, instead: one replay finishes
+                               each chunk)                                                  */
     uint32_t bvh_width;     /* 2 or 4: node layout of the traversal in the wavefront renderer,
                                the megakernel, the adjoint replay and the intersection kernel
                                (4 needs the scene's 4-wide layout; scenes without one use the

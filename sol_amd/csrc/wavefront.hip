@@ -24,6 +24,9 @@ namespace {
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Longest graph (bounce iterations) captured to finish a chunk that fits in the paths in flight.
+constexpr uint32_t kMaxFillIters = 64;
+
 struct Job {
     uint32_t scene;
     uint32_t w, h;
@@ -367,7 +370,15 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     if (L.wide && 3 * (max_depth4 > 0 ? max_depth4 - 1 : 0) > static_cast<uint32_t>(L.lds_stack + ctx.scratch.spill_entries))
         return set_error(SOLHIP_E_TOO_DEEP, "4-wide BVH deeper than the traversal stack");
 
-    const int iters = static_cast<int>(std::max<uint32_t>(2, tu.graph_iters & ~1u));
+    int iters = static_cast<int>(std::max<uint32_t>(2, tu.graph_iters & ~1u));
+    // When every chunk fits in the paths in flight, its start fill takes all of its items and no
+    // path restarts: each path ends within max_path_len bounces and the shadow rays of its last
+    // bounce are traced one iteration later, so max_path_len + 1 iterations (rounded up to even)
+    // finish a chunk in one replay, with no empty iterations after it.
+    const bool one_fill = std::all_of(chunks.begin(), chunks.end(),
+                                      [&](const Chunk& c) { return c.end - c.begin <= want_slots; });
+    const uint64_t need = (static_cast<uint64_t>(cfg.max_path_len) + 2) & ~1ull;
+    if (one_fill && need <= kMaxFillIters) iters = static_cast<int>(std::max<uint64_t>(2, need));
     // Profiling records HIP events between the kernels of every iteration. Events recorded by a
     // graph cannot be timed on this runtime, so a profiled run launches the same iterations
     // eagerly; otherwise the loop replays the captured graph.
