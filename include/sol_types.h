@@ -141,7 +141,9 @@ typedef struct sol_gpu_node {
 
 /* 4-wide GPU node (128 bytes): the boxes of up to four children, each coordinate as an array over
    the children, and the child references (child[i] >= 0: inner node index, < 0: leaf encoded as in
-   sol_gpu_node, SOL_GPU_EMPTY: unused slot, whose box is never tested). */
+   sol_gpu_node, SOL_GPU_EMPTY: unused slot). An unused slot holds the inverted box lo = +inf,
+   hi = -inf, which every ray misses, so the traversal tests its box like any other;
+   solhip_scene_upload writes that box into every unused slot it is given. */
 #define SOL_GPU_EMPTY ((int32_t)0x7FFFFFFF)
 typedef struct sol_gpu_node4 {
     float lo_x[4];

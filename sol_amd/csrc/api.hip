@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -306,8 +307,17 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     s->lights_host.assign(d->lights, d->lights + d->num_lights);
     HIP_TRY(hipMalloc(&s->nodes, sizeof(sol_gpu_node) * d->num_nodes));
     if (d->num_nodes4) {
+        // unused slots get the inverted box, which every ray misses (the traversal tests it)
+        std::vector<sol_gpu_node4> n4(d->nodes4, d->nodes4 + d->num_nodes4);
+        const float inf = std::numeric_limits<float>::infinity();
+        for (sol_gpu_node4& g : n4)
+            for (int c = 0; c < 4; ++c)
+                if (g.child[c] == SOL_GPU_EMPTY) {
+                    g.lo_x[c] = g.lo_y[c] = g.lo_z[c] = inf;
+                    g.hi_x[c] = g.hi_y[c] = g.hi_z[c] = -inf;
+                }
         HIP_TRY(hipMalloc(&s->nodes4, sizeof(sol_gpu_node4) * d->num_nodes4));
-        HIP_TRY(hipMemcpy(s->nodes4, d->nodes4, sizeof(sol_gpu_node4) * d->num_nodes4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->nodes4, n4.data(), sizeof(sol_gpu_node4) * d->num_nodes4, hipMemcpyHostToDevice));
     }
     HIP_TRY(hipMalloc(&s->tris, sizeof(sol_gpu_tri) * d->num_tris));
     HIP_TRY(hipMalloc(&s->shade, sizeof(sol_gpu_shade) * d->num_tris));

@@ -480,7 +480,8 @@ __global__ void k_wide_write(const int* F, const LevelState* st, LevelState* st_
     sol_gpu_node4 g;
     for (int i = 0; i < 4; ++i) {
         if (i >= c) {
-            g.lo_x[i] = g.hi_x[i] = g.lo_y[i] = g.hi_y[i] = g.lo_z[i] = g.hi_z[i] = 0.0f;
+            g.lo_x[i] = g.lo_y[i] = g.lo_z[i] = __builtin_huge_valf();   // the inverted box: never hit
+            g.hi_x[i] = g.hi_y[i] = g.hi_z[i] = -__builtin_huge_valf();
             g.child[i] = SOL_GPU_EMPTY;
             g.pad[i] = 0;
             continue;
@@ -520,10 +521,11 @@ __global__ void k_single_leaf(int root, uint32_t n, const float4* nlo, const flo
     g.pad[0] = g.pad[1] = 0;
     out[0] = g;
     sol_gpu_node4 w;
+    const float inf = __builtin_huge_valf();   // unused slots: the inverted box, never hit
     for (int i = 0; i < 4; ++i) {
-        w.lo_x[i] = i == 0 ? l.x : 0.0f; w.hi_x[i] = i == 0 ? h.x : 0.0f;
-        w.lo_y[i] = i == 0 ? l.y : 0.0f; w.hi_y[i] = i == 0 ? h.y : 0.0f;
-        w.lo_z[i] = i == 0 ? l.z : 0.0f; w.hi_z[i] = i == 0 ? h.z : 0.0f;
+        w.lo_x[i] = i == 0 ? l.x : inf; w.hi_x[i] = i == 0 ? h.x : -inf;
+        w.lo_y[i] = i == 0 ? l.y : inf; w.hi_y[i] = i == 0 ? h.y : -inf;
+        w.lo_z[i] = i == 0 ? l.z : inf; w.hi_z[i] = i == 0 ? h.z : -inf;
         w.child[i] = i == 0 ? ref : SOL_GPU_EMPTY;
         w.pad[i] = 0;
     }

@@ -396,14 +396,11 @@ SOL_DEV void test_node4(const RayPre& r, const Node4& n, float tmax, float& t0, 
     h1 = intersect_box_nf(r, n.near_x.y, n.far_x.y, n.near_y.y, n.far_y.y, n.near_z.y, n.far_z.y, tmax, t1);
     h2 = intersect_box_nf(r, n.near_x.z, n.far_x.z, n.near_y.z, n.far_y.z, n.near_z.z, n.far_z.z, tmax, t2);
     h3 = intersect_box_nf(r, n.near_x.w, n.far_x.w, n.near_y.w, n.far_y.w, n.near_z.w, n.far_z.w, tmax, t3);
+    // an unused slot (SOL_GPU_EMPTY) holds the inverted box: every slab has near +inf, far -inf
     c0 = n.child.x;
     c1 = n.child.y;
     c2 = n.child.z;
     c3 = n.child.w;
-    h0 = h0 && c0 != SOL_GPU_EMPTY;
-    h1 = h1 && c1 != SOL_GPU_EMPTY;
-    h2 = h2 && c2 != SOL_GPU_EMPTY;
-    h3 = h3 && c3 != SOL_GPU_EMPTY;
 }
 
 // Pushes the k (0..3) farther hit children of a 4-wide node, farthest first: c3 (k == 3), c2

@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cmath>
 #include <deque>
+#include <limits>
 #include <unordered_map>
 #include <numeric>
 #include <queue>
@@ -432,9 +433,14 @@ GpuLayout4 to_gpu_layout4(const Bvh& bvh, uint32_t top_levels) {
         g.lo_z[slot] = b.lo.z; g.hi_z[slot] = b.hi.z;
         g.child[slot] = ref;
     };
-    auto empty_node = [] {
+    auto empty_node = [] {   // every slot unused: the inverted box, which no ray hits
         sol_gpu_node4 g{};
-        for (int k = 0; k < 4; ++k) g.child[k] = SOL_GPU_EMPTY;
+        const float inf = std::numeric_limits<float>::infinity();
+        for (int k = 0; k < 4; ++k) {
+            g.lo_x[k] = g.lo_y[k] = g.lo_z[k] = inf;
+            g.hi_x[k] = g.hi_y[k] = g.hi_z[k] = -inf;
+            g.child[k] = SOL_GPU_EMPTY;
+        }
         return g;
     };
     const Node& root = bvh.nodes[0];

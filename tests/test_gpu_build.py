@@ -90,6 +90,10 @@ def check_structure(scene, built):
     for k, g in enumerate(d["nodes4"]):
         used = [c for c in g.child if c != F.GPU_EMPTY]
         assert used
+        for s in range(4):   # unused slots: the inverted box, which every ray misses
+            if g.child[s] == F.GPU_EMPTY:
+                assert g.lo_x[s] == g.lo_y[s] == g.lo_z[s] == np.inf
+                assert g.hi_x[s] == g.hi_y[s] == g.hi_z[s] == -np.inf
         for c in used:
             if c >= 0:
                 assert k < c < len(d["nodes4"])

@@ -100,7 +100,8 @@ def test_gpu_layout4_covers_every_leaf_and_bounds_its_subtrees(make):
         used = 0
         for k in range(4):
             c = int(child[i, k])
-            if c == F.GPU_EMPTY:
+            if c == F.GPU_EMPTY:   # an unused slot holds the inverted box, which every ray misses
+                assert np.all(boxes[i, [0, 2, 4], k] == np.inf) and np.all(boxes[i, [1, 3, 5], k] == -np.inf)
                 continue
             used += 1
             lo = boxes[i, [0, 2, 4], k]
