@@ -166,6 +166,17 @@ SOL_DEV RoundPlan plan_round(RoundLds<NQ>& lds, unsigned* const* qcounters, unsi
     return p;
 }
 
+// Whether any segment still has work items to hand out. Item counters only grow, so once this is
+// false in a launch it stays false for the rest of it: paths that end then reserve no extension
+// entry (they would only become holes).
+SOL_DEV bool items_left(const WaveBuf& b) {
+    bool left = false;
+    for (unsigned k = 0; k < b.nseg; ++k)
+        left = left || __hip_atomic_load(&b.items[item_next(static_cast<int>(k))], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) < b.items[item_end(static_cast<int>(k))];
+    return left;
+}
+
 // A retired lane whose extension entry stays unused marks it as a hole (trace and sort skip it).
 SOL_DEV void write_hole(const WaveBuf& b, int nxt, unsigned entry) {
     *b.ext(nxt, entry, 1) = make_float4(0.0f, 0.0f, 0.0f, as_f(kHole));
@@ -516,6 +527,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     unsigned* qcnt[kNumShadeQueues] = {&b.counts[qcount(kQShade0, seg)], &b.counts[qcount(kQShade0 + 1, seg)],
                                        &b.counts[qcount(kQShade0 + 2, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
+    const bool restart = items_left(b);   // uniform
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
         unsigned i = round * kBlock + threadIdx.x;
@@ -541,12 +553,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 if (dead) retire(b, *b.rec(slot), item_of(*b.ext(cur, ea, 3)));
             }
         }
-        RoundPlan plan = plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, nseg, dest, false, dead);
+        RoundPlan plan =
+            plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, nseg, dest, false, dead && restart);
         if (plan.pos >= 0) *b.shade_entry(dest, seg * b.cap_m + static_cast<unsigned>(plan.pos)) = i;
         if (plan.start) {
             ++started;
             init_path(tabs, b, plan.item, slot, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
-        } else if (dead) {
+        } else if (plan.ext_pos >= 0) {
             write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
         }
     }
@@ -577,6 +590,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
     const int seg = my_segment(b), nseg = static_cast<int>(b.nseg);
     unsigned* scnt[1] = {&b.counts[qcount(kQShadow, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
+    const bool restart = items_left(b);   // uniform
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
         unsigned j = round * kBlock + threadIdx.x;
@@ -643,7 +657,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
             }
             if (!cont) retire(b, *b.rec(slot), item);
         }
-        bool retired = valid && !cont;
+        bool retired = valid && !cont && restart;
         RoundPlan plan = plan_round<1>(lds[parity], scnt, ecnt, b.items, seg, nseg, shoot ? 0 : -1, cont, retired);
         if (plan.pos >= 0) {
             const unsigned a = seg * b.cap_s + static_cast<unsigned>(plan.pos);
