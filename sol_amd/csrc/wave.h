@@ -107,6 +107,9 @@ struct WaveBuf {
     uint32_t slots;
     uint32_t nseg;                    // segments per queue (1, 2, 4 or 8)
     uint32_t merged;                  // 1: one shading queue (queue 0) for every BSDF
+    uint32_t in_place;                // 1: every item of the chunk has a path in the start fill, and
+                                      // a path's record is its item's sample entry (path = samples,
+                                      // slot = item - chunk begin): an ended path needs no copy
     uint32_t cap_e, cap_m, cap_s;     // segment capacities: extension, shade, shadow queues
     uint32_t stride_e, stride_s;      // nseg * cap_e, cap_s (the queues' field strides)
     float4* path;                     // [slots]: slot records

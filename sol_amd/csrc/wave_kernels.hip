@@ -2,9 +2,11 @@
 //   k_trace  closest hit of the extension rays (hit written in place) and any hit of the previous
 //            bounce's shadow rays (unoccluded contribution added to the path or its sample);
 //            traversal only, work fetched by waves in shrinking chunks (guided self-scheduling).
-//   k_sort   hit decoding, emission, material queues; ended paths write their sample and the slot
-//            starts the next work item.
-//   k_shade  one kernel per material queue: NEE, Russian roulette, BSDF sampling.
+//   k_sort   material queues from each hit triangle's kind byte; a path that missed ends there (its
+//            radiance goes to its sample entry, unless the chunk runs in place) and its slot starts
+//            the next work item.
+//   k_shade  one kernel per material queue: hit decoding, emission, NEE, Russian roulette, BSDF
+//            sampling.
 //   k_start  fills every slot at the beginning of a chunk of work.
 //   k_reduce per-pixel sums of the chunk's samples in sample order.
 // Appends are compacted per workgroup: ballots give each lane its rank in its wave, the wave
@@ -411,6 +413,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                     const unsigned a = eqv.entry(e);
                     float4 e1 = *b.ext(cur, a, 1);
                     uint32_t slot = as_u(e1.w);
+                    // a hole's hit reads as a miss (the sort of an in-place chunk reads only hits)
+                    if (slot == kHole) b.hits[e] = make_float4(as_f(0xFFFFFFFFu), 0.0f, 0.0f, 0.0f);
                     if (slot != kHole) {
                         idx = static_cast<int>(e);
                         any = false;
@@ -538,11 +542,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         unsigned ea = 0;
         if (valid) {
             ea = eqv.entry(i);
-            const float4 h = b.hits[i];           // a hole's is in bounds and unused
-            slot = as_u(b.ext(cur, ea, 1)->w);
-            valid = slot != kHole;
+            const int tri = static_cast<int>(as_u(b.hits[i].x));   // a hole's reads as a miss
+            // the slot word: for the scene of a multi-scene path, or to retire an ending path (an
+            // in-place chunk's paths end where they are)
+            if (MULTI || !b.in_place) {
+                slot = as_u(b.ext(cur, ea, 1)->w);
+                valid = slot != kHole;
+            }
             if (valid) {
-                const int tri = static_cast<int>(as_u(h.x));
                 uint32_t kind = 0;
                 if (tri >= 0) {
                     SceneView tmp;
@@ -550,7 +557,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 }
                 if (kind == 0) dead = true;   // a miss, or no BSDF and no light
                 else dest = b.merged ? 0 : queue_of_kind(kind);
-                if (dead) retire(b, *b.rec(slot), item_of(*b.ext(cur, ea, 3)));
+                if (dead && !b.in_place) retire(b, *b.rec(slot), item_of(*b.ext(cur, ea, 3)));
             }
         }
         RoundPlan plan =
@@ -655,7 +662,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
                     rng_next = rng.state;
                 }
             }
-            if (!cont) retire(b, *b.rec(slot), item);
+            if (!cont && !b.in_place) retire(b, *b.rec(slot), item);
         }
         bool retired = valid && !cont && restart;
         RoundPlan plan = plan_round<1>(lds[parity], scnt, ecnt, b.items, seg, nseg, shoot ? 0 : -1, cont, retired);
@@ -714,8 +721,10 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
             if (item < b.items[item_next(static_cast<int>(seg))]) {
                 ++started;
                 const PathStart ps = start_path(tabs, item);
-                *b.rec(slot) = make_float4(0.0f, 0.0f, 0.0f, as_f(ps.scene));
-                *b.ext(nxt, entry, 1) = f4(ps.dir, as_f(slot));
+                // an in-place chunk names its paths by their items: the record is the sample entry
+                const unsigned id = b.in_place ? static_cast<unsigned>(item - b.items[kItemChunkBegin]) : slot;
+                *b.rec(id) = make_float4(0.0f, 0.0f, 0.0f, as_f(ps.scene));
+                *b.ext(nxt, entry, 1) = f4(ps.dir, as_f(id));
                 *b.ext(nxt, entry, 3) = rng_item_word(ps.rng, item);
             } else {
                 write_hole(b, nxt, entry);
@@ -783,7 +792,7 @@ __global__ __launch_bounds__(256) void k_reduce(WaveTables tabs, WaveBuf b, uint
     float r = 0.0f, g = 0.0f, bl = 0.0f;
     for (unsigned long long s = 0; s < spp; ++s) {
         float4 v = b.samples[k0 + s * per_sample + pix];
-        if (sout) sout[s] = v;
+        if (sout) sout[s] = make_float4(v.x, v.y, v.z, 0.0f);   // (an in-place record holds its scene in w)
         r = r + v.x;
         g = g + v.y;
         bl = bl + v.z;

@@ -379,6 +379,13 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
                                       [&](const Chunk& c) { return c.end - c.begin <= want_slots; });
     const uint64_t need = (static_cast<uint64_t>(cfg.max_path_len) + 2) & ~1ull;
     if (one_fill && need <= kMaxFillIters) iters = static_cast<int>(std::max<uint64_t>(2, need));
+    // Such a chunk also runs in place: each path keeps one work item from start to end, so its
+    // record (radiance, scene) is its item's entry of the sample buffer, named by the item's
+    // offset in the chunk, and a path that ends needs no copy.
+    if (one_fill) {
+        L.buf.in_place = 1;
+        L.buf.path = L.buf.samples;
+    }
     // Profiling records HIP events between the kernels of every iteration. Events recorded by a
     // graph cannot be timed on this runtime, so a profiled run launches the same iterations
     // eagerly; otherwise the loop replays the captured graph.
