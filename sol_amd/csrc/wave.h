@@ -15,6 +15,10 @@ namespace solhip {
 
 // Workgroup size of the sort, shade and start kernels.
 constexpr int kWaveBlock = 512;
+// Entries per lane and round of the sort: its rounds hold kWaveSortItems x kWaveBlock entries.
+constexpr int kWaveSortItems = 4;
+// The largest round of any pass (entries appended by one workgroup between two reservations).
+constexpr int kWaveRound = kWaveSortItems * kWaveBlock;
 
 enum : int {
     kQueueDiffuse = 0,
@@ -97,10 +101,10 @@ enum : int {
 //                             target = slot, or kSampleTarget | 0 when the path has ended (the
 //                             contribution then goes to the sample entry of the item)
 //
-// Segment capacities (entries), for a launch of n input entries handed out in rounds of R =
-// kWaveBlock by static round robin over a grid that is a multiple of kMaxSegments: segment s gets
-// the rounds r with r % kMaxSegments == s, at most n / kMaxSegments + 2 R entries per launch and
-// output queue. Shade queues and the shadow queue receive from launches whose inputs sum to at
+// Segment capacities (entries), for a launch of n input entries handed out in rounds of at most
+// R = kWaveRound entries by static round robin over a grid that is a multiple of kMaxSegments:
+// segment s of nseg gets the rounds r with r % nseg == s, at most n / nseg + 2 R entries per
+// launch and output queue. Shade queues and the shadow queue receive from launches whose inputs sum to at
 // most `slots`; an extension queue receives from sort and the three shade launches, whose inputs
 // sum to at most 2 slots.
 struct WaveBuf {

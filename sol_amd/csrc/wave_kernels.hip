@@ -32,6 +32,7 @@ constexpr uint32_t kFrontBit = 1u << 30;
 constexpr uint32_t kSampleTarget = 0xFFFFFFFFu;
 constexpr int kBlock = kWaveBlock;
 constexpr uint32_t kHole = 0xFFFFFFFFu;     // slot field of an unused extension entry
+constexpr int kSortItems = kWaveSortItems;  // entries per lane and round of the sort
 constexpr int kWaves = kBlock / 64;
 
 SOL_DEV uint32_t as_u(float f) { return __float_as_uint(f); }
@@ -83,8 +84,8 @@ struct RoundPlan {
 // Reserves up to `need` work items, own segment first (items[item_next(s)] counts up to
 // items[item_end(s)]), then the others of the nseg segments. Counters only grow, so a relaxed
 // read at or past the end means the segment is exhausted and it is skipped without an atomic.
-template <int NQ>
-SOL_DEV void grant_items(RoundLds<NQ>& lds, unsigned long long* items, int seg, int nseg, unsigned need) {
+template <typename Lds>
+SOL_DEV void grant_items(Lds& lds, unsigned long long* items, int seg, int nseg, unsigned need) {
     unsigned n = 0;
     for (int k = 0; k < nseg && need > 0; ++k) {
         int sk = (seg + k) & (nseg - 1);
@@ -166,6 +167,84 @@ SOL_DEV RoundPlan plan_round(RoundLds<NQ>& lds, unsigned* const* qcounters, unsi
     p.ext_pos = cont ? static_cast<int>(lds.ext_base + lds.pre[NQ][wave] + rank_c)
                      : (retire ? static_cast<int>(lds.ext_base + lds.n_cont + ridx) : -1);
     return p;
+}
+
+// A round of K entries per lane (K x kBlock entries, entry k of lane t at k x kBlock + t): the
+// same reservations as plan_round - plain queues by `dest`, an extension entry and a work item per
+// retiring entry - with one device atomic per non-empty queue for all K x kBlock entries, so a
+// latency-bound pass pays its barriers and atomics once per K entries. Positions follow the order
+// (k, wave, lane). No entry continues a path (the sort's case).
+template <int NQ, int K>
+struct RoundLdsK {
+    unsigned wc[K][NQ + 1][kWaves];    // per-wave counts: plain queues, retire
+    unsigned pre[K][NQ + 1][kWaves];   // their exclusive prefix sums over (k, wave)
+    unsigned base[NQ];
+    unsigned ext_base;
+    unsigned n_pieces;
+    unsigned piece_count[kMaxSegments];
+    unsigned long long piece_base[kMaxSegments];
+};
+
+template <int NQ, int K>
+SOL_DEV void plan_round_k(RoundLdsK<NQ, K>& lds, unsigned* const* qcounters, unsigned* ext_counter,
+                          unsigned long long* items, int seg, int nseg, const int (&dest)[K],
+                          const bool (&retire)[K], RoundPlan (&plan)[K]) {
+    const int wave = threadIdx.x >> 6;
+    const bool lane0 = __lane_id() == 0;
+    const unsigned long long below = lanes_below();
+    unsigned rank[K], rank_r[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        rank[k] = 0;
+#pragma unroll
+        for (int d = 0; d < NQ; ++d) {
+            const unsigned long long m = __ballot(dest[k] == d);
+            if (lane0) lds.wc[k][d][wave] = static_cast<unsigned>(__popcll(m));
+            if (dest[k] == d) rank[k] = static_cast<unsigned>(__popcll(m & below));
+        }
+        const unsigned long long mr = __ballot(retire[k]);
+        if (lane0) lds.wc[k][NQ][wave] = static_cast<unsigned>(__popcll(mr));
+        rank_r[k] = static_cast<unsigned>(__popcll(mr & below));
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ + 1) {
+        const int d = threadIdx.x;
+        unsigned run = 0;
+        for (int k = 0; k < K; ++k)
+            for (int w = 0; w < kWaves; ++w) {
+                lds.pre[k][d][w] = run;
+                run += lds.wc[k][d][w];
+            }
+        if (d < NQ) {
+            lds.base[d] = run ? atomicAdd(qcounters[d], run) : 0u;
+        } else {
+            lds.ext_base = run ? atomicAdd(ext_counter, run) : 0u;
+            grant_items(lds, items, seg, nseg, run);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        RoundPlan& p = plan[k];
+        p.pos = dest[k] >= 0 ? static_cast<int>(lds.base[dest[k]] + lds.pre[k][dest[k]][wave] + rank[k]) : -1;
+        p.start = false;
+        p.item = 0;
+        p.ext_pos = -1;
+        if (retire[k]) {
+            unsigned off = lds.pre[k][NQ][wave] + rank_r[k];
+            p.ext_pos = static_cast<int>(lds.ext_base + off);
+            const unsigned np = lds.n_pieces;
+            for (unsigned q = 0; q < np; ++q) {
+                const unsigned c = lds.piece_count[q];
+                if (off < c) {
+                    p.start = true;
+                    p.item = lds.piece_base[q] + off;
+                    break;
+                }
+                off -= c;
+            }
+        }
+    }
 }
 
 // Whether any segment still has work items to hand out. Item counters only grow, so once this is
@@ -523,7 +602,8 @@ SOL_DEV int queue_of_kind(uint32_t kind) {
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sort(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
                                                  int cur) {
-    __shared__ RoundLds<kNumShadeQueues> lds[2];
+    constexpr int K = kSortItems;
+    __shared__ RoundLdsK<kNumShadeQueues, K> lds[2];
     const SegQueue eqv(b, kQExtend0 + cur, b.cap_e);
     const unsigned count = eqv.size();
     const int nxt = 1 - cur;
@@ -532,42 +612,53 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                                        &b.counts[qcount(kQShade0 + 2, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
     const bool restart = items_left(b);   // uniform
+    // the slot word: for the scene of a multi-scene path, or to retire an ending path (an in-place
+    // chunk's paths end where they are)
+    const bool need_slot = MULTI || !b.in_place;
     unsigned started = 0, parity = 0;
-    for (unsigned round = blockIdx.x; round * kBlock < count; round += gridDim.x, parity ^= 1) {
-        unsigned i = round * kBlock + threadIdx.x;
-        bool valid = i < count;
-        int dest = -1;
-        bool dead = false;
-        uint32_t slot = 0;
-        unsigned ea = 0;
-        if (valid) {
-            ea = eqv.entry(i);
-            const int tri = static_cast<int>(as_u(b.hits[i].x));   // a hole's reads as a miss
-            // the slot word: for the scene of a multi-scene path, or to retire an ending path (an
-            // in-place chunk's paths end where they are)
-            if (MULTI || !b.in_place) {
-                slot = as_u(b.ext(cur, ea, 1)->w);
-                valid = slot != kHole;
-            }
+    for (unsigned round = blockIdx.x; round * (K * kBlock) < count; round += gridDim.x, parity ^= 1) {
+        int dest[K];
+        bool dead[K], ret[K];
+        uint32_t slot[K];
+        unsigned ea[K];
+        int tri[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {   // every load of the round first
+            const unsigned i = (round * K + k) * kBlock + threadIdx.x;
+            const bool valid = i < count;
+            ea[k] = valid ? eqv.entry(i) : 0u;
+            tri[k] = valid ? static_cast<int>(as_u(b.hits[i].x)) : -2;   // a hole's reads as a miss
+            slot[k] = valid && need_slot ? as_u(b.ext(cur, ea[k], 1)->w) : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const bool valid = tri[k] != -2 && !(need_slot && slot[k] == kHole);
+            dest[k] = -1;
+            dead[k] = false;
             if (valid) {
                 uint32_t kind = 0;
-                if (tri >= 0) {
+                if (tri[k] >= 0) {
                     SceneView tmp;
-                    kind = scene_of<MULTI>(single, tabs, b, slot, tmp).kinds[tri];
+                    kind = scene_of<MULTI>(single, tabs, b, slot[k], tmp).kinds[tri[k]];
                 }
-                if (kind == 0) dead = true;   // a miss, or no BSDF and no light
-                else dest = b.merged ? 0 : queue_of_kind(kind);
-                if (dead && !b.in_place) retire(b, *b.rec(slot), item_of(*b.ext(cur, ea, 3)));
+                if (kind == 0) dead[k] = true;   // a miss, or no BSDF and no light
+                else dest[k] = b.merged ? 0 : queue_of_kind(kind);
+                if (dead[k] && !b.in_place) retire(b, *b.rec(slot[k]), item_of(*b.ext(cur, ea[k], 3)));
             }
+            ret[k] = dead[k] && restart;
         }
-        RoundPlan plan =
-            plan_round<kNumShadeQueues>(lds[parity], qcnt, ecnt, b.items, seg, nseg, dest, false, dead && restart);
-        if (plan.pos >= 0) *b.shade_entry(dest, seg * b.cap_m + static_cast<unsigned>(plan.pos)) = i;
-        if (plan.start) {
-            ++started;
-            init_path(tabs, b, plan.item, slot, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
-        } else if (plan.ext_pos >= 0) {
-            write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan.ext_pos));
+        RoundPlan plan[K];
+        plan_round_k<kNumShadeQueues, K>(lds[parity], qcnt, ecnt, b.items, seg, nseg, dest, ret, plan);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const unsigned i = (round * K + k) * kBlock + threadIdx.x;
+            if (plan[k].pos >= 0) *b.shade_entry(dest[k], seg * b.cap_m + static_cast<unsigned>(plan[k].pos)) = i;
+            if (plan[k].start) {
+                ++started;
+                init_path(tabs, b, plan[k].item, slot[k], nxt, seg * b.cap_e + static_cast<unsigned>(plan[k].ext_pos));
+            } else if (plan[k].ext_pos >= 0) {
+                write_hole(b, nxt, seg * b.cap_e + static_cast<unsigned>(plan[k].ext_pos));
+            }
         }
     }
     wave_add_stat(&b.stats[2], started);

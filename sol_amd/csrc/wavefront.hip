@@ -60,7 +60,7 @@ struct WavefrontState {
     }
 
     // Segment capacities (see WaveBuf) of n slots in nseg segments: extension queues
-    // 2 n / nseg + 8 R, shade queues n / nseg + 2 R, shadow queue n / nseg + 6 R, R = kWaveBlock,
+    // 2 n / nseg + 8 R, shade queues n / nseg + 2 R, shadow queue n / nseg + 6 R, R = kWaveRound,
     // rounded up to 64 entries. The bounds hold for the material-split queues (three shade
     // launches) and so for the merged queue (one).
     struct Layout {
@@ -73,7 +73,7 @@ struct WavefrontState {
         }
     };
     static Layout layout(size_t n, size_t nseg) {
-        const size_t R = kWaveBlock;
+        const size_t R = kWaveRound;
         auto round64 = [](size_t v) { return (v + 63) & ~size_t(63); };
         return {round64(2 * n / nseg + 8 * R), round64(n / nseg + 2 * R), round64(n / nseg + 6 * R)};
     }
