@@ -98,7 +98,10 @@ typedef struct solhip_tuning {
                                binary).                                                     */
     uint32_t chunk_items;   /* wavefront: work items (path samples) per chunk of work, the
                                capacity of the per-sample radiance buffer (16 B an item);
-                               65536..2^28, default 192 Mi (3 GiB). Whole tiles make a chunk. */
+                               65536..2^28. Whole tiles make a chunk. Default: path_slots
+                               (at least 65536, at most 192 Mi), so that every chunk fits in
+                               the paths in flight and runs in place (each path keeps one
+                               item, its record is the item's sample entry).                 */
     uint32_t queue_segments;/* wavefront queue split: segments per queue, 1 (default), 2, 4 or
                                8. Each segment has its own length counter, so the appends of
                                a pass spread their device atomics over that many addresses;

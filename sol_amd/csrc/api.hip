@@ -192,7 +192,7 @@ int solhip_set_tuning(solhip_ctx* ctx, const solhip_tuning* t) {
     if (!n.tile_h) n.tile_h = d.tile_h;
     if (!n.graph_iters) n.graph_iters = d.graph_iters;
     if (!n.bvh_width) n.bvh_width = d.bvh_width;
-    if (!n.chunk_items) n.chunk_items = d.chunk_items;
+    if (!n.chunk_items) n.chunk_items = default_chunk_items(n.path_slots);
     if (!n.queue_segments) n.queue_segments = d.queue_segments;
     if (!n.shade_queues) n.shade_queues = d.shade_queues;
     if (n.chunk_items < (1u << 16) || n.chunk_items > (1u << 28)) return fail(SOLHIP_E_INVALID, "chunk_items out of range");

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -51,8 +52,15 @@ inline uint32_t default_path_slots(unsigned long long device_memory) {
     return s;
 }
 
-// Work items per chunk by default: the per-sample radiance buffer holds 192 Mi items (3 GiB).
+// Work items per chunk at most by default (the per-sample radiance buffer, 16 B an item: 3 GiB).
 constexpr uint32_t kDefaultChunkItems = 192u << 20;
+
+// Default chunk size for `path_slots` paths in flight: as many items as paths (within the bounds
+// of chunk_items), so that every chunk fits in the start fill and runs in place - its paths keep
+// one item each and their records are the sample entries.
+inline uint32_t default_chunk_items(uint32_t path_slots) {
+    return std::min<uint32_t>(kDefaultChunkItems, std::max<uint32_t>(path_slots, 1u << 16));
+}
 
 inline solhip_tuning default_tuning(unsigned long long device_memory) {
     solhip_tuning t{};
@@ -63,7 +71,7 @@ inline solhip_tuning default_tuning(unsigned long long device_memory) {
     t.tile_h = 32;
     t.graph_iters = 16;
     t.bvh_width = 4;
-    t.chunk_items = kDefaultChunkItems;
+    t.chunk_items = default_chunk_items(t.path_slots);
     t.queue_segments = 1;
     t.shade_queues = 3;
     return t;

@@ -51,11 +51,14 @@ JOBS = [
 SHARED = {5: 0}
 
 
+# Chunks larger than the paths in flight restart paths from the work items; chunks that fit (chunk
+# 0: as many items as paths) run in place, several of them here.
 @pytest.mark.parametrize("slots,chunk,tiles,segments,shade_queues", [
     (5000, 65536, (16, 8), 8, 3),
     (1 << 20, 1 << 20, (32, 32), 8, 3),
     (5000, 65536, (16, 8), 1, 1),
     (1 << 20, 1 << 20, (32, 32), 2, 1),
+    (1 << 16, 0, (16, 8), 1, 3),
 ])
 def test_batch_matches_oracle_per_job(ctx, slots, chunk, tiles, segments, shade_queues):
     scenes = _scenes()
@@ -84,8 +87,8 @@ def test_batch_matches_oracle_per_job(ctx, slots, chunk, tiles, segments, shade_
         ctx.set_tuning(path_slots=0, chunk_items=0, tile_w=0, tile_h=0, graph_iters=0, queue_segments=0,
                        shade_queues=0)
     assert st.samples == total
-    if chunk == 65536:
-        assert total > 2 * chunk                 # at least three chunks of work
+    if chunk in (0, 65536):
+        assert total > 2 * 65536                 # at least three chunks of work
     # the oracle renders each job on its own; a shared image receives its jobs in job order
     want = {}
     for k, (name, w, h, si, sc) in enumerate(JOBS):
