@@ -131,7 +131,9 @@ SOL_DEV void replay_path(const SceneView& s, const sol_pt_config& cfg, const flo
     }
 }
 
-// One lane per pixel in 8x8 tiles (as the megakernel); pixels with a zero adjoint are skipped.
+// One lane per pixel of an 8x8 tile and one sample: a wave takes (tile, sample) tasks, the samples
+// of a tile consecutively, so that the work balances at the granularity of one path per lane;
+// pixels with a zero adjoint are skipped.
 template <int BLOCK, int LDS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_config cfg, uint32_t w, uint32_t h,
                                                         uint64_t sample_index, uint32_t sample_count,
@@ -152,13 +154,15 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_conf
     Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
     GradSink g{in_lds ? lds_grad : nullptr, d_mat, d_light, mat_floats};
     const uint32_t tiles_x = (w + 7) / 8, tiles_y = (h + 7) / 8;
-    const unsigned long long total = static_cast<unsigned long long>(tiles_x) * tiles_y * 64;
+    const unsigned long long total = static_cast<unsigned long long>(tiles_x) * tiles_y * sample_count * 64;
     const size_t plane = static_cast<size_t>(w) * h;
     unsigned n_ext = 0, n_shadow = 0, n_samples = 0;
     while (true) {
         unsigned long long base = wave_grab(counter, 64);
         if (base >= total) break;
-        uint32_t tile = static_cast<uint32_t>(base / 64);
+        const unsigned long long task = base / 64;
+        const uint32_t tile = static_cast<uint32_t>(task / sample_count);
+        const uint32_t i = static_cast<uint32_t>(task % sample_count);
         uint32_t lane = __lane_id();
         uint32_t x = (tile % tiles_x) * 8 + (lane & 7);
         uint32_t y = (tile / tiles_x) * 8 + (lane >> 3);
@@ -166,15 +170,13 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(SceneView s, sol_pt_conf
         size_t p = static_cast<size_t>(y) * w + x;
         V3 delta{d_image[p], d_image[plane + p], d_image[2 * plane + p]};
         if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) continue;
-        for (uint32_t i = 0; i < sample_count; ++i) {
-            const float4 c = radiance[p * sample_count + i];
-            Pcg rng;
-            rng.seed(pixel_seed(sample_index + i, x, y));
-            V3 org, dir;
-            camera_ray(s.camera, x, y, w, h, rng, org, dir);
-            replay_path<BLOCK, LDS, WIDE>(s, cfg, lds_nodes, st, rng, org, dir, V3{c.x, c.y, c.z}, delta, g, n_ext, n_shadow);
-        }
-        n_samples += sample_count;
+        const float4 c = radiance[p * sample_count + i];
+        Pcg rng;
+        rng.seed(pixel_seed(sample_index + i, x, y));
+        V3 org, dir;
+        camera_ray(s.camera, x, y, w, h, rng, org, dir);
+        replay_path<BLOCK, LDS, WIDE>(s, cfg, lds_nodes, st, rng, org, dir, V3{c.x, c.y, c.z}, delta, g, n_ext, n_shadow);
+        n_samples += 1;
     }
     wave_add(stats + 0, n_ext);
     wave_add(stats + 1, n_shadow);
