@@ -557,7 +557,13 @@ int tune(Context& ctx, unsigned long long items, Trial trial, solhip_tuning* out
         std::vector<solhip_tuning> cands;
         solhip_tuning t = best;
         switch (dim) {
-            case 0: for (uint32_t v : slot_cands) { t.path_slots = v; cands.push_back(t); } break;
+            case 0:   // with chunks of as many items (every chunk then runs in place)
+                for (uint32_t v : slot_cands) {
+                    t.path_slots = v;
+                    t.chunk_items = default_chunk_items(v);
+                    cands.push_back(t);
+                }
+                break;
             case 1: for (uint32_t v : {128u, 256u}) { t.block_size = v; cands.push_back(t); } break;
             case 2: for (uint32_t v : {8u, 16u}) { t.lds_stack = v; cands.push_back(t); } break;
             case 3: for (uint32_t v : {8u, 16u, 32u}) { t.graph_iters = v; cands.push_back(t); } break;
