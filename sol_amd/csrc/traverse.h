@@ -457,14 +457,20 @@ SOL_DEV void order_pair(float& ta, int& ca, float& tb, int& cb) {
 // while other lanes of the wave are still looking for theirs, so that fewer lanes idle in the
 // node loop; then every lane tests the triangles of its leaf. The visiting order does not change
 // the result: the closest hit is unique by (t, original triangle index).
-template <int BLOCK, int LDS_STACK, typename Fetch>
+// With LEAF_WAIT > 0 the node loop also ends once no more than LEAF_WAIT lanes are still looking
+// for their first leaf while some lane holds one: those lanes keep their node and stack and go on
+// in the next step, instead of the whole wave running node iterations for a handful of them.
+template <int BLOCK, int LDS_STACK, int LEAF_WAIT = 0, typename Fetch>
 SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, float tmin, bool any, TraversalState& ts) {
     constexpr float kMiss = __builtin_huge_valf();
     int leaf = 0;          // postponed leaf (leaf references are negative), 0 = none
     int leaf2 = 0;         // a second postponed leaf
     int leaf3 = 0;         // a third
     bool more = true;      // ts.node still has to be visited
-    while (__ballot(more && leaf == 0) != 0) {
+    while (true) {
+        const unsigned long long searching = __ballot(more && leaf == 0);
+        if (searching == 0) break;
+        if (LEAF_WAIT > 0 && __popcll(searching) <= LEAF_WAIT && __ballot(leaf != 0) != 0) break;
         if (more && ts.node >= 0) {
             Node4 n;
             f.node(ts.node, r, n);
