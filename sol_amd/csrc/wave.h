@@ -46,16 +46,20 @@ enum : int {
     kNumQueues = 6
 };
 
+// A small launch of the traversal kernel hands its entries out from kTraceParts consecutive
+// parts, each with a work-fetch counter of its own (see trace_fetch in wave_kernels.hip).
+constexpr int kTraceParts = 8;
+
 // Device counters: the length of segment s of queue q is counter q * kMaxSegments + s, then the
-// work-fetch counter of the traversal kernel (these kNumWaveCounters are cleared for every pass),
-// then the start flag: nonzero while extension queue 0 holds the start fill of a chunk, whose
-// entries leave out their first word (the ray origin is the camera's eye, tmin 0). Counters sit
-// kCounterStride words (256 bytes) apart: device atomics on one cache line serialise, so each
-// counter gets a line of its own.
+// kTraceParts work-fetch counters of the traversal kernel (these kNumWaveCounters are cleared for
+// every pass), then the start flag: nonzero while extension queue 0 holds the start fill of a
+// chunk, whose entries leave out their first word (the ray origin is the camera's eye, tmin 0).
+// Counters sit kCounterStride words (256 bytes) apart: device atomics on one cache line
+// serialise, so each counter gets a line of its own.
 constexpr int kCounterStride = 64;
 SOL_HD constexpr int qcount(int queue, int seg) { return (queue * kMaxSegments + seg) * kCounterStride; }
 enum : int {
-    kNumWaveCounters = kNumQueues * kMaxSegments + 1,
+    kNumWaveCounters = kNumQueues * kMaxSegments + kTraceParts,
     kCntTraceWork = kNumQueues * kMaxSegments * kCounterStride,
     kCntStartFill = kNumWaveCounters * kCounterStride,
     kCounterWords = (kNumWaveCounters + 1) * kCounterStride
