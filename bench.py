@@ -372,6 +372,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the intersection and Cornell box legs")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only-cornell", action="store_true",
+                    help="run only the Cornell box leg (for kernel traces of a small frame) and print its object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. A frame renders on one MI355X; "
@@ -384,6 +386,10 @@ def main():
 
     import sol_amd
     from sol_amd import _ffi as F
+
+    if args.only_cornell:
+        print(json.dumps(bench_cornell(sol_amd, F, torch, not args.no_cpu)), flush=True)
+        return
 
     t0 = time.time()
     scene = sol_amd.Scene.sphere_field()

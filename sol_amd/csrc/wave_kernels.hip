@@ -422,37 +422,66 @@ SOL_DEV Stack make_stack_w(int* lds_stack, const TraversalScratch& sc, int lds_e
 // entries 53.7 (41.7), below 16M 53.7 (41.5).
 constexpr unsigned kTracePartsBelow = 16u << 20;
 
-// First entry of part p of `parts` of the traversal kernel's entries [0, total).
-SOL_DEV unsigned trace_part_begin(unsigned total, unsigned p, unsigned parts) {
-    return static_cast<unsigned>(static_cast<unsigned long long>(total) * p / parts);
-}
+// A wave's place in the traversal kernel's work, the entries [0, total). A launch with no more
+// entries than 64 per wave is handed out by rank: wave g takes entries [64 g, 64 g + 64) and no
+// atomic is made (every persistent wave of such a launch would otherwise make at least one on a
+// few addresses, which set a floor of about 0.1 ms under the small launches at the end of a
+// frame). Larger launches are handed out from `parts` consecutive parts by their counters; the
+// waves of workgroup b take from part b mod parts first (with eight parts, the XCD that runs them,
+// as workgroups are dealt to the XCDs round robin), then from the following parts.
+struct TraceWork {
+    unsigned total, parts;
+    unsigned part;          // the part it takes from now
+    unsigned parts_left;    // parts not yet found used up
+    unsigned taken;         // the part's counter as the wave last saw it
+    unsigned by_rank;       // the wave's entries when handed out by rank: [by_rank, by_rank + 64)
+    bool ranked;            // handed out by rank
 
-// Takes the wave's next chunk of entries [base, end): from its current part while that lasts,
-// then from the following parts; false once every part is used up. Within a part, guided
-// self-scheduling: a chunk is (entries left in the part) / (4 x waves per part), a multiple of 64
-// in [64, 1024], so that a launch starts with few atomics and ends with small chunks. `taken` is
-// the part's counter as the wave last saw it (an estimate of what is left).
-SOL_DEV bool trace_fetch(unsigned* counters, unsigned total, unsigned waves, unsigned parts, unsigned& part,
-                         unsigned& parts_left, unsigned& taken, unsigned& base, unsigned& end) {
-    const unsigned per_part = waves / parts > 0 ? waves / parts : 1u;
-    while (parts_left > 0) {
-        const unsigned lo = trace_part_begin(total, part, parts), hi = trace_part_begin(total, part + 1, parts);
+    SOL_DEV TraceWork(unsigned total_, unsigned block_waves) : total(total_) {
+        parts = total >= kTracePartsBelow ? 1u : static_cast<unsigned>(kTraceParts);
+        part = blockIdx.x % parts;
+        parts_left = parts;
+        taken = 0;
+        const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * block_waves;
+        ranked = total <= waves * 64;
+        by_rank = ranked ? (blockIdx.x * block_waves + threadIdx.x / 64) * 64u : 0u;
+    }
+    SOL_DEV unsigned begin(unsigned p) const {
+        return static_cast<unsigned>(static_cast<unsigned long long>(total) * p / parts);
+    }
+};
+
+// Takes the wave's next chunk of entries [base, end); false once there is none left. Within a
+// part, guided self-scheduling: a chunk is (entries left in the part) / (4 x waves per part), a
+// multiple of 64 in [64, 1024], so that a launch starts with few atomics and ends with small
+// chunks.
+SOL_DEV bool trace_fetch(unsigned* counters, unsigned waves, TraceWork& w, unsigned& base, unsigned& end) {
+    if (w.ranked) {
+        if (w.by_rank >= w.total) return false;
+        base = w.by_rank;
+        end = min(base + 64u, w.total);
+        w.by_rank = w.total;
+        return true;
+    }
+    const unsigned per_part = waves / w.parts > 0 ? waves / w.parts : 1u;
+    while (w.parts_left > 0) {
+        const unsigned lo = w.begin(w.part), hi = w.begin(w.part + 1);
         const unsigned size = hi - lo;
-        const unsigned left = size > taken ? size - taken : 0u;
+        const unsigned left = size > w.taken ? size - w.taken : 0u;
         unsigned chunk = (left / (4 * per_part)) & ~63u;
         chunk = chunk < 64 ? 64 : (chunk > 1024 ? 1024 : chunk);
         unsigned off = 0;
-        if (__lane_id() == 0) off = atomicAdd(&counters[part * kCounterStride], chunk);
+        if (__lane_id() == 0) off = atomicAdd(&counters[w.part * kCounterStride], chunk);
         off = __shfl(off, 0);
         if (off < size) {
-            taken = off;
+            w.taken = off;
             base = lo + off;
             end = min(base + chunk, hi);
             return true;
         }
-        part = part + 1 == parts ? 0u : part + 1;
-        --parts_left;
-        taken = 0;
+        w.part = w.part + 1 == w.parts ? 0u : w.part + 1;
+        --w.parts_left;
+        w.taken = 0;
     }
     return false;
 }
@@ -505,10 +534,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
     RayPre r;
     TraversalState ts;
     unsigned c_next = 0, c_end = 0;   // the wave's current chunk [c_next, c_end)
-    // work parts: workgroup b starts on part b mod parts (with eight parts, its XCD's, as
-    // workgroups are dealt to the XCDs round robin)
-    const unsigned parts = total >= kTracePartsBelow ? 1u : static_cast<unsigned>(kTraceParts);
-    unsigned part = blockIdx.x % parts, parts_left = parts, taken = 0;
+    TraceWork work(total, BLOCK / 64);
     bool exhausted = false;
     unsigned n_closest = 0, n_any = 0;
     while (true) {
@@ -518,9 +544,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
             const unsigned rank = static_cast<unsigned>(__popcll(idle & lanes_below()));
             const unsigned avail = c_end - c_next;
             unsigned base2 = 0, end2 = 0;
-            if (avail < n_idle &&
-                !trace_fetch(&b.counts[kCntTraceWork], total, waves, parts, part, parts_left, taken, base2, end2))
-                exhausted = true;
+            if (avail < n_idle && !trace_fetch(&b.counts[kCntTraceWork], waves, work, base2, end2)) exhausted = true;
             if (idx < 0) {
                 unsigned e = 0xFFFFFFFFu;
                 if (rank < avail) e = c_next + rank;

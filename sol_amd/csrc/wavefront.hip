@@ -38,14 +38,72 @@ struct Job {
 
 } // namespace
 
+// The tables of a render call (scenes, jobs, tiles) in one device buffer, written by one copy from
+// pinned host memory (twenty copies from pageable memory took about 0.3 ms a call). Each table
+// keeps a grow-only slot, so its device address changes only when it outgrows the slot.
+struct TableSet {
+    static constexpr int kMax = 24;
+    size_t cap[kMax] = {}, off[kMax] = {};
+    size_t used = 0;               // bytes of the laid-out slots
+    DeviceBuffer dev;
+    void* host = nullptr;          // pinned staging of `used` bytes
+    hipEvent_t copied = nullptr;   // the last copy out of the staging memory
+
+    TableSet() = default;
+    TableSet(const TableSet&) = delete;
+    TableSet& operator=(const TableSet&) = delete;
+    ~TableSet() {
+        if (copied) {
+            (void)hipEventSynchronize(copied);
+            (void)hipEventDestroy(copied);
+        }
+        if (host) (void)hipHostFree(host);
+    }
+
+    // Copies table i (bytes[i] bytes at data[i]) to the device and gives its address in dst[i];
+    // bumps `generation` when the layout, and so any address, changes.
+    int upload(int n, const void* const* data, const size_t* bytes, void** dst, hipStream_t s, uint64_t& generation) {
+        if (n > kMax) return set_error(SOLHIP_E_INVALID, "too many tables");
+        if (!copied) HIP_OK(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
+        else HIP_OK(hipEventSynchronize(copied));   // the staging memory is free again
+        bool relayout = !dev.ptr;
+        for (int i = 0; i < n; ++i) {
+            if (bytes[i] > cap[i]) {
+                cap[i] = std::max<size_t>(256, 2 * bytes[i]);
+                relayout = true;
+            }
+        }
+        if (relayout) {
+            size_t o = 0;
+            for (int i = 0; i < n; ++i) {
+                off[i] = o;
+                o += align_up(cap[i]);
+            }
+            used = o;
+            HIP_OK(dev.alloc(used));
+            if (host) HIP_OK(hipHostFree(host));
+            host = nullptr;
+            HIP_OK(hipHostMalloc(&host, used, hipHostMallocDefault));
+            ++generation;
+        }
+        size_t end = 0;
+        for (int i = 0; i < n; ++i) {
+            if (bytes[i]) std::memcpy(static_cast<char*>(host) + off[i], data[i], bytes[i]);
+            dst[i] = dev.as<char>() + off[i];
+            end = std::max(end, off[i] + bytes[i]);
+        }
+        if (end) HIP_OK(hipMemcpyAsync(dev.ptr, host, end, hipMemcpyHostToDevice, s));
+        HIP_OK(hipEventRecord(copied, s));
+        return SOLHIP_OK;
+    }
+};
+
 struct WavefrontState {
     uint32_t slots = 0;
     DeviceBuffer arena, counters, samples;
     unsigned long long sample_capacity = 0;
     WaveBuf buf{};
-    DeviceBuffer t_job_sout, t_job_sstride, t_job_soff, t_job_layer;
-    DeviceBuffer t_scenes, t_job_item0, t_job_sample0, t_job_scene, t_job_accum, t_job_w, t_job_h, t_job_spp,
-        t_job_tw, t_job_th, t_tile_job, t_tile_x0, t_tile_y0, t_tile_w, t_tile_h, t_tile_item0;
+    TableSet tables;
     uint64_t generation = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -124,17 +182,6 @@ struct WavefrontState {
         buf.samples = samples.as<float4>();
         buf.sample_capacity = items;
         ++generation;
-        return SOLHIP_OK;
-    }
-
-    template <typename T>
-    int upload(DeviceBuffer& d, const std::vector<T>& v, hipStream_t stream) {
-        size_t bytes = sizeof(T) * std::max<size_t>(1, v.size());
-        if (d.bytes < bytes || !d.ptr) {
-            HIP_OK(d.alloc(bytes * 2));
-            ++generation;
-        }
-        if (!v.empty()) HIP_OK(hipMemcpyAsync(d.ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream));
         return SOLHIP_OK;
     }
 
@@ -293,17 +340,24 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     const std::vector<uint32_t> job_layer = reduction_layers(job_accum, job_w, job_h, num_layers);
 
     hipStream_t s = ctx.stream;
-    if ((rc = st.upload(st.t_scenes, views, s)) || (rc = st.upload(st.t_job_item0, job_item0, s)) ||
-        (rc = st.upload(st.t_job_sample0, job_sample0, s)) || (rc = st.upload(st.t_job_scene, job_scene, s)) ||
-        (rc = st.upload(st.t_job_accum, job_accum, s)) || (rc = st.upload(st.t_job_w, job_w, s)) ||
-        (rc = st.upload(st.t_job_sout, job_sout, s)) || (rc = st.upload(st.t_job_sstride, job_sstride, s)) ||
-        (rc = st.upload(st.t_job_soff, job_soff, s)) || (rc = st.upload(st.t_job_layer, job_layer, s)) ||
-        (rc = st.upload(st.t_job_h, job_h, s)) || (rc = st.upload(st.t_job_spp, job_spp, s)) ||
-        (rc = st.upload(st.t_job_tw, job_tw, s)) || (rc = st.upload(st.t_job_th, job_th, s)) ||
-        (rc = st.upload(st.t_tile_job, tile_job, s)) || (rc = st.upload(st.t_tile_x0, tile_x0, s)) ||
-        (rc = st.upload(st.t_tile_y0, tile_y0, s)) || (rc = st.upload(st.t_tile_w, tile_w, s)) ||
-        (rc = st.upload(st.t_tile_h, tile_h, s)) || (rc = st.upload(st.t_tile_item0, tile_item0, s)))
-        return rc;
+    void* tab[20];
+    {
+        auto entry = [](const auto& v) {
+            return std::make_pair(static_cast<const void*>(v.data()), sizeof(v[0]) * v.size());
+        };
+        const std::pair<const void*, size_t> t[20] = {
+            entry(views),      entry(job_item0), entry(job_sample0), entry(job_scene),  entry(job_accum),
+            entry(job_w),      entry(job_sout),  entry(job_sstride),  entry(job_soff),   entry(job_layer),
+            entry(job_h),      entry(job_spp),   entry(job_tw),       entry(job_th),     entry(tile_job),
+            entry(tile_x0),    entry(tile_y0),   entry(tile_w),       entry(tile_h),     entry(tile_item0)};
+        const void* data[20];
+        size_t bytes[20];
+        for (int i = 0; i < 20; ++i) {
+            data[i] = t[i].first;
+            bytes[i] = t[i].second;
+        }
+        if ((rc = st.tables.upload(20, data, bytes, tab, s, st.generation))) return rc;
+    }
 
     // Chunks of whole tiles whose items fit the sample buffer.
     struct Chunk { uint32_t first, count; unsigned long long begin, end; };
@@ -328,27 +382,27 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     const bool multi = views.size() > 1;
     WaveLaunch L{};
     L.single = views[0];
-    L.tabs.scenes = st.t_scenes.as<SceneView>();
+    L.tabs.scenes = static_cast<SceneView*>(tab[0]);
     L.tabs.num_jobs = num_jobs;
-    L.tabs.job_item0 = st.t_job_item0.as<unsigned long long>();
-    L.tabs.job_sample0 = st.t_job_sample0.as<unsigned long long>();
-    L.tabs.job_scene = st.t_job_scene.as<uint32_t>();
-    L.tabs.job_accum = st.t_job_accum.as<float*>();
-    L.tabs.job_sout = st.t_job_sout.as<float4*>();
-    L.tabs.job_sstride = st.t_job_sstride.as<uint32_t>();
-    L.tabs.job_soff = st.t_job_soff.as<uint32_t>();
-    L.tabs.job_layer = st.t_job_layer.as<uint32_t>();
-    L.tabs.job_w = st.t_job_w.as<uint32_t>();
-    L.tabs.job_h = st.t_job_h.as<uint32_t>();
-    L.tabs.job_spp = st.t_job_spp.as<uint32_t>();
-    L.tabs.job_tw = st.t_job_tw.as<uint32_t>();
-    L.tabs.job_th = st.t_job_th.as<uint32_t>();
-    L.tabs.tile_job = st.t_tile_job.as<uint32_t>();
-    L.tabs.tile_x0 = st.t_tile_x0.as<uint32_t>();
-    L.tabs.tile_y0 = st.t_tile_y0.as<uint32_t>();
-    L.tabs.tile_w = st.t_tile_w.as<uint32_t>();
-    L.tabs.tile_h = st.t_tile_h.as<uint32_t>();
-    L.tabs.tile_item0 = st.t_tile_item0.as<unsigned long long>();
+    L.tabs.job_item0 = static_cast<unsigned long long*>(tab[1]);
+    L.tabs.job_sample0 = static_cast<unsigned long long*>(tab[2]);
+    L.tabs.job_scene = static_cast<uint32_t*>(tab[3]);
+    L.tabs.job_accum = static_cast<float**>(tab[4]);
+    L.tabs.job_w = static_cast<uint32_t*>(tab[5]);
+    L.tabs.job_sout = static_cast<float4**>(tab[6]);
+    L.tabs.job_sstride = static_cast<uint32_t*>(tab[7]);
+    L.tabs.job_soff = static_cast<uint32_t*>(tab[8]);
+    L.tabs.job_layer = static_cast<uint32_t*>(tab[9]);
+    L.tabs.job_h = static_cast<uint32_t*>(tab[10]);
+    L.tabs.job_spp = static_cast<uint32_t*>(tab[11]);
+    L.tabs.job_tw = static_cast<uint32_t*>(tab[12]);
+    L.tabs.job_th = static_cast<uint32_t*>(tab[13]);
+    L.tabs.tile_job = static_cast<uint32_t*>(tab[14]);
+    L.tabs.tile_x0 = static_cast<uint32_t*>(tab[15]);
+    L.tabs.tile_y0 = static_cast<uint32_t*>(tab[16]);
+    L.tabs.tile_w = static_cast<uint32_t*>(tab[17]);
+    L.tabs.tile_h = static_cast<uint32_t*>(tab[18]);
+    L.tabs.tile_item0 = static_cast<unsigned long long*>(tab[19]);
     L.tabs.num_tiles = num_tiles;
     L.buf = st.buf;
     L.buf.slots = want_slots;
