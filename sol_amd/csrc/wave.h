@@ -65,6 +65,13 @@ enum : int {
     kCounterWords = (kNumWaveCounters + 1) * kCounterStride
 };
 
+// Statistics (rays traced, paths started) are summed over kStatSlots copies, kStatStride 64-bit
+// words (256 bytes) apart, each workgroup adding to copy blockIdx % kStatSlots: with one copy the
+// atomic of every wave of every launch landed on the same cache line.
+constexpr int kStatSlots = 32;
+constexpr int kStatStride = 32;
+constexpr int kStatWords = kStatSlots * kStatStride;
+
 // Work items of the current chunk (64-bit words, 256 bytes apart): segment s hands out
 // items[item_next(s)] .. items[item_end(s)] - 1, and items[item_next(s) + 1] is where its range
 // began (the start fill's items run from there to the counter's first value);
@@ -128,7 +135,7 @@ struct WaveBuf {
     float4* sq;                       // [nseg * cap_s * 3]
     unsigned* counts;                 // kCounterWords
     unsigned long long* items;        // kItemWords
-    unsigned long long* stats;        // [0] extension rays [1] shadow rays [2] samples
+    unsigned long long* stats;        // kStatSlots lines of: [0] extension rays [1] shadow rays [2] samples
     float4* samples;                  // [sample_capacity]: radiance of each item of the chunk
     unsigned long long sample_capacity;
 

@@ -47,9 +47,10 @@ SOL_DEV unsigned long long lanes_below() {
     return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
 }
 
-SOL_DEV void wave_add_stat(unsigned long long* counter, unsigned long long v) {
+// Adds the wave's v to statistic `which` (the workgroup's copy of it, see kStatSlots).
+SOL_DEV void wave_add_stat(unsigned long long* stats, int which, unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    if (__lane_id() == 0 && v) atomicAdd(counter, v);
+    if (__lane_id() == 0 && v) atomicAdd(&stats[(blockIdx.x % kStatSlots) * kStatStride + which], v);
 }
 
 // Workgroup-wide queue reservations of one round (one entry per thread). Each thread belongs to
@@ -643,8 +644,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
             }
         }
     }
-    wave_add_stat(&b.stats[0], n_closest);
-    wave_add_stat(&b.stats[1], n_any);
+    wave_add_stat(b.stats, 0, n_closest);
+    wave_add_stat(b.stats, 1, n_any);
 }
 
 // ---- sort ------------------------------------------------------------------------------------------
@@ -722,7 +723,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             }
         }
     }
-    wave_add_stat(&b.stats[2], started);
+    wave_add_stat(b.stats, 2, started);
 }
 
 // ---- shade -----------------------------------------------------------------------------------------
@@ -836,7 +837,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
             }
         }
     }
-    wave_add_stat(&b.stats[2], started);
+    wave_add_stat(b.stats, 2, started);
 }
 
 // ---- start / reset / reduce ------------------------------------------------------------------------
@@ -883,7 +884,7 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
             }
         }
     }
-    wave_add_stat(&b.stats[2], started);
+    wave_add_stat(b.stats, 2, started);
 }
 
 // After trace(k) has consumed extension queue `cur` and the shadow queue: clears what sort(k) and

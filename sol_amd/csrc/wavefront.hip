@@ -110,6 +110,7 @@ struct WavefrontState {
     std::string graph_key;
     std::vector<hipEvent_t> events;
     std::vector<unsigned> host_counts = std::vector<unsigned>(kCounterWords);
+    std::vector<unsigned long long> host_stats = std::vector<unsigned long long>(kStatWords);
 
     ~WavefrontState() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -143,7 +144,8 @@ struct WavefrontState {
             size_t total = 0;
             for (size_t g = 1; g <= kMaxSegments; g *= 2) total = std::max(total, layout(n, g).bytes(n, g));
             HIP_OK(arena.alloc(total));
-            HIP_OK(counters.alloc(sizeof(unsigned) * kCounterWords + align_up(sizeof(unsigned long long) * kItemWords) + 256));
+            HIP_OK(counters.alloc(sizeof(unsigned) * kCounterWords + align_up(sizeof(unsigned long long) * kItemWords) +
+                                  sizeof(unsigned long long) * kStatWords));
             slots = n;
             buf.nseg = 0;   // lay out again
         }
@@ -467,7 +469,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
         }
     }
 
-    HIP_OK(hipMemsetAsync(st.buf.stats, 0, sizeof(unsigned long long) * 4, s));
+    HIP_OK(hipMemsetAsync(st.buf.stats, 0, sizeof(unsigned long long) * kStatWords, s));
     HIP_OK(hipEventRecord(ctx.ev_begin, s));
     uint64_t replays = 0, launches = 0;
     for (const Chunk& c : chunks) {
@@ -509,9 +511,12 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
         launches += num_layers;
     }
     HIP_OK(hipEventRecord(ctx.ev_end, s));
-    unsigned long long stats[3];
-    HIP_OK(hipMemcpyAsync(stats, st.buf.stats, sizeof(stats), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(st.host_stats.data(), st.buf.stats, sizeof(unsigned long long) * kStatWords,
+                          hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    unsigned long long stats[3] = {0, 0, 0};
+    for (int k = 0; k < kStatSlots; ++k)
+        for (int i = 0; i < 3; ++i) stats[i] += st.host_stats[static_cast<size_t>(k) * kStatStride + i];
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, ctx.ev_begin, ctx.ev_end));
     ctx.stats.render_ms += ms;
