@@ -71,6 +71,8 @@ enum : int {
 constexpr int kStatSlots = 32;
 constexpr int kStatStride = 32;
 constexpr int kStatWords = kStatSlots * kStatStride;
+// Word 3 of copy 0: entries left in the queues of chunks that should have ended (k_check_drained).
+constexpr int kStatUndrained = 3;
 
 // Work items of the current chunk (64-bit words, 256 bytes apart): segment s hands out
 // items[item_next(s)] .. items[item_end(s)] - 1, and items[item_next(s) + 1] is where its range

@@ -924,6 +924,16 @@ __global__ void k_set_chunk(WaveBuf b, unsigned long long begin, unsigned long l
     b.counts[kCntStartFill] = 1;
 }
 
+// After an even number of bounce iterations of a chunk that should have ended in them: adds the
+// entries left in extension queue 0 and the shadow queue (zero when it has ended) to statistic
+// kStatUndrained, which the host reads with the others.
+__global__ void k_check_drained(WaveBuf b) {
+    if (threadIdx.x != 0) return;
+    unsigned long long left = 0;
+    for (int k = 0; k < kMaxSegments; ++k) left += b.counts[qcount(kQExtend0, k)] + b.counts[qcount(kQShadow, k)];
+    if (left) atomicAdd(&b.stats[kStatUndrained], left);
+}
+
 // Per pixel of the chunk's tiles of jobs in reduction layer `layer`: sum the samples in sample
 // order, add to the image once.
 __global__ __launch_bounds__(256) void k_reduce(WaveTables tabs, WaveBuf b, uint32_t first_tile,
@@ -1066,6 +1076,12 @@ hipError_t wave_launch_set_chunk(const WaveLaunch& L, unsigned long long begin, 
     (void)hipGetLastError();
     // the start fill that follows writes extension queue 0
     hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(64), 0, L.stream, L.buf, begin, end, 0);
+    return hipGetLastError();
+}
+
+hipError_t wave_launch_check_drained(const WaveLaunch& L) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_check_drained, dim3(1), dim3(64), 0, L.stream, L.buf);
     return hipGetLastError();
 }
 

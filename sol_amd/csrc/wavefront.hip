@@ -469,6 +469,11 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
         }
     }
 
+    // A chunk that fits in the paths in flight ends within one replay of `need` iterations (see
+    // above): rather than a round trip to the host to see its queues empty, k_check_drained counts
+    // what is left in them into a statistic read with the others at the end, which must be zero.
+    // (A profiled run reads its events after each replay, so it keeps the round trip.)
+    const bool drains_in_one = one_fill && iters == static_cast<int>(need) && !profile;
     HIP_OK(hipMemsetAsync(st.buf.stats, 0, sizeof(unsigned long long) * kStatWords, s));
     HIP_OK(hipEventRecord(ctx.ev_begin, s));
     uint64_t replays = 0, launches = 0;
@@ -485,6 +490,11 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
             }
             ++replays;
             launches += static_cast<uint64_t>(iters) * 6;
+            if (drains_in_one) {
+                HIP_OK(wave_launch_check_drained(L));
+                ++launches;
+                break;
+            }
             // After an even number of iterations the next extension queue is queue 0; shadow rays
             // of the last shade still need a trace.
             std::vector<unsigned>& counts = st.host_counts;
@@ -517,6 +527,9 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     unsigned long long stats[3] = {0, 0, 0};
     for (int k = 0; k < kStatSlots; ++k)
         for (int i = 0; i < 3; ++i) stats[i] += st.host_stats[static_cast<size_t>(k) * kStatStride + i];
+    if (st.host_stats[kStatUndrained] != 0)
+        return set_error(SOLHIP_E_HIP, "a chunk of work did not end within its bounce iterations (" +
+                                           std::to_string(st.host_stats[kStatUndrained]) + " queue entries left)");
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, ctx.ev_begin, ctx.ev_end));
     ctx.stats.render_ms += ms;
