@@ -1,7 +1,8 @@
 // Wavefront kernels.
 //   k_trace  closest hit of the extension rays (hit written in place) and any hit of the previous
 //            bounce's shadow rays (unoccluded contribution added to the path or its sample);
-//            traversal only, work fetched by waves in shrinking chunks (guided self-scheduling).
+//            traversal only, work fetched by waves in shrinking chunks (guided self-scheduling)
+//            from one counter, or from eight part counters or by rank in smaller launches.
 //   k_sort   material queues from each hit triangle's kind byte; a path that missed ends there (its
 //            radiance goes to its sample entry, unless the chunk runs in place) and its slot starts
 //            the next work item.
@@ -15,7 +16,7 @@
 //
 // The per-bounce arithmetic is the shared code of path.h, in the order of trace_path
 // (src/algorithms/path_tracer.cpp:31-120 of sol): NEE of vertex k is added by trace(k+1), before
-// the emission of hit k+1 is added by sort(k+1).
+// the emission of hit k+1 is added by shade(k+1).
 #include <algorithm>
 
 #include "launch.h"
