@@ -24,6 +24,7 @@
 #include "traverse.h"
 #include "wave.h"
 #include "wave_launch.h"
+#include "work.h"
 
 namespace solhip {
 
@@ -42,10 +43,6 @@ SOL_DEV float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 SOL_DEV V3 xyz(float4 v) { return V3{v.x, v.y, v.z}; }
 SOL_DEV unsigned long long item_of(float4 p2) {
     return static_cast<unsigned long long>(as_u(p2.z)) | (static_cast<unsigned long long>(as_u(p2.w)) << 32);
-}
-SOL_DEV unsigned long long lanes_below() {
-    int lane = __lane_id();
-    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
 }
 
 // Adds the wave's v to statistic `which` (the workgroup's copy of it, see kStatSlots).
@@ -412,95 +409,6 @@ SOL_DEV Stack make_stack_w(int* lds_stack, const TraversalScratch& sc, int lds_e
 }
 
 // ---- trace -----------------------------------------------------------------------------------------
-
-// Launches of at least this many entries fetch from one counter, smaller ones from kTraceParts
-// counters. Device atomics on one address serialise: a wave takes 64 entries at a time towards the
-// end of a launch, so a launch of a million rays (the 256 x 256, 16-spp Cornell box) makes some
-// 16,000 fetches in a few hundred microseconds, and with one counter they were its bound. A large
-// launch fetches 1024 entries at a time, and with one counter all XCDs work on one narrow window
-// of the queue, which the shared MALL serves better than eight windows. Measured (ms per frame,
-// trace ms in brackets): Cornell box one counter 3.11 (2.50), eight parts always 2.14 (1.55); 1080p
-// sphere field one counter 54.5 (42.4), eight parts always 54.8 (43.0), eight parts below 4M
-// entries 53.7 (41.7), below 16M 53.7 (41.5).
-constexpr unsigned kTracePartsBelow = 16u << 20;
-
-// A wave's place in the traversal kernel's work, the entries [0, total). A launch with no more
-// entries than 64 per wave is handed out by rank: wave g takes entries [64 g, 64 g + 64) and no
-// atomic is made (every persistent wave of such a launch would otherwise make at least one on a
-// few addresses, which set a floor of about 0.1 ms under the small launches at the end of a
-// frame). Larger launches are handed out from `parts` consecutive parts by their counters; the
-// waves of workgroup b take from part b mod parts first (with eight parts, the XCD that runs them,
-// as workgroups are dealt to the XCDs round robin), then from the following parts.
-struct TraceWork {
-    unsigned total, parts;
-    unsigned part;          // the part it takes from now
-    unsigned parts_left;    // parts not yet found used up
-    unsigned taken;         // the part's counter as the wave last saw it
-    unsigned by_rank;       // the wave's entries when handed out by rank: [by_rank, by_rank + 64)
-    bool ranked;            // handed out by rank
-
-    SOL_DEV TraceWork(unsigned total_, unsigned block_waves) : total(total_) {
-        parts = total >= kTracePartsBelow ? 1u : static_cast<unsigned>(kTraceParts);
-        part = blockIdx.x % parts;
-        parts_left = parts;
-        taken = 0;
-        const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * block_waves;
-        ranked = total <= waves * 64;
-        by_rank = ranked ? (blockIdx.x * block_waves + threadIdx.x / 64) * 64u : 0u;
-    }
-    SOL_DEV unsigned begin(unsigned p) const {
-        return static_cast<unsigned>(static_cast<unsigned long long>(total) * p / parts);
-    }
-};
-
-// Takes the wave's next chunk of entries [base, end); false once there is none left. Within a
-// part, guided self-scheduling: a chunk is (entries left in the part) / (4 x waves per part), a
-// multiple of 64 in [64, 1024], so that a launch starts with few atomics and ends with small
-// chunks.
-SOL_DEV bool trace_fetch(unsigned* counters, unsigned waves, TraceWork& w, unsigned& base, unsigned& end) {
-    if (w.ranked) {
-        if (w.by_rank >= w.total) return false;
-        base = w.by_rank;
-        end = min(base + 64u, w.total);
-        w.by_rank = w.total;
-        return true;
-    }
-    const unsigned per_part = waves / w.parts > 0 ? waves / w.parts : 1u;
-    while (w.parts_left > 0) {
-        const unsigned lo = w.begin(w.part), hi = w.begin(w.part + 1);
-        const unsigned size = hi - lo;
-        const unsigned left = size > w.taken ? size - w.taken : 0u;
-        unsigned chunk = (left / (4 * per_part)) & ~63u;
-        chunk = chunk < 64 ? 64 : (chunk > 1024 ? 1024 : chunk);
-        unsigned off = 0;
-        if (__lane_id() == 0) off = atomicAdd(&counters[w.part * kCounterStride], chunk);
-        off = __shfl(off, 0);
-        if (off < size) {
-            w.taken = off;
-            base = lo + off;
-            end = min(base + chunk, hi);
-            return true;
-        }
-        w.part = w.part + 1 == w.parts ? 0u : w.part + 1;
-        --w.parts_left;
-        w.taken = 0;
-    }
-    return false;
-}
-
-// Lanes whose ray has ended take new entries as soon as this many lanes of the wave are idle, so
-// that a wave does not wait for its slowest ray before fetching more work. Measured on the 1080p
-// sphere field (trace ms per 64-spp frame): 8 lanes 63.7, 16 62.1, 24 60.7, 32 59.7, 40 59.9,
-// 48 60.4, 56 62.2, 64 69.3; refills cost a divergent detour, leaf postponing keeps waiting lanes
-// busy, so refilling half a wave at a time pays best.
-constexpr int kRefillLanes = 32;
-
-// A traversal step stops descending once this many lanes or fewer still look for their first
-// leaf (traverse4_step's LEAF_WAIT): in a third of the node iterations of a 1080p frame fewer than
-// 8 of the 64 lanes were still searching. Measured with instrumented counters on the 1080p sphere
-// field (ms per frame, instrumented build): 0 lanes 72.5, 2 70.2, 4 70.0, 8 70.9, 12 72.1,
-// 16 73.5, 32 80.9.
-constexpr int kLeafWaitLanes = 4;
 
 // Persistent traversal with per-lane refill: every lane runs its own ray one traversal step at a
 // time; when kRefillLanes lanes are idle the wave hands them the next entries of its chunk
