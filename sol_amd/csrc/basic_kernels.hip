@@ -1,91 +1,44 @@
 // Batched intersection (Geometry::intersect_closest / intersect_any, src/triangle_mesh.cpp:64-117
 // of sol) and the plain GPU megakernel (PathTracer::render + trace_path,
 // src/algorithms/path_tracer.cpp:9-120), one thread per pixel looping over its samples.
-// Both are persistent: the megakernel's waves fetch 64 pixels at a time from an atomic counter, the
-// intersection kernel schedules its rays as the wavefront renderer's traversal kernel does.
+// Both are persistent: each wave fetches 64 work items at a time from an atomic counter.
 #include <algorithm>
 
 #include "launch.h"
 #include "megapath.h"
-#include "work.h"
 
 namespace solhip {
 
 namespace {
 
-// Rays [0, n) of one launch, with the traversal kernel's scheduling (work.h): every lane runs its
-// own ray one traversal step at a time and the wave hands the next rays of its chunk to its idle
-// lanes once kRefillLanes of them are idle, chunks coming from one counter, eight part counters
-// or ranks depending on n. Before, each wave took 64 rays from one counter and traced them to the
-// end, waiting for its slowest lane: 4M rays made 65,536 atomics on one address.
 template <bool ANY, int BLOCK, int LDS, bool WIDE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void intersect_kernel(
-    SceneView s, const sol_ray* __restrict__ rays, unsigned n, sol_hit* __restrict__ hits,
-    int32_t* __restrict__ occluded, unsigned* counters, TraversalScratch sc) {
+__global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol_ray* __restrict__ rays, uint64_t n,
+                                                          sol_hit* __restrict__ hits, int32_t* __restrict__ occluded,
+                                                          unsigned long long* counter, TraversalScratch sc) {
     __shared__ float4 lds_nodes[kTopLdsFloat4<WIDE>];
     __shared__ int lds_stack[LDS * BLOCK];
     stage_top_nodes<BLOCK, WIDE>(s, lds_nodes);
-    const Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
-    const unsigned waves = gridDim.x * (BLOCK / 64);
-    const FetchUniform fu(s, lds_nodes);
-    const FetchUniform4 fu4(s, lds_nodes);
-    TraceWork work(n, BLOCK / 64);
-    int idx = -1;   // ray this lane traces, -1 = idle
-    float tmin = 0.0f, tmax0 = 0.0f;
-    RayPre r;
-    TraversalState ts;
-    unsigned c_next = 0, c_end = 0;   // the wave's current chunk [c_next, c_end)
-    bool exhausted = false;
+    Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
     while (true) {
-        const unsigned long long idle = __ballot(idx < 0);
-        const unsigned n_idle = static_cast<unsigned>(__popcll(idle));
-        if (!exhausted && n_idle >= static_cast<unsigned>(kRefillLanes)) {
-            const unsigned rank = static_cast<unsigned>(__popcll(idle & lanes_below()));
-            const unsigned avail = c_end - c_next;
-            unsigned base2 = 0, end2 = 0;
-            if (avail < n_idle && !trace_fetch(counters, waves, work, base2, end2)) exhausted = true;
-            if (idx < 0) {
-                unsigned e = 0xFFFFFFFFu;
-                if (rank < avail) e = c_next + rank;
-                else if (base2 + (rank - avail) < end2) e = base2 + (rank - avail);
-                if (e != 0xFFFFFFFFu) {
-                    const float4* rp = reinterpret_cast<const float4*>(rays + e);
-                    const float4 a = rp[0], b = rp[1];
-                    r = make_ray_pre(V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z});
-                    tmin = a.w;
-                    tmax0 = b.w;
-                    ts.start(tmax0);
-                    idx = static_cast<int>(e);
-                }
-            }
-            if (avail >= n_idle) {
-                c_next += n_idle;
-            } else if (end2 > base2) {
-                c_next = min(base2 + (n_idle - avail), end2);
-                c_end = end2;
-            } else {
-                c_next = c_end;
-            }
-        }
-        if (__ballot(idx >= 0) == 0) {
-            if (exhausted) break;
-            continue;
-        }
-        if (idx >= 0) {
-            const int status = WIDE ? traverse4_step<BLOCK, LDS, kLeafWaitLanes>(fu4, st, r, tmin, ANY, ts)
-                                    : traverse_step<BLOCK, LDS>(fu, st, r, tmin, ANY, ts);
-            if (status != kStepContinue) {
-                const unsigned i = static_cast<unsigned>(idx);
-                if (ANY) {
-                    occluded[i] = status == kStepOccluded ? 1 : 0;
-                } else {
-                    const bool found = ts.hit.prim >= 0;
-                    reinterpret_cast<float4*>(hits)[i] =
-                        make_float4(__int_as_float(found ? ts.hit.prim : -1), found ? ts.hit.t : tmax0,
-                                    found ? ts.hit.u : 0.0f, found ? ts.hit.v : 0.0f);
-                }
-                idx = -1;
-            }
+        unsigned long long base = wave_grab(counter, 64);
+        if (base >= n) break;
+        uint64_t i = base + __lane_id();
+        if (i >= n) continue;
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        float4 a = rp[0], b = rp[1];
+        V3 org{a.x, a.y, a.z}, dir{b.x, b.y, b.z};
+        RayPre r = make_ray_pre(org, dir);
+        HitRecord hit;
+        bool found = trace_ray<ANY, BLOCK, LDS, WIDE>(s, lds_nodes, st, r, a.w, b.w, hit);
+        if (ANY) {
+            occluded[i] = found ? 1 : 0;
+        } else {
+            sol_hit h;
+            h.prim = found ? hit.prim : -1;
+            h.t = found ? hit.t : b.w;
+            h.u = found ? hit.u : 0.0f;
+            h.v = found ? hit.v : 0.0f;
+            reinterpret_cast<float4*>(hits)[i] = make_float4(__int_as_float(h.prim), h.t, h.u, h.v);
         }
     }
 }
@@ -146,32 +99,20 @@ __global__ __launch_bounds__(256) void zero_f32_kernel(float* p, size_t n) {
         p[i] = 0.0f;
 }
 
-// Launches of at most 2^31 rays; the kTraceParts work counters (kCounterStride words apart) are
-// cleared before each.
 template <int BLOCK, int LDS>
 hipError_t intersect_impl(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
-                          int32_t* occluded, unsigned* counters, const TraversalScratch& sc, int grid,
+                          int32_t* occluded, unsigned long long* counter, const TraversalScratch& sc, int grid,
                           bool wide, hipStream_t stream) {
-    constexpr uint64_t kMaxRays = 1ull << 31;
+    (void)hipGetLastError();
     dim3 g(grid), b(BLOCK);
-    for (uint64_t first = 0; first < n; first += kMaxRays) {
-        const unsigned m = static_cast<unsigned>(std::min<uint64_t>(kMaxRays, n - first));
-        const sol_ray* rr = rays + first;
-        sol_hit* hh = hits ? hits + first : nullptr;
-        int32_t* oo = occluded ? occluded + first : nullptr;
-        hipError_t e = launch_zero_u64(reinterpret_cast<unsigned long long*>(counters), kTraceParts * kCounterStride / 2, stream);
-        if (e != hipSuccess) return e;
-        (void)hipGetLastError();
-        if (any_hit) {
-            if (wide) hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, true>), g, b, 0, stream, s, rr, m, hh, oo, counters, sc);
-            else hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, false>), g, b, 0, stream, s, rr, m, hh, oo, counters, sc);
-        } else {
-            if (wide) hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, true>), g, b, 0, stream, s, rr, m, hh, oo, counters, sc);
-            else hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, false>), g, b, 0, stream, s, rr, m, hh, oo, counters, sc);
-        }
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (any_hit) {
+        if (wide) hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, true>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
+        else hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, false>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
+    } else {
+        if (wide) hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, true>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
+        else hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, false>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
     }
-    return hipSuccess;
+    return hipGetLastError();
 }
 
 template <int BLOCK, int LDS>
@@ -205,9 +146,9 @@ hipError_t megakernel_impl(const SceneView& s, const sol_pt_config& cfg, uint32_
     }
 
 hipError_t launch_intersect(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
-                            int32_t* occluded, unsigned* work_counters, const TraversalScratch& scratch,
+                            int32_t* occluded, unsigned long long* work_counter, const TraversalScratch& scratch,
                             const LaunchShape& shape, hipStream_t stream) {
-    SOLHIP_DISPATCH(intersect_impl, s, rays, n, any_hit, hits, occluded, work_counters, scratch, shape.grid,
+    SOLHIP_DISPATCH(intersect_impl, s, rays, n, any_hit, hits, occluded, work_counter, scratch, shape.grid,
                     shape.wide && s.nodes4 != nullptr, stream)
 }
 

@@ -26,9 +26,8 @@ struct LaunchShape {
 // Counters a kernel adds to: [0] extension rays, [1] shadow rays, [2] samples.
 using StatCounters = unsigned long long*;
 
-// work_counters: kTraceParts counters kCounterStride words apart (cleared here before each launch)
 hipError_t launch_intersect(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
-                            int32_t* occluded, unsigned* work_counters, const TraversalScratch& scratch,
+                            int32_t* occluded, unsigned long long* work_counter, const TraversalScratch& scratch,
                             const LaunchShape& shape, hipStream_t stream);
 
 hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint32_t width, uint32_t height,

@@ -1,6 +1,6 @@
-// Work fetch of the persistent traversal kernels (the wavefront renderer's k_trace and the
-// batched intersection kernel): guided self-scheduling of a launch's entries over one counter,
-// eight part counters or ranks, and the per-lane refill and leaf-wait thresholds.
+// Work fetch of the wavefront renderer's persistent traversal kernel (k_trace): guided
+// self-scheduling of a launch's entries over one counter, eight part counters or ranks, and the
+// per-lane refill and leaf-wait thresholds.
 #pragma once
 
 #include "common.h"
