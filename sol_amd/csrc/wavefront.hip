@@ -66,7 +66,7 @@ struct TableSet {
         if (n > kMax) return set_error(SOLHIP_E_INVALID, "too many tables");
         if (!copied) HIP_OK(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
         else HIP_OK(hipEventSynchronize(copied));   // the staging memory is free again
-        bool relayout = !dev.ptr;
+        bool relayout = !dev.ptr || !host;   // also after an allocation that failed
         for (int i = 0; i < n; ++i) {
             if (bytes[i] > cap[i]) {
                 cap[i] = std::max<size_t>(256, 2 * bytes[i]);
