@@ -468,11 +468,11 @@ int solhip_intersect(solhip_scene* s, const sol_ray* rays, uint64_t n, int any_h
     LaunchShape shape = ctx->shape(ctx->tuning.block_size, ctx->tuning.lds_stack);
     int st = ctx->ensure_scratch(shape.grid * shape.block);
     if (st) return st;
-    HIP_TRY(launch_zero_u64(ctx->counters, 1, ctx->stream));
+    HIP_TRY(ctx->isect_counters.reserve(sizeof(unsigned) * kIntersectCounterWords));
     if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
     HIP_TRY(launch_intersect(s->view, d_rays, n, any_hit != 0, any_hit ? nullptr : static_cast<sol_hit*>(d_out),
-                             any_hit ? static_cast<int32_t*>(d_out) : nullptr, ctx->counters, ctx->scratch, shape,
-                             ctx->stream));
+                             any_hit ? static_cast<int32_t*>(d_out) : nullptr, ctx->isect_counters.as<unsigned>(),
+                             ctx->scratch, shape, ctx->stream));
     if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->ev_end, ctx->stream));
     if (!device_ptrs) HIP_TRY(hipMemcpyAsync(out, d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -1,44 +1,55 @@
 // Batched intersection (Geometry::intersect_closest / intersect_any, src/triangle_mesh.cpp:64-117
 // of sol) and the plain GPU megakernel (PathTracer::render + trace_path,
 // src/algorithms/path_tracer.cpp:9-120), one thread per pixel looping over its samples.
-// Both are persistent: each wave fetches 64 work items at a time from an atomic counter.
+// Both are persistent: the megakernel's waves fetch 64 pixels at a time from an atomic counter,
+// the intersection kernel's waves take chunks of rays with the traversal kernel's work fetch.
 #include <algorithm>
 
 #include "launch.h"
 #include "megapath.h"
+#include "work.h"
 
 namespace solhip {
 
 namespace {
 
+// Rays [0, n) of one launch (n < 2^32). Each wave takes a chunk of rays with the traversal
+// kernel's work fetch (work.h: guided chunks of 64 to 1024 rays from eight part counters, or from
+// one counter above 16M rays; 64 rays by rank, with no atomic, when the launch has no more than 64
+// a wave) and traces the chunk 64 rays at a time, one per lane, each batch to its end. One
+// counter handing out 64 rays per atomic (65,536 atomics on one address for 4M rays) was the
+// bound of this kernel.
 template <bool ANY, int BLOCK, int LDS, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol_ray* __restrict__ rays, uint64_t n,
+__global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol_ray* __restrict__ rays, unsigned n,
                                                           sol_hit* __restrict__ hits, int32_t* __restrict__ occluded,
-                                                          unsigned long long* counter, TraversalScratch sc) {
+                                                          unsigned* counters, TraversalScratch sc) {
     __shared__ float4 lds_nodes[kTopLdsFloat4<WIDE>];
     __shared__ int lds_stack[LDS * BLOCK];
     stage_top_nodes<BLOCK, WIDE>(s, lds_nodes);
     Stack st = make_stack<BLOCK, LDS>(lds_stack, sc);
-    while (true) {
-        unsigned long long base = wave_grab(counter, 64);
-        if (base >= n) break;
-        uint64_t i = base + __lane_id();
-        if (i >= n) continue;
-        const float4* rp = reinterpret_cast<const float4*>(rays + i);
-        float4 a = rp[0], b = rp[1];
-        V3 org{a.x, a.y, a.z}, dir{b.x, b.y, b.z};
-        RayPre r = make_ray_pre(org, dir);
-        HitRecord hit;
-        bool found = trace_ray<ANY, BLOCK, LDS, WIDE>(s, lds_nodes, st, r, a.w, b.w, hit);
-        if (ANY) {
-            occluded[i] = found ? 1 : 0;
-        } else {
-            sol_hit h;
-            h.prim = found ? hit.prim : -1;
-            h.t = found ? hit.t : b.w;
-            h.u = found ? hit.u : 0.0f;
-            h.v = found ? hit.v : 0.0f;
-            reinterpret_cast<float4*>(hits)[i] = make_float4(__int_as_float(h.prim), h.t, h.u, h.v);
+    const unsigned waves = gridDim.x * (BLOCK / 64);
+    TraceWork work(n, BLOCK / 64);
+    unsigned base = 0, end = 0;
+    while (trace_fetch(counters, waves, work, base, end)) {
+        for (unsigned first = base; first < end; first += 64) {
+            const unsigned i = first + static_cast<unsigned>(__lane_id());
+            if (i >= end) continue;
+            const float4* rp = reinterpret_cast<const float4*>(rays + i);
+            float4 a = rp[0], b = rp[1];
+            V3 org{a.x, a.y, a.z}, dir{b.x, b.y, b.z};
+            RayPre r = make_ray_pre(org, dir);
+            HitRecord hit;
+            bool found = trace_ray<ANY, BLOCK, LDS, WIDE>(s, lds_nodes, st, r, a.w, b.w, hit);
+            if (ANY) {
+                occluded[i] = found ? 1 : 0;
+            } else {
+                sol_hit h;
+                h.prim = found ? hit.prim : -1;
+                h.t = found ? hit.t : b.w;
+                h.u = found ? hit.u : 0.0f;
+                h.v = found ? hit.v : 0.0f;
+                reinterpret_cast<float4*>(hits)[i] = make_float4(__int_as_float(h.prim), h.t, h.u, h.v);
+            }
         }
     }
 }
@@ -99,20 +110,32 @@ __global__ __launch_bounds__(256) void zero_f32_kernel(float* p, size_t n) {
         p[i] = 0.0f;
 }
 
+// Launches of at most 2^31 rays, the kTraceParts work counters (kCounterStride words apart)
+// cleared before each.
 template <int BLOCK, int LDS>
 hipError_t intersect_impl(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
-                          int32_t* occluded, unsigned long long* counter, const TraversalScratch& sc, int grid,
-                          bool wide, hipStream_t stream) {
-    (void)hipGetLastError();
+                          int32_t* occluded, unsigned* counters, const TraversalScratch& sc, int grid, bool wide,
+                          hipStream_t stream) {
+    constexpr uint64_t kMaxRays = 1ull << 31;
     dim3 g(grid), b(BLOCK);
-    if (any_hit) {
-        if (wide) hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, true>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
-        else hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, false>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
-    } else {
-        if (wide) hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, true>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
-        else hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, false>), g, b, 0, stream, s, rays, n, hits, occluded, counter, sc);
+    for (uint64_t first = 0; first < n; first += kMaxRays) {
+        const unsigned m = static_cast<unsigned>(std::min<uint64_t>(kMaxRays, n - first));
+        const sol_ray* rr = rays + first;
+        sol_hit* hh = hits ? hits + first : nullptr;
+        int32_t* oo = occluded ? occluded + first : nullptr;
+        hipError_t e = launch_zero_u64(reinterpret_cast<unsigned long long*>(counters), kIntersectCounterWords / 2, stream);
+        if (e != hipSuccess) return e;
+        (void)hipGetLastError();
+        if (any_hit) {
+            if (wide) hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, true>), g, b, 0, stream, s, rr, m, hh, oo, counters, sc);
+            else hipLaunchKernelGGL((intersect_kernel<true, BLOCK, LDS, false>), g, b, 0, stream, s, rr, m, hh, oo, counters, sc);
+        } else {
+            if (wide) hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, true>), g, b, 0, stream, s, rr, m, hh, oo, counters, sc);
+            else hipLaunchKernelGGL((intersect_kernel<false, BLOCK, LDS, false>), g, b, 0, stream, s, rr, m, hh, oo, counters, sc);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 template <int BLOCK, int LDS>
@@ -146,9 +169,9 @@ hipError_t megakernel_impl(const SceneView& s, const sol_pt_config& cfg, uint32_
     }
 
 hipError_t launch_intersect(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
-                            int32_t* occluded, unsigned long long* work_counter, const TraversalScratch& scratch,
+                            int32_t* occluded, unsigned* work_counters, const TraversalScratch& scratch,
                             const LaunchShape& shape, hipStream_t stream) {
-    SOLHIP_DISPATCH(intersect_impl, s, rays, n, any_hit, hits, occluded, work_counter, scratch, shape.grid,
+    SOLHIP_DISPATCH(intersect_impl, s, rays, n, any_hit, hits, occluded, work_counters, scratch, shape.grid,
                     shape.wide && s.nodes4 != nullptr, stream)
 }
 

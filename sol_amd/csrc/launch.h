@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.h"
+#include "wave.h"
 
 namespace solhip {
 
@@ -26,8 +27,13 @@ struct LaunchShape {
 // Counters a kernel adds to: [0] extension rays, [1] shadow rays, [2] samples.
 using StatCounters = unsigned long long*;
 
+// Work counters of the intersection kernel: kTraceParts counters kCounterStride words apart (an
+// even number of 32-bit words; launch_intersect clears them before each launch).
+constexpr int kIntersectCounterWords = kTraceParts * kCounterStride;
+static_assert(kIntersectCounterWords % 2 == 0, "cleared as 64-bit words");
+
 hipError_t launch_intersect(const SceneView& s, const sol_ray* rays, uint64_t n, bool any_hit, sol_hit* hits,
-                            int32_t* occluded, unsigned long long* work_counter, const TraversalScratch& scratch,
+                            int32_t* occluded, unsigned* work_counters, const TraversalScratch& scratch,
                             const LaunchShape& shape, hipStream_t stream);
 
 hipError_t launch_megakernel(const SceneView& s, const sol_pt_config& cfg, uint32_t width, uint32_t height,

@@ -90,6 +90,7 @@ struct Context {
     WavefrontState* wave = nullptr;
     DeviceBuffer adj_radiance, adj_image;   // forward pass of solhip_backward (grow-only)
     DeviceBuffer isect_rays, isect_out;     // host-pointer staging of solhip_intersect (grow-only)
+    DeviceBuffer isect_counters;            // work counters of the intersection kernel (launch_intersect)
     DeviceBuffer host_image;                // device image of solhip_render for host images (grow-only)
     DeviceBuffer build_ws;                  // temporaries of solhip_scene_build (grow-only)
     void* pinned = nullptr;                 // pinned host staging (grow-only)

@@ -1,6 +1,6 @@
-// Work fetch of the wavefront renderer's persistent traversal kernel (k_trace): guided
-// self-scheduling of a launch's entries over one counter, eight part counters or ranks, and the
-// per-lane refill and leaf-wait thresholds.
+// Work fetch of the persistent traversal kernels (the wavefront renderer's k_trace and the batched
+// intersection kernel): guided self-scheduling of a launch's entries over one counter, eight part
+// counters or ranks; and k_trace's per-lane refill and leaf-wait thresholds.
 #pragma once
 
 #include "common.h"
@@ -47,7 +47,9 @@ struct TraceWork {
         taken = 0;
         const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * block_waves;
         ranked = total <= waves * 64;
-        by_rank = ranked ? (blockIdx.x * block_waves + threadIdx.x / 64) * 64u : 0u;
+        // the wave's index in the workgroup, read as a scalar (the same in every lane of the wave)
+        const unsigned wave = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(threadIdx.x)) / 64u;
+        by_rank = ranked ? (blockIdx.x * block_waves + wave) * 64u : 0u;
     }
     SOL_DEV unsigned begin(unsigned p) const {
         return static_cast<unsigned>(static_cast<unsigned long long>(total) * p / parts);
@@ -73,9 +75,10 @@ SOL_DEV bool trace_fetch(unsigned* counters, unsigned waves, TraceWork& w, unsig
         const unsigned left = size > w.taken ? size - w.taken : 0u;
         unsigned chunk = (left / (4 * per_part)) & ~63u;
         chunk = chunk < 64 ? 64 : (chunk > 1024 ? 1024 : chunk);
+        // one atomic for the wave, by its first active lane; its result is read back as a scalar
         unsigned off = 0;
-        if (__lane_id() == 0) off = atomicAdd(&counters[w.part * kCounterStride], chunk);
-        off = __shfl(off, 0);
+        if (__lane_id() == __builtin_amdgcn_readfirstlane(__lane_id())) off = atomicAdd(&counters[w.part * kCounterStride], chunk);
+        off = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(off));
         if (off < size) {
             w.taken = off;
             base = lo + off;
