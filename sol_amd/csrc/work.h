@@ -56,9 +56,17 @@ struct TraceWork {
     }
 };
 
+// Smallest chunk of the guided self-scheduling below: two entries per lane. One per lane made the
+// atomics of the part counters the bound of launches of a few million entries (Cornell box
+// 256 x 256, 16 spp: 758 -> 775 Msamples/s at 128; 1080p sphere field frame 52.07 -> 51.78 ms;
+// batched intersection of 4M rays on the sphere field, closest hits 5574 -> 5751 Mrays/s, any hits
+// 6333 -> 6674). At 256 the tails of the launches grow (Cornell 769, 1M-triangle soup closest hits
+// 2903 -> 2796 Mrays/s).
+constexpr unsigned kMinChunk = 128;
+
 // Takes the wave's next chunk of entries [base, end); false once there is none left. Within a
 // part, guided self-scheduling: a chunk is (entries left in the part) / (4 x waves per part), a
-// multiple of 64 in [64, 1024], so that a launch starts with few atomics and ends with small
+// multiple of 64 in [kMinChunk, 1024], so that a launch starts with few atomics and ends with small
 // chunks.
 SOL_DEV bool trace_fetch(unsigned* counters, unsigned waves, TraceWork& w, unsigned& base, unsigned& end) {
     if (w.ranked) {
@@ -74,7 +82,7 @@ SOL_DEV bool trace_fetch(unsigned* counters, unsigned waves, TraceWork& w, unsig
         const unsigned size = hi - lo;
         const unsigned left = size > w.taken ? size - w.taken : 0u;
         unsigned chunk = (left / (4 * per_part)) & ~63u;
-        chunk = chunk < 64 ? 64 : (chunk > 1024 ? 1024 : chunk);
+        chunk = chunk < kMinChunk ? kMinChunk : (chunk > 1024 ? 1024 : chunk);
         // one atomic for the wave, by its first active lane; its result is read back as a scalar
         unsigned off = 0;
         if (__lane_id() == __builtin_amdgcn_readfirstlane(__lane_id())) off = atomicAdd(&counters[w.part * kCounterStride], chunk);
