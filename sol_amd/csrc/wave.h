@@ -28,6 +28,14 @@ enum : int {
     kQueueAll = 3          // the merged queue's kernel (its entries are stored as queue 0)
 };
 
+// Diffuse / glossy (Phong, and the diffuse-Phong mix) / specular: the queues follow the BSDF's
+// shading work, the mixed BSDF shares the Phong queue (it runs both lobes).
+SOL_HD int queue_of(int type) {
+    if (type == SOL_BSDF_DIFFUSE) return kQueueDiffuse;
+    if (type == SOL_BSDF_PHONG || type == SOL_BSDF_INTERP) return kQueuePhong;
+    return kQueueSpecular;
+}
+
 // Queues are split into nseg segments (WaveBuf::nseg: 1, 2, 4 or 8, a launch parameter), each with
 // its own length counter: a workgroup appends to segment blockIdx % nseg only, so the device
 // atomics of the appends spread over nseg addresses. Readers see a queue as the concatenation of

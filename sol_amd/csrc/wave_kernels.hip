@@ -298,13 +298,6 @@ SOL_DEV const SceneView& scene_of(const SceneView& single, const WaveTables& t, 
     return tmp;
 }
 
-// Diffuse / glossy (Phong, and the diffuse-Phong mix) / specular: the queues follow the BSDF's
-// shading work, the mixed BSDF shares the Phong queue (it runs both lobes).
-SOL_DEV int queue_of(int type) {
-    if (type == SOL_BSDF_DIFFUSE) return kQueueDiffuse;
-    if (type == SOL_BSDF_PHONG || type == SOL_BSDF_INTERP) return kQueuePhong;
-    return kQueueSpecular;
-}
 
 // ---- work items ------------------------------------------------------------------------------------
 

@@ -21,6 +21,7 @@ struct WaveLaunch {
     int grid_start;           // of the grid must be resident at once
     bool multi;
     bool wide;                // every scene has the 4-wide layout: k_trace traverses it
+    uint32_t shade_mask;      // bit q: material queue q can receive entries (its kernel is launched)
     hipStream_t stream;
 };
 

@@ -220,7 +220,7 @@ hipError_t enqueue_iteration(const WaveLaunch& L, int cur, hipEvent_t* ev) {
         if ((e = wave_launch_shade(L, kQueueAll, nxt)) != hipSuccess) return e;
     } else {
         for (int q = 0; q < kNumShadeQueues; ++q)
-            if ((e = wave_launch_shade(L, q, nxt)) != hipSuccess) return e;
+            if ((L.shade_mask >> q & 1u) && (e = wave_launch_shade(L, q, nxt)) != hipSuccess) return e;
     }
     if (ev && (e = hipEventRecord(ev[3], L.stream)) != hipSuccess) return e;
     return hipSuccess;
@@ -231,10 +231,10 @@ std::string graph_key(const WaveLaunch& L, const WavefrontState& st, int iters) 
     k.append(reinterpret_cast<const char*>(&L.single), sizeof(L.single));
     k.append(reinterpret_cast<const char*>(&L.tabs), sizeof(L.tabs));
     k.append(reinterpret_cast<const char*>(&L.buf), sizeof(L.buf));
-    int meta[13] = {L.block,          L.lds_stack,      L.grid_traverse,  L.grid_shade,
+    int meta[14] = {L.block,          L.lds_stack,      L.grid_traverse,  L.grid_shade,
                     L.grid_sort,      L.grid_kshade[0], L.grid_kshade[1], L.grid_kshade[2],
                     L.grid_kshade[3], L.grid_start,     L.multi ? 1 : 0,  L.wide ? 1 : 0,
-                    iters};
+                    static_cast<int>(L.shade_mask), iters};
     k.append(reinterpret_cast<const char*>(meta), sizeof(meta));
     k.append(reinterpret_cast<const char*>(&st.generation), sizeof(st.generation));
     k.append(reinterpret_cast<const char*>(&L.scratch), sizeof(L.scratch));
@@ -276,7 +276,7 @@ std::vector<uint32_t> reduction_layers(const std::vector<float*>& accum, const s
 
 // Runs the jobs to completion. Jobs index `views` through Job::scene.
 int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::vector<Job>& jobs,
-                  const sol_pt_config& cfg, uint32_t max_depth, uint32_t max_depth4) {
+                  const sol_pt_config& cfg, uint32_t max_depth, uint32_t max_depth4, uint32_t shade_mask) {
     if (!ctx.wave) ctx.wave = new WavefrontState();
     WavefrontState& st = *ctx.wave;
     const solhip_tuning& tu = ctx.tuning;
@@ -414,6 +414,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     L.grid_traverse = persistent_grid(ctx.info.compute_units, L.block);
     L.grid_shade = persistent_grid(ctx.info.compute_units, kWaveBlock);
     L.multi = multi;
+    L.shade_mask = shade_mask;
     L.wide = tu.bvh_width == 4 &&
              std::all_of(views.begin(), views.end(), [](const SceneView& v) { return v.nodes4 != nullptr; });
     HIP_OK(wave_set_grids(L, ctx.info.compute_units));
@@ -474,6 +475,8 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     // what is left in them into a statistic read with the others at the end, which must be zero.
     // (A profiled run reads its events after each replay, so it keeps the round trip.)
     const bool drains_in_one = one_fill && iters == static_cast<int>(need) && !profile;
+    // trace, reset, sort and the shading kernels launched
+    const uint64_t per_iteration = 3 + (L.buf.merged ? 1 : static_cast<uint64_t>(__builtin_popcount(L.shade_mask & 7u)));
     HIP_OK(hipMemsetAsync(st.buf.stats, 0, sizeof(unsigned long long) * kStatWords, s));
     HIP_OK(hipEventRecord(ctx.ev_begin, s));
     uint64_t replays = 0, launches = 0;
@@ -489,7 +492,7 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
                     HIP_OK(enqueue_iteration(L, it & 1, profile ? &st.events[it * 4] : nullptr));
             }
             ++replays;
-            launches += static_cast<uint64_t>(iters) * 6;
+            launches += static_cast<uint64_t>(iters) * per_iteration;
             if (drains_in_one) {
                 HIP_OK(wave_launch_check_drained(L));
                 ++launches;
@@ -541,6 +544,16 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     return SOLHIP_OK;
 }
 
+// Material queues a scene's hits can enter: the queue of each of its materials' BSDFs, and the
+// diffuse queue, which also takes the hits of lights without a BSDF. The kernels of the others
+// are not launched (a scene of diffuse materials shades with one kernel a bounce).
+uint32_t shade_mask_of(const solhip_scene& s) {
+    uint32_t mask = 1u << kQueueDiffuse;
+    for (const sol_material& m : s.materials_host)
+        if (m.type != SOL_BSDF_NONE) mask |= 1u << queue_of(m.type);
+    return mask;
+}
+
 } // namespace
 
 int render_wavefront(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint32_t h, uint64_t si, uint32_t sc,
@@ -548,7 +561,7 @@ int render_wavefront(solhip_scene& s, const sol_pt_config& cfg, uint32_t w, uint
     if (cfg.max_path_len == 0) return SOLHIP_OK;
     std::vector<SceneView> views{s.view};
     std::vector<Job> jobs{Job{0, w, h, si, sc, d_accum, d_samples}};
-    return run_wavefront(*s.ctx, views, jobs, cfg, s.bvh_depth, s.bvh4_depth);
+    return run_wavefront(*s.ctx, views, jobs, cfg, s.bvh_depth, s.bvh4_depth, shade_mask_of(s));
 }
 
 } // namespace solhip
@@ -564,7 +577,7 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
     std::vector<SceneView> views;
     std::unordered_map<const solhip_scene*, uint32_t> index;
     std::vector<Job> list;
-    uint32_t depth = 0, depth4 = 0;
+    uint32_t depth = 0, depth4 = 0, shade_mask = 0;
     for (uint32_t i = 0; i < num_jobs; ++i) {
         const solhip_batch_job& j = jobs[i];
         if (!j.scene || !j.accum || j.width == 0 || j.height == 0) return set_error(SOLHIP_E_INVALID, "invalid batch job");
@@ -579,6 +592,7 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
             views.push_back(j.scene->view);
             depth = std::max(depth, j.scene->bvh_depth);
             depth4 = std::max(depth4, j.scene->bvh4_depth);
+            shade_mask |= shade_mask_of(*j.scene);
         } else {
             si = it->second;
         }
@@ -587,7 +601,7 @@ int solhip_render_batch(solhip_ctx* ctx, const solhip_batch_job* jobs, uint32_t 
     if (list.empty() || cfg->max_path_len == 0) return SOLHIP_OK;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return set_error(SOLHIP_E_HIP, hipGetErrorString(e));
-    int rc = run_wavefront(*ctx, views, list, *cfg, depth, depth4);
+    int rc = run_wavefront(*ctx, views, list, *cfg, depth, depth4, shade_mask);
     if (rc) return rc;
     return ctx->check_overflow();
 }

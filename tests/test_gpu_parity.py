@@ -177,6 +177,35 @@ def test_material_kind_changes_reach_the_wavefront_queues(ctx):
     np.testing.assert_array_equal(a, b)
 
 
+def test_wavefront_launches_only_the_queues_the_materials_use(ctx):
+    # A scene whose materials are all of one kind shades with that kind's kernel (and the diffuse
+    # one, which also takes the lights); changing the kinds, and back, changes the kernels the
+    # captured bounce loop launches.
+    scene = sol_amd.Scene.sphere_field(num_spheres=30, subdivision=2, seed=9, extent=4.0, radius_max=1.0)
+    g = ctx.upload(scene)
+    base = scene.materials()
+    cfg = _cfg(max_path_len=10)
+    a, b = sol_amd.new_image(48, 32), sol_amd.new_image(48, 32)
+    for kind in (F.BSDF_DIFFUSE, F.BSDF_PHONG, F.BSDF_MIRROR, F.BSDF_GLASS, F.BSDF_INTERP, None):
+        mats = scene.materials() if kind is not None else base
+        if kind is not None:
+            for m in mats:
+                if m.type != F.BSDF_NONE:
+                    m.type = kind
+                    m.ks[:] = [0.4, 0.4, 0.4]
+                    m.kt[:] = [1.0, 1.0, 1.0]
+                    m.ns = m.ns_sample = 30.0
+                    m.eta = 1.0 / 1.5
+                    m.mix = 0.5
+        g.set_materials(mats)
+        a[:] = 0
+        b[:] = 0
+        g.render(a, 0, 4, cfg, renderer=F.RENDERER_MEGAKERNEL)
+        g.render(b, 0, 4, cfg, renderer=F.RENDERER_WAVEFRONT)
+        assert a.any()
+        np.testing.assert_array_equal(a, b)
+
+
 def test_wavefront_profiling_mode_matches_graph_mode(ctx, cornell):
     scene, _ = cornell
     g = ctx.upload(scene)
