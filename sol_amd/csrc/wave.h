@@ -94,10 +94,11 @@ enum : int {
     kItemWords = kItemChunkBegin + 1
 };
 
-// One bounce of all paths in flight is four launches: trace (closest hit of the extension rays
-// and any hit of the previous bounce's shadow rays), reset, sort (material queues from the hit
-// triangle's kind byte; paths that missed end there) and shade (one kernel per material queue:
-// hit decoding, emission, NEE, Russian roulette, BSDF sampling).
+// One bounce of all paths in flight is trace (closest hit of the extension rays and any hit of
+// the previous bounce's shadow rays), sort (material queues from the hit triangle's kind byte;
+// paths that missed end there) and shade (one kernel per material queue in use: hit decoding,
+// emission, NEE, Russian roulette, BSDF sampling); trace and sort also clear the counters the
+// kernels after them append to.
 // A path's state travels with it in its queue entries, written at compacted positions
 // (coalesced, field-major): extension rays (64 B: the ray and the path state), hits (16 B), shade
 // entries (4 B: the index of the extension entry and its hit), shadow rays (48 B). The ray, hit,

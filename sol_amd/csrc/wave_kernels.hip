@@ -418,6 +418,32 @@ SOL_DEV Stack make_stack_w(int* lds_stack, const TraversalScratch& sc, int lds_e
 
 // ---- trace -----------------------------------------------------------------------------------------
 
+// Counters are cleared by the first threads of workgroup 0 of a kernel that does not use them,
+// between their last reader and their first writer (no kernel of its own: a launch costs about
+// 5 µs in the bounce loop, and no memset, which graph replays do not take well).
+//   trace(k): the shade queues and the next extension queue, which sort(k) and shade(k) append to
+//   (their last readers were shade(k-1) and trace(k-1) / sort(k-1)).
+//   sort(k): the shadow queue, which trace(k) has read and shade(k) appends to, and the work
+//   counters of the traversal kernel, which trace(k) has used and trace(k+1) uses. When cur is 1,
+//   extension queue 0 has been traced and shaded since the last chunk started, so it no longer holds
+//   a start fill: the flag is cleared too.
+SOL_DEV void clear_for_sort(const WaveBuf& b, int cur) {
+    constexpr int n = (kNumShadeQueues + 1) * kMaxSegments;
+    if (blockIdx.x == 0 && threadIdx.x < n) {
+        const int q = static_cast<int>(threadIdx.x) / kMaxSegments, g = static_cast<int>(threadIdx.x) % kMaxSegments;
+        b.counts[qcount(q < kNumShadeQueues ? kQShade0 + q : kQExtend0 + (1 - cur), g)] = 0;
+    }
+}
+SOL_DEV void clear_for_shade(const WaveBuf& b, int cur) {
+    if (blockIdx.x == 0 && threadIdx.x < kMaxSegments + kTraceParts) {
+        const int i = static_cast<int>(threadIdx.x);
+        if (i < kMaxSegments) b.counts[qcount(kQShadow, i)] = 0;
+        else b.counts[kCntTraceWork + (i - kMaxSegments) * kCounterStride] = 0;
+        if (i == 0 && cur == 1) b.counts[kCntStartFill] = 0;
+    }
+}
+
+
 // Persistent traversal with per-lane refill: every lane runs its own ray one traversal step at a
 // time; when kRefillLanes lanes are idle the wave hands them the next entries of its chunk
 // (fetching a new chunk when the current one runs out). Entries [0, n_ext) are the closest-hit
@@ -429,6 +455,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                                                  int cur, TraversalScratch sc) {
     __shared__ float4 lds_nodes[kTopLdsFloat4<WIDE>];
     __shared__ int lds_stack[LDS * BLOCK];
+    clear_for_sort(b, cur);
     const SegQueue eqv(b, kQExtend0 + cur, b.cap_e), sqv(b, kQShadow, b.cap_s);
     const unsigned n_ext = eqv.size();
     const unsigned n_shadow = sqv.size();
@@ -583,6 +610,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                                                  int cur) {
     constexpr int K = kSortItems;
     __shared__ RoundLdsK<kNumShadeQueues, K> lds[2];
+    clear_for_shade(b, cur);
     const SegQueue eqv(b, kQExtend0 + cur, b.cap_e);
     const unsigned count = eqv.size();
     const int nxt = 1 - cur;
@@ -757,7 +785,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
     wave_add_stat(b.stats, 2, started);
 }
 
-// ---- start / reset / reduce ------------------------------------------------------------------------
+// ---- start / chunk / reduce ------------------------------------------------------------------------
 
 // The start fill lays the slots out as the rounds of the other passes would: slot s is in round
 // s / kBlock, round r in segment r % nseg, and a slot's place in its segment follows the rounds in
@@ -802,16 +830,6 @@ __global__ __launch_bounds__(kBlock) void k_start(WaveTables tabs, WaveBuf b, in
         }
     }
     wave_add_stat(b.stats, 2, started);
-}
-
-// After trace(k) has consumed extension queue `cur` and the shadow queue: clears what sort(k) and
-// shade(k) append to (kernel, not a memset: it is captured in graphs). When cur is 1, extension
-// queue 0 has been traced, sorted and shaded since the last chunk started, so it no longer holds a
-// start fill.
-__global__ void k_reset(WaveBuf b, int cur) {
-    int i = threadIdx.x;
-    if (i < kNumWaveCounters && i / kMaxSegments != kQExtend0 + cur) b.counts[i * kCounterStride] = 0;
-    if (i == 0 && cur == 1) b.counts[kCntStartFill] = 0;
 }
 
 // Work items of a chunk [begin, end) by segment, consecutive ranges in segment order: segment g
@@ -982,12 +1000,6 @@ hipError_t wave_launch_shade(const WaveLaunch& L, int queue, int nxt) {
 hipError_t wave_launch_start(const WaveLaunch& L, int nxt) {
     (void)hipGetLastError();
     hipLaunchKernelGGL(k_start, dim3(L.grid_start), dim3(kBlock), 0, L.stream, L.tabs, L.buf, nxt);
-    return hipGetLastError();
-}
-
-hipError_t wave_launch_reset(const WaveLaunch& L, int cur) {
-    (void)hipGetLastError();
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, L.stream, L.buf, cur);
     return hipGetLastError();
 }
 

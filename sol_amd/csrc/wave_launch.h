@@ -34,7 +34,6 @@ hipError_t wave_launch_sort(const WaveLaunch& L, int cur);
 // queue: kQueueDiffuse, kQueuePhong, kQueueSpecular, or kQueueAll (the merged queue)
 hipError_t wave_launch_shade(const WaveLaunch& L, int queue, int nxt);
 hipError_t wave_launch_start(const WaveLaunch& L, int nxt);
-hipError_t wave_launch_reset(const WaveLaunch& L, int cur);
 hipError_t wave_launch_set_chunk(const WaveLaunch& L, unsigned long long begin, unsigned long long end);
 hipError_t wave_launch_check_drained(const WaveLaunch& L);
 hipError_t wave_launch_reduce(const WaveLaunch& L, uint32_t first_tile, uint32_t num_tiles, uint32_t max_tile_pixels,

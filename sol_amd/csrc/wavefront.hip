@@ -205,7 +205,7 @@ void Context::release_wavefront() {
 namespace {
 
 // One bounce of every path in flight: trace (closest hits of queue `cur`, any hits of the shadow
-// queue), reset, sort, shade (three material queues, or the merged one). Paths that end in sort or
+// queue), sort, shade (the material queues in use, or the merged one). Paths that end in sort or
 // shade start their slot's next work item there.
 hipError_t enqueue_iteration(const WaveLaunch& L, int cur, hipEvent_t* ev) {
     int nxt = 1 - cur;
@@ -213,7 +213,6 @@ hipError_t enqueue_iteration(const WaveLaunch& L, int cur, hipEvent_t* ev) {
     if (ev && (e = hipEventRecord(ev[0], L.stream)) != hipSuccess) return e;
     if ((e = wave_launch_trace(L, cur)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[1], L.stream)) != hipSuccess) return e;
-    if ((e = wave_launch_reset(L, cur)) != hipSuccess) return e;
     if ((e = wave_launch_sort(L, cur)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[2], L.stream)) != hipSuccess) return e;
     if (L.buf.merged) {
@@ -475,8 +474,8 @@ int run_wavefront(Context& ctx, const std::vector<SceneView>& views, const std::
     // what is left in them into a statistic read with the others at the end, which must be zero.
     // (A profiled run reads its events after each replay, so it keeps the round trip.)
     const bool drains_in_one = one_fill && iters == static_cast<int>(need) && !profile;
-    // trace, reset, sort and the shading kernels launched
-    const uint64_t per_iteration = 3 + (L.buf.merged ? 1 : static_cast<uint64_t>(__builtin_popcount(L.shade_mask & 7u)));
+    // trace, sort and the shading kernels launched
+    const uint64_t per_iteration = 2 + (L.buf.merged ? 1 : static_cast<uint64_t>(__builtin_popcount(L.shade_mask & 7u)));
     HIP_OK(hipMemsetAsync(st.buf.stats, 0, sizeof(unsigned long long) * kStatWords, s));
     HIP_OK(hipEventRecord(ctx.ev_begin, s));
     uint64_t replays = 0, launches = 0;
