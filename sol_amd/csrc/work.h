@@ -25,19 +25,27 @@ SOL_DEV unsigned long long lanes_below() {
 // entries 53.7 (41.7), below 16M 53.7 (41.5).
 constexpr unsigned kTracePartsBelow = 16u << 20;
 
+// Launches of at most this many batches of 64 entries per wave are handed out by rank, with no
+// atomic. A device atomic on one address waits behind the others on it: when the waves of a launch
+// fetch faster than they trace, as in a launch of a million primary rays into the Cornell box,
+// the part counters' atomics took 105 of its 119 µs with the traversal switched off (118.6 -> 13.0
+// µs by rank). Launches of up to 4 / 16 / 64 / 512 batches a wave by rank: Cornell box 830 ->
+// 993 / 998 / 994 / 992 Msamples/s, 1080p frame 51.8 -> 51.85 / 52.0 / 52.2 / 64.4 ms (ray costs
+// vary, and a large launch needs the dynamic hand-out).
+constexpr unsigned kRankedBatches = 4;
+
 // A wave's place in the traversal kernel's work, the entries [0, total). A launch with no more
-// entries than 64 per wave is handed out by rank: wave g takes entries [64 g, 64 g + 64) and no
-// atomic is made (every persistent wave of such a launch would otherwise make at least one on a
-// few addresses, which set a floor of about 0.1 ms under the small launches at the end of a
-// frame). Larger launches are handed out from `parts` consecutive parts by their counters; the
-// waves of workgroup b take from part b mod parts first (with eight parts, the XCD that runs them,
-// as workgroups are dealt to the XCDs round robin), then from the following parts.
+// entries than kRankedBatches x 64 per wave is handed out by rank: wave g of W takes the batches of
+// 64 entries g, g + W, g + 2W, ... and no atomic is made. Larger launches are handed out from
+// `parts` consecutive parts by their counters; the waves of workgroup b take from part b mod parts
+// first (with eight parts, the XCD that runs them, as workgroups are dealt to the XCDs round
+// robin), then from the following parts.
 struct TraceWork {
     unsigned total, parts;
     unsigned part;          // the part it takes from now
     unsigned parts_left;    // parts not yet found used up
     unsigned taken;         // the part's counter as the wave last saw it
-    unsigned by_rank;       // the wave's entries when handed out by rank: [by_rank, by_rank + 64)
+    unsigned by_rank;       // the wave's next batch when handed out by rank: [by_rank, by_rank + 64)
     bool ranked;            // handed out by rank
 
     SOL_DEV TraceWork(unsigned total_, unsigned block_waves) : total(total_) {
@@ -46,7 +54,7 @@ struct TraceWork {
         parts_left = parts;
         taken = 0;
         const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * block_waves;
-        ranked = total <= waves * 64;
+        ranked = total <= waves * 64 * kRankedBatches;
         // the wave's index in the workgroup, read as a scalar (the same in every lane of the wave)
         const unsigned wave = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(threadIdx.x)) / 64u;
         by_rank = ranked ? (blockIdx.x * block_waves + wave) * 64u : 0u;
@@ -73,7 +81,8 @@ SOL_DEV bool trace_fetch(unsigned* counters, unsigned waves, TraceWork& w, unsig
         if (w.by_rank >= w.total) return false;
         base = w.by_rank;
         end = min(base + 64u, w.total);
-        w.by_rank = w.total;
+        const unsigned stride = waves * 64u;   // a persistent grid holds a few thousand waves
+        w.by_rank = w.total - base > stride ? base + stride : w.total;
         return true;
     }
     const unsigned per_part = waves / w.parts > 0 ? waves / w.parts : 1u;
