@@ -461,6 +461,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
     const unsigned n_shadow = sqv.size();
     const unsigned total = n_ext + n_shadow;
     if (total == 0) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&b.stats[0], static_cast<unsigned long long>(n_ext));
+        atomicAdd(&b.stats[1], static_cast<unsigned long long>(n_shadow));
+    }
     // extension queue 0 holding a chunk's start fill: origins are the camera's eye
     const bool start_fill = cur == 0 && b.counts[kCntStartFill] != 0;
     if (!MULTI) {
@@ -481,7 +485,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
     unsigned c_next = 0, c_end = 0;   // the wave's current chunk [c_next, c_end)
     TraceWork work(total, BLOCK / 64);
     bool exhausted = false;
-    unsigned n_closest = 0, n_any = 0;
+    unsigned n_holes = 0;   // extension entries without a ray
     while (true) {
         const unsigned long long idle = __ballot(idx < 0);
         const unsigned n_idle = static_cast<unsigned>(__popcll(idle));
@@ -499,7 +503,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                     float4 e1 = *b.ext(cur, a, 1);
                     uint32_t slot = as_u(e1.w);
                     // a hole's hit reads as a miss (the sort of an in-place chunk reads only hits)
-                    if (slot == kHole) b.hits[e] = make_float4(as_f(0xFFFFFFFFu), 0.0f, 0.0f, 0.0f);
+                    if (slot == kHole) {
+                        b.hits[e] = make_float4(as_f(0xFFFFFFFFu), 0.0f, 0.0f, 0.0f);
+                        ++n_holes;
+                    }
                     if (slot != kHole) {
                         idx = static_cast<int>(e);
                         any = false;
@@ -518,7 +525,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                             fg = FetchGlobal(sv);
                             fg4 = FetchGlobal4(sv);
                         }
-                        ++n_closest;
                     }
                 } else if (e != 0xFFFFFFFFu) {
                     const unsigned sa = sqv.entry(e - n_ext);
@@ -541,7 +547,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         fg = FetchGlobal(tabs.scenes[sid]);
                         fg4 = FetchGlobal4(tabs.scenes[sid]);
                     }
-                    ++n_any;
                 }
             }
             if (avail >= n_idle) {
@@ -588,8 +593,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
             }
         }
     }
-    wave_add_stat(b.stats, 0, n_closest);
-    wave_add_stat(b.stats, 1, n_any);
+    // rays traced: the launch's entries, counted once, less the holes (added modulo 2^64)
+    wave_add_stat(b.stats, 0, 0ull - static_cast<unsigned long long>(n_holes));
 }
 
 // ---- sort ------------------------------------------------------------------------------------------
