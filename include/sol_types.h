@@ -162,7 +162,9 @@ typedef struct sol_gpu_tri {
     float v0[3];
     int32_t prim;
     float v1[3];
-    int32_t pad0;
+    int32_t pad0;      /* ignored on upload; the GPU library keeps the triangle's kind here (BSDF
+                          of its material + 1 in bits 0-2, bit 3 if it is a light), recomputed on
+                          upload and whenever the materials change, for the wavefront traversal */
     float v2[3];
     int32_t pad1;
 } sol_gpu_tri;

@@ -33,20 +33,20 @@ int fail(int code, const std::string& msg) {
         }                                                                                        \
     } while (0)
 
-// BSDF kind of every shading record, and every triangle's kind byte, from the current material
-// table.
-__global__ void k_shade_kinds(sol_gpu_shade* shade, uint8_t* kinds, uint32_t n, const sol_material* mats) {
+// BSDF kind of every shading record, and every triangle's kind word (which the traversal kernel
+// packs into its hits), from the current material table.
+__global__ void k_shade_kinds(sol_gpu_shade* shade, sol_gpu_tri* tris, uint32_t n, const sol_material* mats) {
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int m = shade[i].material;
     const int bsdf = m >= 0 ? mats[m].type : SOL_BSDF_NONE;
     shade[i].bsdf = bsdf;
-    kinds[i] = tri_kind(bsdf, shade[i].light);
+    tris[i].pad0 = tri_kind(bsdf, shade[i].light);
 }
 
 int refresh_kinds(solhip_scene* s) {
     hipLaunchKernelGGL(k_shade_kinds, dim3((s->num_tris + 255) / 256), dim3(256), 0, s->ctx->stream, s->shade,
-                       s->kinds, s->num_tris, s->materials);
+                       s->tris, s->num_tris, s->materials);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     return SOLHIP_OK;
@@ -321,7 +321,6 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     }
     HIP_TRY(hipMalloc(&s->tris, sizeof(sol_gpu_tri) * d->num_tris));
     HIP_TRY(hipMalloc(&s->shade, sizeof(sol_gpu_shade) * d->num_tris));
-    HIP_TRY(hipMalloc(&s->kinds, d->num_tris));
     HIP_TRY(hipMalloc(&s->materials, sizeof(sol_material) * std::max<uint32_t>(1, d->num_materials)));
     HIP_TRY(hipMalloc(&s->lights, sizeof(sol_light) * std::max<uint32_t>(1, d->num_lights)));
     HIP_TRY(hipMemcpy(s->nodes, d->nodes, sizeof(sol_gpu_node) * d->num_nodes, hipMemcpyHostToDevice));
@@ -338,7 +337,6 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     v.top_nodes4 = std::min<uint32_t>(d->top_nodes4, d->num_nodes4);
     v.tris = reinterpret_cast<const float4*>(s->tris);
     v.shade = reinterpret_cast<const float4*>(s->shade);
-    v.kinds = s->kinds;
     v.materials = s->materials;
     v.lights = s->lights;
     v.num_lights = d->num_lights;
@@ -360,7 +358,6 @@ void solhip_scene_destroy(solhip_scene* s) {
     if (s->nodes4) (void)hipFree(s->nodes4);
     (void)hipFree(s->tris);
     (void)hipFree(s->shade);
-    (void)hipFree(s->kinds);
     (void)hipFree(s->materials);
     (void)hipFree(s->lights);
     solhip_ctx* ctx = static_cast<solhip_ctx*>(s->ctx);

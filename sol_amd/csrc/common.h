@@ -160,7 +160,6 @@ struct SceneView {
     uint32_t top_nodes4;
     const float4* tris;    // 3 x float4 per triangle (sol_gpu_tri), leaf order
     const float4* shade;   // 4 x float4 per triangle (sol_gpu_shade), leaf order
-    const uint8_t* kinds;  // 1 byte per triangle, leaf order: tri_kind(bsdf, light)
     const sol_material* materials;
     const sol_light* lights;
     uint32_t num_lights;
@@ -170,14 +169,20 @@ struct SceneView {
     sol_camera camera;
 };
 
-// What the hit sorting needs to know of a triangle, in one byte so that the per-triangle table
-// (leaf order, 1 B a triangle) stays in L2: the BSDF kind of its material + 1 (0: no BSDF) in bits
-// 0-2, bit 3 set if the triangle is a light.
+// What the hit sorting needs to know of a triangle, its kind: the BSDF kind of its material + 1
+// (0: no BSDF) in bits 0-2, bit 3 set if the triangle is a light.
 SOL_HD uint8_t tri_kind(int bsdf, int light) {
     return static_cast<uint8_t>((bsdf + 1) | (light >= 0 ? 8 : 0));
 }
 SOL_HD int kind_bsdf(uint32_t kind) { return static_cast<int>(kind & 7u) - 1; }
 SOL_HD bool kind_emissive(uint32_t kind) { return (kind & 8u) != 0; }
+
+// The wavefront renderer's hits carry the hit triangle's kind next to its leaf-order index: index
+// | kind << kHitKindShift (scenes hold fewer than 2^28 triangles, and a kind is at most 13, so no
+// hit reads as the miss marker 0xFFFFFFFF). The kind comes from the fourth word of the triangle's
+// second vertex row (sol_gpu_tri::pad0), which the library keeps equal to tri_kind.
+constexpr uint32_t kHitKindShift = 28;
+constexpr uint32_t kHitIndexMask = (1u << kHitKindShift) - 1u;
 
 // ---- ray / triangle ------------------------------------------------------------------------------
 

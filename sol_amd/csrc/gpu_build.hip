@@ -536,7 +536,7 @@ __global__ void k_single_leaf(int root, uint32_t n, const float4* nlo, const flo
 // cross(v1 - v0, v2 - v0) in fp32 without contraction).
 __global__ void k_gather(const unsigned* order, uint32_t n, const float* P, const float* N, const uint32_t* I,
                          const int32_t* tmat, const int32_t* tlight, const sol_material* mats, sol_gpu_tri* tris,
-                         sol_gpu_shade* shade, uint8_t* kinds) {
+                         sol_gpu_shade* shade) {
     uint32_t s = blockIdx.x * kBB + threadIdx.x;
     if (s >= n) return;
     const uint32_t t = order[s];
@@ -547,7 +547,9 @@ __global__ void k_gather(const unsigned* order, uint32_t n, const float* P, cons
     g.v1[0] = b.x; g.v1[1] = b.y; g.v1[2] = b.z;
     g.v2[0] = c.x; g.v2[1] = c.y; g.v2[2] = c.z;
     g.prim = static_cast<int32_t>(t);
-    g.pad0 = g.pad1 = 0;
+    const int32_t bsdf = tmat[t] >= 0 ? mats[tmat[t]].type : SOL_BSDF_NONE;
+    g.pad0 = tri_kind(bsdf, tlight[t]);   // the kind word the traversal kernel packs into its hits
+    g.pad1 = 0;
     tris[s] = g;
     sol_gpu_shade h;
     const uint32_t idx[3] = {i0, i1, i2};
@@ -561,7 +563,6 @@ __global__ void k_gather(const unsigned* order, uint32_t n, const float* P, cons
     h.bsdf = h.material >= 0 ? mats[h.material].type : SOL_BSDF_NONE;
     h.pad = 0;
     shade[s] = h;
-    kinds[s] = tri_kind(h.bsdf, h.light);
 }
 
 // SAH cost of the binary layout: C_t x area of every inner node + C_i x triangles x area of every
@@ -831,19 +832,17 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     // 4. leaf-order triangles and shading records
     Buf<sol_gpu_tri> tris;
     Buf<sol_gpu_shade> shade;
-    Buf<uint8_t> kinds;
     Buf<sol_material> mats;
     Buf<sol_light> lights;
     BUILD_TRY(tris.alloc(n));
     BUILD_TRY(shade.alloc(n));
-    BUILD_TRY(kinds.alloc(n));
     BUILD_TRY(mats.alloc(m.num_materials));
     BUILD_TRY(lights.alloc(m.num_lights));
     if (m.num_materials)
         BUILD_TRY(hipMemcpyAsync(mats.p, m.materials, sizeof(sol_material) * m.num_materials, hipMemcpyHostToDevice, st));
     BUILD_TRY(hipMemcpyAsync(lights.p, m.lights, sizeof(sol_light) * m.num_lights, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kBB), 0, st, w.order, n, w.P, w.N, w.I, w.tmat, w.tlight,
-                       mats.p, tris.p, shade.p, kinds.p);
+                       mats.p, tris.p, shade.p);
     BUILD_TRY(hipGetLastError());
     float sah = 0.0f;
     if (stats) {
@@ -875,7 +874,6 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     s->nodes4 = gnodes4.release();
     s->tris = tris.release();
     s->shade = shade.release();
-    s->kinds = kinds.release();
     s->materials = mats.release();
     s->lights = lights.release();
     SceneView& v = s->view;
@@ -885,7 +883,6 @@ int build_scene(Context& ctx, const solhip_mesh_desc& m, const solhip_build_conf
     v.top_nodes4 = top_nodes4;
     v.tris = reinterpret_cast<const float4*>(s->tris);
     v.shade = reinterpret_cast<const float4*>(s->shade);
-    v.kinds = s->kinds;
     v.materials = s->materials;
     v.lights = s->lights;
     v.num_lights = m.num_lights;

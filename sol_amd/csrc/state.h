@@ -120,7 +120,6 @@ struct solhip_scene {
     sol_gpu_node4* nodes4 = nullptr;   // may be null (the wavefront traversal then uses nodes)
     sol_gpu_tri* tris = nullptr;
     sol_gpu_shade* shade = nullptr;
-    uint8_t* kinds = nullptr;          // tri_kind of every triangle (leaf order)
     sol_material* materials = nullptr;
     sol_light* lights = nullptr;
     uint32_t num_nodes = 0, num_tris = 0, num_materials = 0, num_lights = 0, bvh_depth = 0, bvh4_depth = 0;

@@ -102,7 +102,7 @@ struct FetchUniform {
     SOL_DEV void tri(uint32_t i, float4& p0, float4& p1, float4& p2) const {
         uint32_t o = i * 48u;
         p0 = ld4(tris, o);
-        p1 = ld3(tris, o + 16);
+        p1 = ld4(tris, o + 16);   // w: the triangle's kind word
         p2 = ld3(tris, o + 32);
     }
 };
@@ -149,6 +149,14 @@ struct TraversalState {
 
 enum : int { kStepContinue = 0, kStepDone = 1, kStepOccluded = 2 };
 
+// The hit's leaf-order triangle index, with the triangle's kind (the w word of its second vertex
+// row) packed above it when PACK_KIND (the wavefront traversal kernel: its hit sorting then needs
+// no per-triangle table).
+template <bool PACK_KIND>
+SOL_DEV int hit_slot(uint32_t index, const float4& p1) {
+    return static_cast<int>(PACK_KIND ? (index | (__float_as_uint(p1.w) << kHitKindShift)) : index);
+}
+
 template <int BLOCK, int LDS_STACK>
 SOL_DEV bool stack_pop(const Stack& st, TraversalState& ts) {
     if (ts.sp == 0) return false;
@@ -163,7 +171,7 @@ SOL_DEV bool stack_pop(const Stack& st, TraversalState& ts) {
 // triangles, pop. Returns kStepDone when the stack is empty (closest hit in ts.hit, if any),
 // kStepOccluded when an any-hit query found a hit, else kStepContinue. Descending first and
 // testing leaves after keeps the lanes of a wave in the same code (inner nodes, then leaves).
-template <int BLOCK, int LDS_STACK, typename Fetch>
+template <int BLOCK, int LDS_STACK, bool PACK_KIND = false, typename Fetch>
 SOL_DEV int traverse_step(const Fetch& f, const Stack& st, const RayPre& r, float tmin, bool any, TraversalState& ts) {
     while (ts.node >= 0) {
         float4 a, b, c, d;
@@ -205,7 +213,7 @@ SOL_DEV int traverse_step(const Fetch& f, const Stack& st, const RayPre& r, floa
                 if (t > tmin && t < ts.tmax) return kStepOccluded;
             } else if (t > tmin && (t < ts.tmax || (t == ts.tmax && ts.hit.prim >= 0 && prim < ts.hit.prim))) {
                 ts.tmax = t;
-                ts.hit.slot = static_cast<int>(first + i);
+                ts.hit.slot = hit_slot<PACK_KIND>(first + i, p1);
                 ts.hit.prim = prim;
                 ts.hit.t = t;
                 ts.hit.u = u;
@@ -347,7 +355,7 @@ struct FetchUniform4 {
     SOL_DEV void tri(uint32_t i, float4& p0, float4& p1, float4& p2) const {
         uint32_t o = i * 48u;
         p0 = FetchUniform::ld4(tris, o);
-        p1 = FetchUniform::ld3(tris, o + 16);
+        p1 = FetchUniform::ld4(tris, o + 16);   // w: the triangle's kind word
         p2 = FetchUniform::ld3(tris, o + 32);
     }
 };
@@ -460,7 +468,7 @@ SOL_DEV void order_pair(float& ta, int& ca, float& tb, int& cb) {
 // With LEAF_WAIT > 0 the node loop also ends once no more than LEAF_WAIT lanes are still looking
 // for their first leaf while some lane holds one: those lanes keep their node and stack and go on
 // in the next step, instead of the whole wave running node iterations for a handful of them.
-template <int BLOCK, int LDS_STACK, int LEAF_WAIT = 0, typename Fetch>
+template <int BLOCK, int LDS_STACK, int LEAF_WAIT = 0, bool PACK_KIND = false, typename Fetch>
 SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, float tmin, bool any, TraversalState& ts) {
     constexpr float kMiss = __builtin_huge_valf();
     int leaf = 0;          // postponed leaf (leaf references are negative), 0 = none
@@ -518,7 +526,7 @@ SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, flo
                     if (t > tmin && t < ts.tmax) return kStepOccluded;
                 } else if (t > tmin && (t < ts.tmax || (t == ts.tmax && ts.hit.prim >= 0 && prim < ts.hit.prim))) {
                     ts.tmax = t;
-                    ts.hit.slot = static_cast<int>(i);
+                    ts.hit.slot = hit_slot<PACK_KIND>(i, p1);
                     ts.hit.prim = prim;
                     ts.hit.t = t;
                     ts.hit.u = u;

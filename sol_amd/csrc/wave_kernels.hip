@@ -3,9 +3,9 @@
 //            bounce's shadow rays (unoccluded contribution added to the path or its sample);
 //            traversal only, work fetched by waves in shrinking chunks (guided self-scheduling)
 //            from one counter, or from eight part counters or by rank in smaller launches.
-//   k_sort   material queues from each hit triangle's kind byte; a path that missed ends there (its
-//            radiance goes to its sample entry, unless the chunk runs in place) and its slot starts
-//            the next work item.
+//   k_sort   material queues from each hit triangle's kind (packed into the hit by the traversal
+//            kernel); a path that missed ends there (its radiance goes to its sample entry, unless
+//            the chunk runs in place) and its slot starts the next work item.
 //   k_shade  one kernel per material queue: hit decoding, emission, NEE, Russian roulette, BSDF
 //            sampling.
 //   k_start  fills every slot at the beginning of a chunk of work.
@@ -34,6 +34,7 @@ constexpr uint32_t kFrontBit = 1u << 30;
 constexpr uint32_t kSampleTarget = 0xFFFFFFFFu;
 constexpr int kBlock = kWaveBlock;
 constexpr uint32_t kHole = 0xFFFFFFFFu;     // slot field of an unused extension entry
+constexpr uint32_t kMissWord = 0xFFFFFFFFu;  // first word of a miss's hit
 constexpr int kSortItems = kWaveSortItems;  // entries per lane and round of the sort
 constexpr int kWaves = kBlock / 64;
 
@@ -564,10 +565,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
         }
         if (idx >= 0) {
             int status;
-            if (WIDE) status = MULTI ? traverse4_step<BLOCK, LDS, kLeafWaitLanes>(fg4, st, r, tmin, any, ts)
-                                     : traverse4_step<BLOCK, LDS, kLeafWaitLanes>(fu4, st, r, tmin, any, ts);
-            else status = MULTI ? traverse_step<BLOCK, LDS>(fg, st, r, tmin, any, ts)
-                                : traverse_step<BLOCK, LDS>(fu, st, r, tmin, any, ts);
+            if (WIDE) status = MULTI ? traverse4_step<BLOCK, LDS, kLeafWaitLanes, true>(fg4, st, r, tmin, any, ts)
+                                     : traverse4_step<BLOCK, LDS, kLeafWaitLanes, true>(fu4, st, r, tmin, any, ts);
+            else status = MULTI ? traverse_step<BLOCK, LDS, true>(fg, st, r, tmin, any, ts)
+                                : traverse_step<BLOCK, LDS, true>(fu, st, r, tmin, any, ts);
             if (status != kStepContinue) {
                 const unsigned e = static_cast<unsigned>(idx);
                 if (!any) {
@@ -608,11 +609,10 @@ SOL_DEV int queue_of_kind(uint32_t kind) {
 }
 
 // Material queues of the extension queue's hits: each hit's entry index goes to the queue of its
-// triangle's kind (a byte per triangle, held in L2). A path that missed, or hit a triangle that
-// neither scatters nor emits, ends here; its slot starts the next work item.
-template <bool MULTI>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sort(SceneView single, WaveTables tabs, WaveBuf b, sol_pt_config cfg,
-                                                 int cur) {
+// triangle's kind (packed into the hit by the traversal kernel, so the sort reads nothing but the
+// hit, in one scene as in many). A path that missed, or hit a triangle that neither scatters nor
+// emits, ends here; its slot starts the next work item.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sort(WaveTables tabs, WaveBuf b, int cur) {
     constexpr int K = kSortItems;
     __shared__ RoundLdsK<kNumShadeQueues, K> lds[2];
     clear_for_shade(b, cur);
@@ -624,35 +624,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                                        &b.counts[qcount(kQShade0 + 2, seg)]};
     unsigned* ecnt = &b.counts[qcount(kQExtend0 + nxt, seg)];
     const bool restart = items_left(b);   // uniform
-    // the slot word: for the scene of a multi-scene path, or to retire an ending path (an in-place
-    // chunk's paths end where they are)
-    const bool need_slot = MULTI || !b.in_place;
+    // the slot word: to retire an ending path (an in-place chunk's paths end where they are)
+    const bool need_slot = !b.in_place;
     unsigned started = 0, parity = 0;
     for (unsigned round = blockIdx.x; round * (K * kBlock) < count; round += gridDim.x, parity ^= 1) {
         int dest[K];
         bool dead[K], ret[K];
         uint32_t slot[K];
         unsigned ea[K];
-        int tri[K];
+        uint32_t tri[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {   // every load of the round first
             const unsigned i = (round * K + k) * kBlock + threadIdx.x;
             const bool valid = i < count;
             ea[k] = valid ? eqv.entry(i) : 0u;
-            tri[k] = valid ? static_cast<int>(as_u(b.hits[i].x)) : -2;   // a hole's reads as a miss
+            tri[k] = valid ? as_u(b.hits[i].x) : kMissWord;   // index | kind; a hole's reads as a miss
             slot[k] = valid && need_slot ? as_u(b.ext(cur, ea[k], 1)->w) : 0u;
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const bool valid = tri[k] != -2 && !(need_slot && slot[k] == kHole);
+            const unsigned i = (round * K + k) * kBlock + threadIdx.x;
+            const bool valid = i < count && !(need_slot && slot[k] == kHole);
             dest[k] = -1;
             dead[k] = false;
             if (valid) {
-                uint32_t kind = 0;
-                if (tri[k] >= 0) {
-                    SceneView tmp;
-                    kind = scene_of<MULTI>(single, tabs, b, slot[k], tmp).kinds[tri[k]];
-                }
+                const uint32_t kind = tri[k] == kMissWord ? 0u : tri[k] >> kHitKindShift;
                 if (kind == 0) dead[k] = true;   // a miss, or no BSDF and no light
                 else dest[k] = b.merged ? 0 : queue_of_kind(kind);
                 if (dead[k] && !b.in_place) retire(b, *b.rec(slot[k]), item_of(*b.ext(cur, ea[k], 3)));
@@ -723,7 +719,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QUEUE ==
             const uint32_t depth0 = as_u(e0.w);
             const float4 e2 = ext_throughput(b, cur, ea, depth0);
             const V3 dir = xyz(e1);
-            const Surface sf = decode_hit(s, xyz(e0), dir, static_cast<int>(as_u(h.x)), h.y, h.z, h.w);
+            const Surface sf = decode_hit(s, xyz(e0), dir, static_cast<int>(as_u(h.x) & kHitIndexMask), h.y, h.z, h.w);
             const float pick = 1.0f / static_cast<float>(s.num_lights);
             item = item_of(e3);
             V3 T = xyz(e2);
@@ -951,14 +947,13 @@ hipError_t resident_grid(K kernel, int compute_units, int cap, int& grid) {
 hipError_t wave_set_grids(WaveLaunch& L, int compute_units) {
     hipError_t e;
     const int cap = L.grid_shade;
+    if ((e = resident_grid(k_sort, compute_units, cap, L.grid_sort)) != hipSuccess) return e;
     if (L.multi) {
-        if ((e = resident_grid(k_sort<true>, compute_units, cap, L.grid_sort)) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<0, true>, compute_units, cap, L.grid_kshade[0])) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<1, true>, compute_units, cap, L.grid_kshade[1])) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<2, true>, compute_units, cap, L.grid_kshade[2])) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<kQueueAll, true>, compute_units, cap, L.grid_kshade[3])) != hipSuccess) return e;
     } else {
-        if ((e = resident_grid(k_sort<false>, compute_units, cap, L.grid_sort)) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<0, false>, compute_units, cap, L.grid_kshade[0])) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<1, false>, compute_units, cap, L.grid_kshade[1])) != hipSuccess) return e;
         if ((e = resident_grid(k_shade<2, false>, compute_units, cap, L.grid_kshade[2])) != hipSuccess) return e;
@@ -976,10 +971,7 @@ hipError_t wave_launch_trace(const WaveLaunch& L, int cur) {
 
 hipError_t wave_launch_sort(const WaveLaunch& L, int cur) {
     (void)hipGetLastError();
-    if (L.multi)
-        hipLaunchKernelGGL((k_sort<true>), dim3(L.grid_sort), dim3(kBlock), 0, L.stream, L.single, L.tabs, L.buf, L.cfg, cur);
-    else
-        hipLaunchKernelGGL((k_sort<false>), dim3(L.grid_sort), dim3(kBlock), 0, L.stream, L.single, L.tabs, L.buf, L.cfg, cur);
+    hipLaunchKernelGGL(k_sort, dim3(L.grid_sort), dim3(kBlock), 0, L.stream, L.tabs, L.buf, cur);
     return hipGetLastError();
 }
 

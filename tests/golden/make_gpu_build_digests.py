@@ -37,7 +37,9 @@ def make_scene(sol_amd, spec):
 
 def digest(built):
     """Digest of the built arrays. The box of an unused 4-wide slot is a marker (the inverted box
-    today, zeros when the digests were recorded), so it is hashed as zeros."""
+    today, zeros when the digests were recorded), so it is hashed as zeros; so is the kind word of
+    a triangle record (pad0: the material's kind, kept by the library for the traversal kernel,
+    zero when the digests were recorded)."""
     d = built.download()
     h = hashlib.sha256()
     for k in ("nodes", "nodes4", "tris", "shade"):
@@ -48,6 +50,9 @@ def digest(built):
                     if x.child[c] == 0x7FFFFFFF:
                         for f in ("lo_x", "hi_x", "lo_y", "hi_y", "lo_z", "hi_z"):
                             getattr(x, f)[c] = 0.0
+            elif k == "tris":
+                x = type(x).from_buffer_copy(bytes(x))
+                x.pad0 = 0
             h.update(bytes(x))
     i = d["info"]
     return {"sha256": h.hexdigest(), "num_nodes": i.num_nodes, "num_nodes4": i.num_nodes4, "depth": i.bvh_depth,

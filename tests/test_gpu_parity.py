@@ -206,6 +206,32 @@ def test_wavefront_launches_only_the_queues_the_materials_use(ctx):
         np.testing.assert_array_equal(a, b)
 
 
+def test_triangle_records_carry_their_kind(ctx):
+    # The traversal kernel packs each hit triangle's kind (BSDF + 1 in bits 0-2, light bit 3) from
+    # the fourth word of its record's second row; the hit sorting reads nothing else. That word must
+    # follow the shading record's material and light on upload, on a GPU build and after
+    # set_materials.
+    scene = sol_amd.Scene.sphere_field(num_spheres=20, subdivision=1, seed=3, extent=3.0, radius_max=1.0)
+
+    def check(g):
+        d = g.download()
+        kinds = np.array([t.pad0 for t in d["tris"]])
+        want = np.array([(sh.bsdf + 1) | (8 if sh.light >= 0 else 0) for sh in d["shade"]])
+        np.testing.assert_array_equal(kinds, want)
+        return set(kinds.tolist())
+
+    g = ctx.upload(scene)
+    seen = check(g)
+    assert len(seen) >= 3                     # diffuse, Phong or glass spheres, lights, the ground
+    check(ctx.build(scene))
+    mats = scene.materials()
+    for m in mats:
+        if m.type == F.BSDF_DIFFUSE:
+            m.type = F.BSDF_MIRROR
+    g.set_materials(mats)
+    assert check(g) != seen
+
+
 def test_wavefront_profiling_mode_matches_graph_mode(ctx, cornell):
     scene, _ = cornell
     g = ctx.upload(scene)
