@@ -383,6 +383,33 @@ struct FetchGlobal4 {
     }
 };
 
+// Per-lane scenes (batches) with the wave's LDS copy of the first nodes of one scene: a lane of
+// that scene reads its nodes below `top` from the copy (rows picked by address as in
+// FetchUniform4), every other node and every triangle through global loads.
+struct FetchMulti4 {
+    FetchGlobal4 g;
+    const float4* lds;   // the wave's copy
+    int top;             // nodes [0, top) of this lane's scene are in the copy (0: not this scene)
+
+    SOL_DEV void node(int cur, const RayPre& r, Node4& n) const {
+        if (cur < top) {
+            const uint32_t o = static_cast<uint32_t>(cur) * 128u;
+            const uint32_t ax = near_row(r.inv.x, o), ay = near_row(r.inv.y, o | 32u), az = near_row(r.inv.z, o | 64u);
+            const char* p = reinterpret_cast<const char*>(lds);
+            n.near_x = *reinterpret_cast<const float4*>(p + ax);
+            n.far_x = *reinterpret_cast<const float4*>(p + (ax ^ 16u));
+            n.near_y = *reinterpret_cast<const float4*>(p + ay);
+            n.far_y = *reinterpret_cast<const float4*>(p + (ay ^ 16u));
+            n.near_z = *reinterpret_cast<const float4*>(p + az);
+            n.far_z = *reinterpret_cast<const float4*>(p + (az ^ 16u));
+            n.child = as_int4(*reinterpret_cast<const float4*>(p + o + 96));
+        } else {
+            g.node(cur, r, n);
+        }
+    }
+    SOL_DEV void tri(uint32_t i, float4& p0, float4& p1, float4& p2) const { g.tri(i, p0, p1, p2); }
+};
+
 template <int BLOCK, int LDS_STACK>
 SOL_DEV void stack_push(const Stack& st, TraversalState& ts, int v) {
     if (ts.sp < LDS_STACK) {

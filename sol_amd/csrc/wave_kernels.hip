@@ -473,6 +473,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
         else load_top_nodes<BLOCK>(single, lds_nodes);
     }
     const Stack st = make_stack_w<BLOCK>(lds_stack, sc, LDS);
+    // Several scenes (4-wide): each wave keeps the first kWaveNodes nodes of one scene in its share
+    // of lds_nodes, the scene of its first busy lane, staged again when that changes.
+    constexpr int kWaveNodes = kTopNodes4Max * 64 / BLOCK;
+    float4* const wave_nodes = lds_nodes + (threadIdx.x / 64) * (kWaveNodes * 8);
+    uint32_t wave_scene = 0xFFFFFFFFu;   // wave-uniform
+    int wave_top = 0;
+    uint32_t lane_scene = 0;             // the scene of this lane's ray
     const unsigned waves = gridDim.x * (BLOCK / 64);
     const FetchUniform fu(single, lds_nodes);
     const FetchUniform4 fu4(single, lds_nodes);
@@ -522,7 +529,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         r = make_ray_pre(xyz(e0), xyz(e1));
                         ts.start(FLT_MAX);
                         if (MULTI) {
-                            const SceneView& sv = tabs.scenes[as_u(b.rec(slot)->w)];
+                            lane_scene = as_u(b.rec(slot)->w);
+                            const SceneView& sv = tabs.scenes[lane_scene];
                             fg = FetchGlobal(sv);
                             fg4 = FetchGlobal4(sv);
                         }
@@ -545,6 +553,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                         } else {
                             sid = as_u(b.rec(target)->w);
                         }
+                        lane_scene = sid;
                         fg = FetchGlobal(tabs.scenes[sid]);
                         fg4 = FetchGlobal4(tabs.scenes[sid]);
                     }
@@ -559,13 +568,28 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 5
                 c_next = c_end;
             }
         }
-        if (__ballot(idx >= 0) == 0) {
+        const unsigned long long busy = __ballot(idx >= 0);
+        if (busy == 0) {
             if (exhausted) break;
             continue;
         }
+        if (MULTI && WIDE) {
+            const uint32_t lead_scene = static_cast<uint32_t>(
+                __builtin_amdgcn_readlane(static_cast<int>(lane_scene), static_cast<int>(__builtin_ctzll(busy))));
+            if (lead_scene != wave_scene) {   // uniform: every lane stages its part of the copy
+                wave_scene = lead_scene;
+                const SceneView& sv = tabs.scenes[lead_scene];
+                wave_top = static_cast<int>(min(sv.num_nodes4, static_cast<uint32_t>(kWaveNodes)));
+                for (int i = __lane_id(); i < wave_top * 8; i += 64) wave_nodes[i] = sv.nodes4[i];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
         if (idx >= 0) {
             int status;
-            if (WIDE) status = MULTI ? traverse4_step<BLOCK, LDS, kLeafWaitLanes, true>(fg4, st, r, tmin, any, ts)
+            if (WIDE) status = MULTI ? traverse4_step<BLOCK, LDS, kLeafWaitLanes, true>(
+                                           FetchMulti4{fg4, wave_nodes, lane_scene == wave_scene ? wave_top : 0}, st, r, tmin, any, ts)
                                      : traverse4_step<BLOCK, LDS, kLeafWaitLanes, true>(fu4, st, r, tmin, any, ts);
             else status = MULTI ? traverse_step<BLOCK, LDS, true>(fg, st, r, tmin, any, ts)
                                 : traverse_step<BLOCK, LDS, true>(fu, st, r, tmin, any, ts);
