@@ -250,7 +250,8 @@ int solhip_scene_upload(solhip_ctx* ctx, const solhip_scene_desc* d, solhip_scen
     if (d->num_lights && !d->lights) return fail(SOLHIP_E_INVALID, "null light table");
     if (d->num_materials && !d->materials) return fail(SOLHIP_E_INVALID, "null material table");
     if (const char* e = check_materials(d->materials, d->num_materials)) return fail(SOLHIP_E_INVALID, e);
-    if (d->num_tris >= (1u << 28)) return fail(SOLHIP_E_INVALID, "too many triangles");
+    // a hit carries the triangle index below kHitKindShift bits (and the kind above)
+    if (d->num_tris >= (1u << kHitKindShift)) return fail(SOLHIP_E_INVALID, "too many triangles");
     // traversal addresses nodes and triangles with 32-bit byte offsets
     if (static_cast<uint64_t>(d->num_tris) * sizeof(sol_gpu_tri) > 0xFFFFFFFFull ||
         static_cast<uint64_t>(d->num_nodes) * sizeof(sol_gpu_node) > 0xFFFFFFFFull)
@@ -376,7 +377,7 @@ int solhip_scene_build(solhip_ctx* ctx, const solhip_mesh_desc* m, const solhip_
     if (!m->num_lights || !m->lights) return fail(SOLHIP_E_NO_LIGHTS, "scene without light");
     if (m->num_materials && !m->materials) return fail(SOLHIP_E_INVALID, "null material table");
     if (const char* e = check_materials(m->materials, m->num_materials)) return fail(SOLHIP_E_INVALID, e);
-    if (m->num_tris >= (1u << 28)) return fail(SOLHIP_E_INVALID, "too many triangles");
+    if (m->num_tris >= (1u << kHitKindShift)) return fail(SOLHIP_E_INVALID, "too many triangles");
     for (uint64_t i = 0; i < 3ull * m->num_tris; ++i)
         if (m->indices[i] >= m->num_vertices) return fail(SOLHIP_E_INVALID, "vertex index out of range");
     for (uint32_t t = 0; t < m->num_tris; ++t) {
