@@ -39,7 +39,9 @@ __global__ __launch_bounds__(BLOCK) void intersect_kernel(SceneView s, const sol
             V3 org{a.x, a.y, a.z}, dir{b.x, b.y, b.z};
             RayPre r = make_ray_pre(org, dir);
             HitRecord hit;
-            bool found = trace_ray<ANY, BLOCK, LDS, WIDE>(s, lds_nodes, st, r, a.w, b.w, hit);
+            // any-hit queries nearest child first: 1M-triangle soup 1942 -> 2080 Mrays/s, sphere field
+            // 6704 -> 6980 (the megakernel's shadow rays keep the slot order: 5 % slower this way)
+            bool found = trace_ray<ANY, BLOCK, LDS, WIDE, true>(s, lds_nodes, st, r, a.w, b.w, hit);
             if (ANY) {
                 occluded[i] = found ? 1 : 0;
             } else {

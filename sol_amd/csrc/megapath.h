@@ -28,11 +28,12 @@ SOL_DEV Stack make_stack(int* lds_stack, const TraversalScratch& sc) {
 // One sample of PathTracer::trace_path (src/algorithms/path_tracer.cpp:31-120 of sol): returns its
 // radiance. n_ext / n_shadow count the traced rays.
 // Closest / any hit of a wave-uniform scene over the binary (WIDE false) or 4-wide layout; the top
-// levels of that layout are staged in lds_nodes.
-template <bool ANY, int BLOCK, int LDS, bool WIDE>
+// levels of that layout are staged in lds_nodes. ANY_NEAR_FIRST: an any-hit query visits the hit
+// children of a 4-wide node nearest first, as a closest-hit query does, instead of in slot order.
+template <bool ANY, int BLOCK, int LDS, bool WIDE, bool ANY_NEAR_FIRST = false>
 SOL_DEV bool trace_ray(const SceneView& s, const float4* lds_nodes, const Stack& st, const RayPre& r, float tmin,
                        float tmax, HitRecord& hit) {
-    if (WIDE) return traverse4_with<ANY, BLOCK, LDS>(FetchUniform4(s, lds_nodes), st, r, tmin, tmax, hit);
+    if (WIDE) return traverse4_with<ANY, BLOCK, LDS, ANY_NEAR_FIRST>(FetchUniform4(s, lds_nodes), st, r, tmin, tmax, hit);
     return traverse<ANY, BLOCK, LDS>(s, lds_nodes, st, r, tmin, tmax, hit);
 }
 

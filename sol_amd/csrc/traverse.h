@@ -495,7 +495,10 @@ SOL_DEV void order_pair(float& ta, int& ca, float& tb, int& cb) {
 // With LEAF_WAIT > 0 the node loop also ends once no more than LEAF_WAIT lanes are still looking
 // for their first leaf while some lane holds one: those lanes keep their node and stack and go on
 // in the next step, instead of the whole wave running node iterations for a handful of them.
-template <int BLOCK, int LDS_STACK, int LEAF_WAIT = 0, bool PACK_KIND = false, typename Fetch>
+// PACK_KIND packs the hit triangle's kind into the hit (hit_slot). An any-hit query visits the hit
+// children in slot order (push_hit_children_any, no sorting) unless ANY_NEAR_FIRST.
+template <int BLOCK, int LDS_STACK, int LEAF_WAIT = 0, bool PACK_KIND = false, bool ANY_NEAR_FIRST = false,
+          typename Fetch>
 SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, float tmin, bool any, TraversalState& ts) {
     constexpr float kMiss = __builtin_huge_valf();
     int leaf = 0;          // postponed leaf (leaf references are negative), 0 = none
@@ -516,7 +519,7 @@ SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, flo
             const int hits = int(h0) + int(h1) + int(h2) + int(h3);
             if (hits == 0) {
                 more = stack_pop<BLOCK, LDS_STACK>(st, ts);
-            } else if (any && ts.sp + 3 <= LDS_STACK) {
+            } else if (!ANY_NEAR_FIRST && any && ts.sp + 3 <= LDS_STACK) {
                 ts.node = push_hit_children_any<BLOCK>(st, ts, hits, h0, h1, h2, h3, c0, c1, c2, c3);
             } else {
                 float k0 = h0 ? t0 : kMiss, k1 = h1 ? t1 : kMiss, k2 = h2 ? t2 : kMiss, k3 = h3 ? t3 : kMiss;
@@ -566,13 +569,13 @@ SOL_DEV int traverse4_step(const Fetch& f, const Stack& st, const RayPre& r, flo
 }
 
 // A whole traversal over the 4-wide layout, looping traverse4_step.
-template <bool ANY, int BLOCK, int LDS_STACK, typename Fetch>
+template <bool ANY, int BLOCK, int LDS_STACK, bool ANY_NEAR_FIRST = false, typename Fetch>
 SOL_DEV bool traverse4_with(const Fetch& f, const Stack& st, const RayPre& r, float tmin, float tmax, HitRecord& hit) {
     TraversalState ts;
     ts.start(tmax);
     int status;
     do {
-        status = traverse4_step<BLOCK, LDS_STACK>(f, st, r, tmin, ANY, ts);
+        status = traverse4_step<BLOCK, LDS_STACK, 0, false, ANY_NEAR_FIRST>(f, st, r, tmin, ANY, ts);
     } while (status == kStepContinue);
     hit = ts.hit;
     if (ANY) return status == kStepOccluded;
