@@ -374,6 +374,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--only-cornell", action="store_true",
                     help="run only the Cornell box leg (for kernel traces of a small frame) and print its object")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the 64-scene batch leg (for kernel traces of the batch) and print its object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. A frame renders on one MI355X; "
@@ -389,6 +391,9 @@ def main():
 
     if args.only_cornell:
         print(json.dumps(bench_cornell(sol_amd, F, torch, not args.no_cpu)), flush=True)
+        return
+    if args.only_batch:
+        print(json.dumps(bench_batched(sol_amd, F, torch, args.steps)), flush=True)
         return
 
     t0 = time.time()
