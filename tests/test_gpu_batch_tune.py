@@ -24,7 +24,10 @@ def ctx():
     return sol_amd.Context(0)
 
 
-def test_batch_workload_equals_single_renders(ctx):
+@pytest.mark.parametrize("block", [256, 128, 64])
+def test_batch_workload_equals_single_renders(ctx, block):
+    # The multi-scene traversal kernel splits its workgroup's LDS copy of top BVH nodes between
+    # its waves (20, 40 or 80 nodes a wave for 256-, 128- or 64-thread workgroups).
     specs = batch_specs(count=8, seed=7, max_spheres=300, sides=(64, 128, 256), spps=(1, 4))
     cfg = sol_amd.PathTracerConfig(max_path_len=BATCH_DEPTH)
     gs = [ctx.upload(sp.scene()) for sp in specs]
@@ -35,7 +38,11 @@ def test_batch_workload_equals_single_renders(ctx):
         want.append(img)
     bufs = [torch.zeros((3, sp.height, sp.width), dtype=torch.float32, device="cuda") for sp in specs]
     torch.cuda.synchronize()
-    ctx.render_batch([(g, sp.width, sp.height, 0, sp.spp, b.data_ptr()) for sp, g, b in zip(specs, gs, bufs)], cfg)
+    ctx.set_tuning(block_size=block)
+    try:
+        ctx.render_batch([(g, sp.width, sp.height, 0, sp.spp, b.data_ptr()) for sp, g, b in zip(specs, gs, bufs)], cfg)
+    finally:
+        ctx.set_tuning(block_size=0)
     for sp, b, wimg in zip(specs, bufs, want):
         got = b.cpu().numpy()
         assert got.any()
