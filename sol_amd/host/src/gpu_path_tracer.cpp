@@ -42,12 +42,18 @@ void GpuPathTracer::render(Image& image, size_t sample_index, size_t sample_coun
     cfg.max_survival_prob = config_.max_survival_prob;
     cfg.ray_offset = config_.ray_offset;
     size_t w = image.width(), h = image.height(), n = w * h;
+    // The render adds each pixel's sum of samples into a zeroed planar image, which is then added
+    // to the image's channels: image + (0 + s) is image + s bit for bit (a sum is never -0), and the
+    // image is not copied in.
     std::vector<float> planar(3 * n);
-    for (int c = 0; c < 3; ++c) std::copy(image.channel(c).get(), image.channel(c).get() + n, planar.begin() + c * n);
     check(solhip_render(gpu_scene_, &cfg, static_cast<int>(kind_), static_cast<uint32_t>(w), static_cast<uint32_t>(h),
                         sample_index, static_cast<uint32_t>(sample_count), planar.data(), 0),
           "solhip_render");
-    for (int c = 0; c < 3; ++c) std::copy(planar.begin() + c * n, planar.begin() + (c + 1) * n, image.channel(c).get());
+    for (int c = 0; c < 3; ++c) {
+        float* dst = image.channel(c).get();
+        const float* src = planar.data() + c * n;
+        for (size_t i = 0; i < n; ++i) dst[i] += src[i];
+    }
 }
 
 } // namespace sol
